@@ -1,0 +1,573 @@
+/*
+ * ycn_oracle.c -- CPU FP64 restatement of the reference hot path.
+ *
+ * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, __graft_entry__.smoke()
+ * and bench.py's cpu_baseline leg, never by the product library.
+ *
+ * Built with gcc -O3 -fopenmp -ffp-contract=off (oracle/Makefile).  All reductions
+ * use fixed 1024-element chunks summed in chunk order, so results do not depend on
+ * the OpenMP thread count.
+ */
+#include "ycn_oracle.h"
+
+#include <math.h>
+#include <stdlib.h>
+#include <string.h>
+#ifdef _OPENMP
+#include <omp.h>
+#endif
+
+#ifndef M_PI
+#define M_PI 3.14159265358979323846
+#endif
+
+typedef struct { double re, im; } cplx;
+
+static inline cplx c_add(cplx a, cplx b) { cplx c = {a.re + b.re, a.im + b.im}; return c; }
+static inline cplx c_sub(cplx a, cplx b) { cplx c = {a.re - b.re, a.im - b.im}; return c; }
+static inline cplx c_scl(double s, cplx a) { cplx c = {s * a.re, s * a.im}; return c; }
+static inline cplx c_mul(cplx a, cplx b) {
+    cplx c = {a.re * b.re - a.im * b.im, a.re * b.im + a.im * b.re};
+    return c;
+}
+static inline cplx c_conj(cplx a) { cplx c = {a.re, -a.im}; return c; }
+
+int orc_set_threads(int n) {
+#ifdef _OPENMP
+    if (n > 0) omp_set_num_threads(n);
+    return omp_get_max_threads();
+#else
+    (void)n;
+    return 1;
+#endif
+}
+
+/* ========================================================================== */
+/* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers:  */
+/* as easy as 1, 2, 3"), replacing cuRAND XORWOW (colouredNoise.cpp:31-36,     */
+/* FermionicDrift.cu:89-94).  Pinned by the Random123 known-answer vectors.    */
+/* ========================================================================== */
+void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
+    uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
+    uint32_t k0 = key[0], k1 = key[1];
+    for (int round = 0; round < 10; ++round) {
+        if (round > 0) { k0 += 0x9E3779B9u; k1 += 0xBB67AE85u; }
+        uint64_t p0 = (uint64_t)0xD2511F53u * c0;
+        uint64_t p1 = (uint64_t)0xCD9E8D57u * c2;
+        uint32_t n0 = (uint32_t)(p1 >> 32) ^ c1 ^ k0;
+        uint32_t n1 = (uint32_t)p1;
+        uint32_t n2 = (uint32_t)(p0 >> 32) ^ c3 ^ k1;
+        uint32_t n3 = (uint32_t)p0;
+        c0 = n0; c1 = n1; c2 = n2; c3 = n3;
+    }
+    out[0] = c0; out[1] = c1; out[2] = c2; out[3] = c3;
+}
+
+/* Box-Muller (Box & Muller 1958) on one Philox block: two 53-bit uniforms,
+ * u1 in (0,1], u2 in [0,1); n0 = rho cos(2 pi u2), n1 = rho sin(2 pi u2). */
+static void normal_pair(uint64_t seed, uint32_t stream, uint64_t step, uint64_t q, double *n0,
+                        double *n1) {
+    uint32_t ctr[4] = {(uint32_t)q, (uint32_t)(q >> 32), (uint32_t)step, (uint32_t)(step >> 32)};
+    uint32_t key[2] = {(uint32_t)seed, stream};
+    uint32_t o[4];
+    orc_philox4x32_10(ctr, key, o);
+    uint64_t a = (((uint64_t)o[0] << 32) | o[1]) >> 11;
+    uint64_t b = (((uint64_t)o[2] << 32) | o[3]) >> 11;
+    double u1 = (double)(a + 1) * 0x1p-53;
+    double u2 = (double)b * 0x1p-53;
+    double rho = sqrt(-2.0 * log(u1));
+    double th = (2.0 * M_PI) * u2;
+    *n0 = rho * cos(th);
+    *n1 = rho * sin(th);
+}
+
+void orc_normals(uint64_t seed, uint32_t stream, uint64_t step, int64_t n, double *out) {
+    int64_t npair = (n + 1) / 2;
+#pragma omp parallel for schedule(static)
+    for (int64_t q = 0; q < npair; ++q) {
+        double a, b;
+        normal_pair(seed, stream, step, (uint64_t)q, &a, &b);
+        out[2 * q] = a;
+        if (2 * q + 1 < n) out[2 * q + 1] = b;
+    }
+}
+
+void orc_hot_start(uint64_t initSeed, int64_t V, double *phi) {
+#pragma omp parallel for schedule(static)
+    for (int64_t i = 0; i < V; ++i) {
+        uint32_t ctr[4] = {(uint32_t)i, (uint32_t)((uint64_t)i >> 32), 0u, 0u};
+        uint32_t key[2] = {(uint32_t)initSeed, 99u};
+        uint32_t o[4];
+        orc_philox4x32_10(ctr, key, o);
+        double u = (double)((((uint64_t)o[0] << 32) | o[1]) >> 11) * 0x1p-53;
+        phi[i] = 1.0 - 2.0 * u;
+    }
+}
+
+/* ========================================================================== */
+/* FFT: recursive mixed-radix decimation in time, any length (stands in for   */
+/* cuFFT D2Z/Z2D, colouredNoise.cpp:88-94,161,172).  Pinned against numpy.fft  */
+/* and the O(n^2) DFT.                                                          */
+/* ========================================================================== */
+static int smallest_factor(int n) {
+    if (n % 2 == 0) return 2;
+    for (int p = 3; p * p <= n; p += 2)
+        if (n % p == 0) return p;
+    return n;
+}
+
+/* out[k] = sum_j in[j*stride] w^(jk); w table has n0 entries with w[j] = e^{sign 2 pi i j/n0},
+ * and n divides n0 (tw_step = n0/n). */
+static void fft_rec(int n, const cplx *in, int stride, cplx *out, const cplx *w, int n0,
+                    cplx *tmp) {
+    if (n == 1) { out[0] = in[0]; return; }
+    int p = smallest_factor(n), m = n / p;
+    for (int q = 0; q < p; ++q) fft_rec(m, in + (size_t)q * stride, stride * p, out + (size_t)q * m, w, n0, tmp);
+    int ts = n0 / n;
+    memcpy(tmp, out, sizeof(cplx) * (size_t)n);
+    for (int s = 0; s < p; ++s)
+        for (int k = 0; k < m; ++k) {
+            int kk = k + m * s;
+            cplx acc = {0.0, 0.0};
+            for (int q = 0; q < p; ++q) {
+                long e = ((long)q * kk) % n;
+                acc = c_add(acc, c_mul(w[e * ts], tmp[(size_t)q * m + k]));
+            }
+            out[kk] = acc;
+        }
+}
+
+static cplx *twiddles(int n, int sign) {
+    cplx *w = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    for (int j = 0; j < n; ++j) {
+        long double a = (long double)sign * 2.0L * 3.141592653589793238462643383279502884L * j / n;
+        w[j].re = (double)cosl(a);
+        w[j].im = (double)sinl(a);
+    }
+    return w;
+}
+
+void orc_fft_c2c(int n, const double *in, double *out, int sign) {
+    cplx *w = twiddles(n, sign < 0 ? -1 : 1);
+    cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    cplx *res = (cplx *)malloc(sizeof(cplx) * (size_t)n);
+    fft_rec(n, (const cplx *)in, 1, res, w, n, tmp);
+    memcpy(out, res, sizeof(cplx) * (size_t)n);
+    free(res); free(tmp); free(w);
+}
+
+/* 2D transform of a complex Nt x Nx array in place (rows then columns). */
+static void fft2d(int Nt, int Nx, cplx *a, int sign) {
+    cplx *wx = twiddles(Nx, sign), *wt = twiddles(Nt, sign);
+#pragma omp parallel
+    {
+        int nmax = Nt > Nx ? Nt : Nx;
+        cplx *line = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
+        cplx *res = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
+        cplx *tmp = (cplx *)malloc(sizeof(cplx) * (size_t)nmax);
+#pragma omp for schedule(static)
+        for (int t = 0; t < Nt; ++t) {
+            fft_rec(Nx, a + (size_t)t * Nx, 1, res, wx, Nx, tmp);
+            memcpy(a + (size_t)t * Nx, res, sizeof(cplx) * (size_t)Nx);
+        }
+#pragma omp for schedule(static)
+        for (int x = 0; x < Nx; ++x) {
+            for (int t = 0; t < Nt; ++t) line[t] = a[(size_t)t * Nx + x];
+            fft_rec(Nt, line, 1, res, wt, Nt, tmp);
+            for (int t = 0; t < Nt; ++t) a[(size_t)t * Nx + x] = res[t];
+        }
+        free(line); free(res); free(tmp);
+    }
+    free(wx); free(wt);
+}
+
+/* GlobalCoordinateToMomentumSquared (colouredNoise.cpp:16-29) for nDim = 2 with
+ * Sizes = {Nt, Nx}: negative marks redundant r2c modes (x index > floor(Nx/2)). */
+static double momentum_squared(int Nt, int Nx, int ct, int cx) {
+    int sizes[2] = {Nt, Nx}, coo[2] = {ct, cx};
+    double momentumSquared = 0.0, sgn = 1.0;
+    if (coo[1] > (int)floor(Nx / 2)) sgn = -1.0;
+    for (int n = 0; n < 2; ++n) {
+        int fullSize = sizes[n];
+        double tmp = (coo[n] <= (int)floor(fullSize / 2) ? coo[n] : coo[n] - fullSize) * 2.0 * M_PI / fullSize;
+        momentumSquared += tmp * tmp;
+    }
+    return momentumSquared * sgn;
+}
+
+/* mask build (colouredNoise.cpp:138-145): 1/N kept, 0 cut, half-spectrum layout. */
+void orc_mask(int Nt, int Nx, double cutFraction, double *mask) {
+    double momentumCut = sqrt(2.0) * M_PI * cutFraction;
+    double cutSq = momentumCut * momentumCut;
+    int N = Nt * Nx, arr = 0;
+    for (int pos = 0; pos < N; ++pos) {
+        int ct = pos / Nx, cx = pos % Nx;  /* IndexToGlobalCoords (colouredNoise.cpp:5-14) */
+        double momSq = momentum_squared(Nt, Nx, ct, cx);
+        if (momSq >= 0) mask[arr++] = (momSq <= cutSq ? 1.0 / N : 0.0);
+    }
+}
+
+/* xi -> D2Z -> mask -> Z2D (colouredNoise.cpp:155-178), via full complex FFTs. */
+void orc_coloured_filter(int Nt, int Nx, double cutFraction, const double *xi, double *out) {
+    int H = Nx / 2 + 1;
+    size_t V = (size_t)Nt * Nx;
+    cplx *a = (cplx *)malloc(sizeof(cplx) * V);
+    double *mask = (double *)malloc(sizeof(double) * (size_t)Nt * H);
+    orc_mask(Nt, Nx, cutFraction, mask);
+    for (size_t i = 0; i < V; ++i) { a[i].re = xi[i]; a[i].im = 0.0; }
+    fft2d(Nt, Nx, a, -1);
+    /* keep the half spectrum times the mask, rebuild the rest by Hermitian symmetry (the
+     * C2R convention: only kx <= Nx/2 is read) */
+    for (int t = 0; t < Nt; ++t)
+        for (int k = 0; k < H; ++k) a[(size_t)t * Nx + k] = c_scl(mask[(size_t)t * H + k], a[(size_t)t * Nx + k]);
+    for (int t = 0; t < Nt; ++t)
+        for (int k = H; k < Nx; ++k) a[(size_t)t * Nx + k] = c_conj(a[(size_t)((Nt - t) % Nt) * Nx + (Nx - k)]);
+    fft2d(Nt, Nx, a, +1);
+    for (size_t i = 0; i < V; ++i) out[i] = a[i].re;
+    free(a); free(mask);
+}
+
+void orc_coloured_noise(const orc_params *p, uint64_t step, double *noise) {
+    size_t V = (size_t)p->Nt * p->Nx;
+    double *xi = (double *)malloc(sizeof(double) * V);
+    orc_normals(p->seed, 0u, step, (int64_t)V, xi);
+    orc_coloured_filter(p->Nt, p->Nx, p->cutFraction, xi, noise);
+    free(xi);
+}
+
+/* ========================================================================== */
+/* Wilson-Dirac operator with Yukawa term                                      */
+/* ========================================================================== */
+static inline int site(int Nt, int Nx, int t, int x) {
+    (void)Nt;
+    return t * Nx + x;  /* vecToFlat (Lattice.cu:16-18) */
+}
+
+/* out = D in (dagger = 0, MatrixType::Normal) or D^dagger in (dagger = 1).
+ * applyD_gpu (Dirac.cu:37-47) = zero, applyDiagonal (:49-70), applyHopping (:144-190);
+ * neighbour tables as in the DiracOP constructor (Dirac.cu:10-21). */
+void orc_dirac(const orc_params *p, const double *phi, const double *in_, double *out_, int dagger) {
+    const int Nt = p->Nt, Nx = p->Nx, V = Nt * Nx;
+    const cplx *in = (const cplx *)in_;
+    cplx *out = (cplx *)out_;
+    const double r = p->r;
+    const double two = 2.0 * r, mass = p->M, g = p->g;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < V; ++i) {
+        double d = two + mass + g * phi[i];
+        for (int k = 0; k < 4; ++k) {
+            cplx o = {0.0, 0.0};
+            out[4 * i + k] = c_add(o, c_scl(d, in[4 * i + k]));
+        }
+    }
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < V; ++i) {
+        int nt = i / Nx, nx = i % Nx;   /* flatToVec (Lattice.cu:20-25) */
+        double sgn0 = (nt == (Nt - 1)) ? -1.0 : 1.0;
+        double sgn1 = (nt == 0) ? -1.0 : 1.0;
+        int IUP0 = 4 * site(Nt, Nx, (nt + 1 + Nt) % Nt, nx);
+        int IUP1 = 4 * site(Nt, Nx, nt, (nx + 1 + Nx) % Nx);
+        int IDN0 = 4 * site(Nt, Nx, (nt - 1 + Nt) % Nt, nx);
+        int IDN1 = 4 * site(Nt, Nx, nt, (nx - 1 + Nx) % Nx);
+        for (int f = 0; f < 4; f += 2) {
+            cplx a, b;
+            if (!dagger) {
+                /* Dirac.cu:166-174 */
+                a = c_sub(c_sub(c_sub(c_scl(-sgn1 * 0.5 * (r + 1.0), in[IDN0 + f]),
+                                      c_scl(sgn0 * 0.5 * (r - 1.0), in[IUP0 + f])),
+                                c_scl(0.5, c_sub(c_scl(r, in[IUP1 + f]), in[IUP1 + f + 1]))),
+                          c_scl(0.5, c_add(c_scl(r, in[IDN1 + f]), in[IDN1 + f + 1])));
+                b = c_sub(c_add(c_sub(c_scl(-sgn1 * 0.5 * (r - 1.0), in[IDN0 + f + 1]),
+                                      c_scl(sgn0 * 0.5 * (r + 1.0), in[IUP0 + f + 1])),
+                                c_scl(0.5, c_sub(in[IUP1 + f], c_scl(r, in[IUP1 + f + 1])))),
+                          c_scl(0.5, c_add(in[IDN1 + f], c_scl(r, in[IDN1 + f + 1]))));
+            } else {
+                /* Dirac.cu:177-185 */
+                a = c_sub(c_sub(c_sub(c_scl(-sgn0 * 0.5 * (r + 1.0), in[IUP0 + f]),
+                                      c_scl(sgn1 * 0.5 * (r - 1.0), in[IDN0 + f])),
+                                c_scl(0.5, c_sub(c_scl(r, in[IDN1 + f]), in[IDN1 + f + 1]))),
+                          c_scl(0.5, c_add(c_scl(r, in[IUP1 + f]), in[IUP1 + f + 1])));
+                b = c_sub(c_add(c_sub(c_scl(-sgn0 * 0.5 * (r - 1.0), in[IUP0 + f + 1]),
+                                      c_scl(sgn1 * 0.5 * (r + 1.0), in[IDN0 + f + 1])),
+                                c_scl(0.5, c_sub(in[IDN1 + f], c_scl(r, in[IDN1 + f + 1])))),
+                          c_scl(0.5, c_add(in[IUP1 + f], c_scl(r, in[IUP1 + f + 1]))));
+            }
+            out[4 * i + f] = c_add(out[4 * i + f], a);
+            out[4 * i + f + 1] = c_add(out[4 * i + f + 1], b);
+        }
+    }
+}
+
+/* solve_kernel's operator (CGsolver.cu:162-182): mtype 1 (Dagger): D then D^dag,
+ * i.e. D^dag D; mtype 0 (Normal): D^dag then D, i.e. D D^dag. */
+void orc_normal_op(const orc_params *p, const double *phi, const double *in, double *out, int mtype) {
+    size_t n = (size_t)8 * p->Nt * p->Nx;
+    double *tmp = (double *)malloc(sizeof(double) * n);
+    if (mtype == 1) {
+        orc_dirac(p, phi, in, tmp, 0);
+        orc_dirac(p, phi, tmp, out, 1);
+    } else {
+        orc_dirac(p, phi, in, tmp, 1);
+        orc_dirac(p, phi, tmp, out, 0);
+    }
+    free(tmp);
+}
+
+/* sum_i conj(a_i) b_i over n complex entries (gpuDotProduct, reductions.cu:145-174),
+ * fixed 1024-entry chunks summed in order. */
+#define CHUNK 1024
+static cplx cdot(size_t n, const cplx *a, const cplx *b) {
+    size_t nch = (n + CHUNK - 1) / CHUNK;
+    cplx *part = (cplx *)malloc(sizeof(cplx) * (nch ? nch : 1));
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < nch; ++c) {
+        cplx s = {0.0, 0.0};
+        size_t e = (c + 1) * CHUNK < n ? (c + 1) * CHUNK : n;
+        for (size_t i = c * CHUNK; i < e; ++i) s = c_add(s, c_mul(c_conj(a[i]), b[i]));
+        part[c] = s;
+    }
+    cplx s = {0.0, 0.0};
+    for (size_t c = 0; c < nch; ++c) s = c_add(s, part[c]);
+    free(part);
+    return s;
+}
+
+static double dsum(size_t n, const double *a) {
+    size_t nch = (n + CHUNK - 1) / CHUNK;
+    double *part = (double *)malloc(sizeof(double) * (nch ? nch : 1));
+#pragma omp parallel for schedule(static)
+    for (size_t c = 0; c < nch; ++c) {
+        double s = 0.0;
+        size_t e = (c + 1) * CHUNK < n ? (c + 1) * CHUNK : n;
+        for (size_t i = c * CHUNK; i < e; ++i) s += a[i];
+        part[c] = s;
+    }
+    double s = 0.0;
+    for (size_t c = 0; c < nch; ++c) s += part[c];
+    free(part);
+    return s;
+}
+
+double orc_sum(int64_t n, const double *v) { return dsum((size_t)n, v); }
+
+static double cabs_(cplx a) { return hypot(a.re, a.im); }
+
+/* complex a / b (thrust::complex division) */
+static cplx c_div(cplx a, cplx b) {
+    double den = b.re * b.re + b.im * b.im;
+    cplx c = {(a.re * b.re + a.im * b.im) / den, (a.im * b.re - a.re * b.im) / den};
+    return c;
+}
+
+/* CG on A = D^dag D (mtype 1) or D D^dag (mtype 0): solve_kernel (CGsolver.cu:125-223)
+ * with x0 = 0, r = p = b, alpha = rr / (p.Ap) (complex), beta = |r'.r'| / rr. */
+int orc_cg(const orc_params *p, const double *phi, const double *b_, double *x_, int mtype,
+           int maxIterOverride, orc_cg_stats *st) {
+    const size_t n = (size_t)4 * p->Nt * p->Nx;
+    const cplx *b = (const cplx *)b_;
+    cplx *x = (cplx *)x_;
+    cplx *r = (cplx *)malloc(sizeof(cplx) * n), *pp = (cplx *)malloc(sizeof(cplx) * n);
+    cplx *Ap = (cplx *)malloc(sizeof(cplx) * n);
+    int maxIter = maxIterOverride > 0 ? maxIterOverride : p->cgMaxIter;
+    double bnorm = sqrt(cabs_(cdot(n, b, b)));
+    double stopAbs = p->cgTolMode == 0 ? p->cgTol : p->cgTol * bnorm;
+    int k = 0, restarts = 0, converged = 0;
+    double rr = 0.0, trueRel = 0.0;
+
+    for (size_t i = 0; i < n; ++i) { x[i].re = 0.0; x[i].im = 0.0; r[i] = b[i]; pp[i] = b[i]; }
+    for (;;) {
+        rr = cabs_(cdot(n, r, r));
+        while (k < maxIter && sqrt(rr) > stopAbs) {
+            orc_normal_op(p, phi, (const double *)pp, (double *)Ap, mtype);
+            cplx dot = cdot(n, pp, Ap);
+            cplx rrc = {rr, 0.0};
+            cplx alpha = c_div(rrc, dot);
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < n; ++i) x[i] = c_add(x[i], c_mul(alpha, pp[i]));
+            cplx malpha = {-alpha.re, -alpha.im};
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < n; ++i) r[i] = c_add(r[i], c_mul(malpha, Ap[i]));
+            double rrn = cabs_(cdot(n, r, r));
+            double beta = rrn / rr;
+            rr = rrn;
+#pragma omp parallel for schedule(static)
+            for (size_t i = 0; i < n; ++i) pp[i] = c_add(r[i], c_scl(beta, pp[i]));
+            ++k;
+        }
+        converged = sqrt(rr) <= stopAbs;
+        /* true residual b - A x, recomputed in FP64 */
+        orc_normal_op(p, phi, (const double *)x, (double *)Ap, mtype);
+        for (size_t i = 0; i < n; ++i) Ap[i] = c_sub(b[i], Ap[i]);
+        double tr = sqrt(cabs_(cdot(n, Ap, Ap)));
+        trueRel = bnorm > 0 ? tr / bnorm : tr;
+        if (p->cgTolMode == 1 && trueRel > p->cgTol && k < maxIter && restarts < 4) {
+            /* reliable restart from the current x */
+            for (size_t i = 0; i < n; ++i) { r[i] = Ap[i]; pp[i] = Ap[i]; }
+            ++restarts;
+            continue;
+        }
+        if (p->cgTolMode == 1) converged = trueRel <= p->cgTol;
+        break;
+    }
+    if (st) { st->iters = k; st->restarts = restarts; st->converged = converged; st->rr = rr; st->true_rel_resid = trueRel; }
+    free(r); free(pp); free(Ap);
+    return converged ? 0 : 1;
+}
+
+/* ========================================================================== */
+/* fermion force                                                               */
+/* ========================================================================== */
+void orc_fermion_noise(uint64_t seed, int j, uint64_t step, int64_t V, double *eta) {
+    /* fillNormalRND (FermionicDrift.cu:141-149): eta_i = (N(0,1), 0) for i < 4V */
+    double *n = (double *)malloc(sizeof(double) * (size_t)(4 * V));
+    orc_normals(seed, 1u + (uint32_t)j, step, 4 * V, n);
+    for (int64_t i = 0; i < 4 * V; ++i) { eta[2 * i] = n[i]; eta[2 * i + 1] = 0.0; }
+    free(n);
+}
+
+/* getForce (FermionicDrift.cu:96-119) + computeDrift (:122-139); with nNoise > 1 the
+ * drift is the mean of the nNoise estimators. */
+int orc_fermion_drift_eta(const orc_params *p, const double *phi, const double *eta_,
+                          double *drift, orc_cg_stats *st) {
+    const int V = p->Nt * p->Nx;
+    const size_t n = (size_t)4 * V;
+    double *y = (double *)malloc(sizeof(double) * 2 * n), *chi = (double *)malloc(sizeof(double) * 2 * n);
+    int fails = 0;
+    orc_cg_stats acc = {0, 0, 1, 0.0, 0.0};
+    double fac = -p->g * (p->driftMode == 1 ? p->cutFraction * p->cutFraction : 1.0);
+    for (int i = 0; i < V; ++i) drift[i] = 0.0;
+    for (int j = 0; j < p->nNoise; ++j) {
+        const cplx *eta = (const cplx *)(eta_ + (size_t)j * 2 * n);
+        orc_cg_stats s;
+        fails += orc_cg(p, phi, (const double *)eta, y, 1, 0, &s);
+        acc.iters += s.iters; acc.restarts += s.restarts; acc.converged &= s.converged;
+        if (s.true_rel_resid > acc.true_rel_resid) acc.true_rel_resid = s.true_rel_resid;
+        acc.rr = s.rr;
+        orc_dirac(p, phi, y, chi, 0);
+        const cplx *c = (const cplx *)chi;
+#pragma omp parallel for schedule(static)
+        for (int i = 0; i < V; ++i) {
+            cplx s4 = c_mul(c_conj(c[4 * i + 0]), eta[4 * i + 0]);
+            s4 = c_add(s4, c_mul(c_conj(c[4 * i + 1]), eta[4 * i + 1]));
+            s4 = c_add(s4, c_mul(c_conj(c[4 * i + 2]), eta[4 * i + 2]));
+            s4 = c_add(s4, c_mul(c_conj(c[4 * i + 3]), eta[4 * i + 3]));
+            double v = fac * s4.re;
+            if (p->nNoise == 1) drift[i] = v; else drift[i] += v;
+        }
+    }
+    if (p->nNoise > 1)
+        for (int i = 0; i < V; ++i) drift[i] /= (double)p->nNoise;
+    if (st) *st = acc;
+    free(y); free(chi);
+    return fails;
+}
+
+int orc_fermion_drift(const orc_params *p, const double *phi, uint64_t step, double *drift,
+                      orc_cg_stats *st) {
+    const int64_t V = (int64_t)p->Nt * p->Nx;
+    double *eta = (double *)malloc(sizeof(double) * (size_t)(8 * V * p->nNoise));
+    for (int j = 0; j < p->nNoise; ++j) orc_fermion_noise(p->seed, j, step, V, eta + (size_t)j * 8 * V);
+    int rc = orc_fermion_drift_eta(p, phi, eta, drift, st);
+    free(eta);
+    return rc;
+}
+
+/* ========================================================================== */
+/* bosonic drift and Euler-Maruyama update                                     */
+/* ========================================================================== */
+/* Laplace::func2 (Laplace.cu:61-100): 5 sorted columns per row, +1 off-diagonal,
+ * -2*nDim on the diagonal. */
+void orc_laplace_csr(int Nt, int Nx, int *I, int *J, double *val) {
+    int N = Nt * Nx;
+    I[0] = 0;
+    for (int r = 0; r < N; ++r) {
+        int t = r / Nx, x = r % Nx, c[5];
+        c[0] = r;
+        c[1] = ((t + 1) % Nt) * Nx + x;
+        c[2] = ((t - 1 + Nt) % Nt) * Nx + x;
+        c[3] = t * Nx + (x + 1) % Nx;
+        c[4] = t * Nx + (x - 1 + Nx) % Nx;
+        for (int a = 1; a < 5; ++a)          /* insertion sort = std::sort result */
+            for (int b = a; b > 0 && c[b - 1] > c[b]; --b) { int tmp = c[b]; c[b] = c[b - 1]; c[b - 1] = tmp; }
+        for (int a = 0; a < 5; ++a) {
+            J[5 * r + a] = c[a];
+            val[5 * r + a] = (c[a] != r ? 1.0 : -4.0);
+        }
+        I[r + 1] = 5 * (r + 1);
+    }
+}
+
+/* Run (langevin_gpu_v2.cu:65-117): PotentialAndMass (:9-27), gpuSpMV (:41-63),
+ * gpuMaxAbsReduce + clamp (:77-91, reductions.cu:30-63), Evolve (:29-37). */
+double orc_bosonic_update(const orc_params *p, double *phi, double *drift, const double *noise,
+                          double *eps) {
+    const int Nt = p->Nt, Nx = p->Nx, N = Nt * Nx;
+    int *I = (int *)malloc(sizeof(int) * (size_t)(N + 1)), *J = (int *)malloc(sizeof(int) * (size_t)5 * N);
+    double *val = (double *)malloc(sizeof(double) * (size_t)5 * N);
+    orc_laplace_csr(Nt, Nx, I, J, val);
+    const double myLambda = p->lambda / 6.0;
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) {
+        double interaction = 0.0;
+        double tmp = phi[i];
+        interaction += tmp * tmp;
+        drift[i] += phi[i] * (p->m2 + myLambda * interaction);
+        double output = 0.0;
+        for (int j = 0; j < 5; ++j) output += val[5 * i + j] * phi[J[5 * i + j]];
+        drift[i] -= output;
+    }
+    double mx = 0.0;
+    for (int i = 0; i < N; ++i) { double a = drift[i] > 0 ? drift[i] : -drift[i]; mx = a > mx ? a : mx; }
+    if (mx > 1e5) mx = 1e5;
+    else if (mx < 1e-5) mx = 1e-5;
+    double e = p->adaptive ? p->epsBar * 30.0 / mx : p->epsBar;
+    const double fact = sqrt(2.0 * e);
+#pragma omp parallel for schedule(static)
+    for (int i = 0; i < N; ++i) phi[i] -= e * drift[i] + fact * noise[i];
+    *eps = e;
+    free(I); free(J); free(val);
+    return mx;
+}
+
+/* main.cu:305-315: cn(); fDrift.getForce(drift); kli.Run(kAll) */
+int orc_langevin_step(const orc_params *p, double *phi, uint64_t step, double *eps,
+                      orc_cg_stats *st) {
+    size_t V = (size_t)p->Nt * p->Nx;
+    double *noise = (double *)malloc(sizeof(double) * V), *drift = (double *)malloc(sizeof(double) * V);
+    orc_coloured_noise(p, step, noise);
+    int rc = orc_fermion_drift(p, phi, step, drift, st);
+    orc_bosonic_update(p, phi, drift, noise, eps);
+    free(noise); free(drift);
+    return rc;
+}
+
+/* ========================================================================== */
+/* measurement                                                                 */
+/* ========================================================================== */
+/* gpuTimeSlices (reductions.cu:65-104): sum over x for each t */
+void orc_time_slices(int Nt, int Nx, const double *phi, double *slices) {
+    for (int t = 0; t < Nt; ++t) {
+        double s = 0.0;
+        for (int x = 0; x < Nx; ++x) s += phi[(size_t)t * Nx + x];
+        slices[t] = s;
+    }
+}
+
+/* main.cu:441-469: source in = delta_{x,0} (1,1,0,0); CG on DD^dag (Normal); then
+ * applyD(out, in, Dagger); gpuTimeSlices_spinors (reductions.cu:176-191). */
+int orc_correlator(const orc_params *p, const double *phi, double *corr, orc_cg_stats *st) {
+    const int Nt = p->Nt, Nx = p->Nx;
+    const size_t n = (size_t)8 * Nt * Nx;
+    double *in = (double *)calloc(n, sizeof(double)), *out = (double *)calloc(n, sizeof(double));
+    in[0] = 1.0; in[2] = 1.0;
+    int rc = orc_cg(p, phi, in, out, 0, 0, st);
+    orc_dirac(p, phi, out, in, 1);
+    for (int t = 0; t < Nt; ++t) {
+        double s = 0.0;
+        for (int x = 0; x < Nx; ++x)
+            for (int k = 0; k < 4; ++k) s += in[2 * (4 * ((size_t)t * Nx + x) + k)];
+        corr[t] = s;
+    }
+    free(in); free(out);
+    return rc;
+}

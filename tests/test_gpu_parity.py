@@ -1,0 +1,212 @@
+"""HIP path (libycn.so through the C-ABI) against the CPU FP64 oracle on identical inputs.
+
+Tolerance: 1e-12 relative (max-abs error / max-abs value) for every field, the bar the
+hot path is held to; CG iteration counts must agree to within one iteration (different
+but fixed reduction orders can move the stopping iteration across the threshold).
+"""
+import numpy as np
+import pytest
+
+import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+RTOL = 1e-12
+
+
+def rel(a, b):
+    a = np.asarray(a)
+    b = np.asarray(b)
+    return float(np.max(np.abs(a - b)) / max(np.max(np.abs(b)), 1e-300))
+
+
+def mk(Y, **kw):
+    base = dict(Nt=16, Nx=16, m2=-0.2, lam=0.5, g=0.3, M=0.3, r=1.0, cutFraction=0.5, driftMode=0,
+                epsBar=0.01, adaptive=0, seed=1234, cgTol=1e-10, cgTolMode=1, cgMaxIter=10000)
+    base.update(kw)
+    ycn = {k: v for k, v in base.items()}
+    p = Y.default_params(**ycn)
+    fields = [f for f, _ in O.OrcParams._fields_]
+    orc = O.params(**{k: v for k, v in base.items() if k in fields})
+    return p, orc
+
+
+SIZES = [(16, 16), (32, 32), (6, 32), (2, 32), (10, 12), (22, 32), (64, 48)]
+
+
+@pytest.mark.parametrize("Nt,Nx", SIZES)
+@pytest.mark.parametrize("s", [1.0, 0.5, 0.25])
+def test_coloured_filter(gpu, Nt, Nx, s):
+    Y = gpu
+    p, _ = mk(Y, Nt=Nt, Nx=Nx, cutFraction=s)
+    xi = np.random.default_rng(Nt * 100 + Nx).standard_normal((Nt, Nx))
+    with Y.Context(p) as c:
+        out = c.coloured_filter(xi)
+    ref = O.coloured_filter(xi, s)
+    assert rel(out, ref) < RTOL
+    if s == 1.0:
+        assert rel(out, xi) < 1e-13  # white-noise identity: every mode is kept
+
+
+@pytest.mark.parametrize("Nt,Nx", [(32, 32), (6, 32), (64, 64)])
+def test_coloured_noise_philox(gpu, Nt, Nx):
+    Y = gpu
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, seed=1234)
+    with Y.Context(p) as c:
+        for step in (0, 7, 123456789012):
+            out = c.coloured_noise(step)
+            assert rel(out, O.coloured_noise(orc, step)) < RTOL
+
+
+@pytest.mark.parametrize("Nt,Nx", [(16, 16), (6, 10), (2, 32), (64, 64)])
+@pytest.mark.parametrize("dagger", [False, True])
+def test_dirac(gpu, Nt, Nx, dagger):
+    Y = gpu
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.05, g=0.5)
+    rng = np.random.default_rng(5)
+    phi = rng.uniform(-1, 1, (Nt, Nx))
+    psi = rng.standard_normal((Nt * Nx, 4)) + 1j * rng.standard_normal((Nt * Nx, 4))
+    with Y.Context(p) as c:
+        out = c.dirac_apply(psi, phi=phi, dagger=dagger)
+    assert rel(out, O.dirac(orc, phi, psi, dagger)) < RTOL
+
+
+@pytest.mark.parametrize("mtype", [0, 1])
+@pytest.mark.parametrize("tolmode", [0, 1])
+def test_cg(gpu, mtype, tolmode):
+    Y = gpu
+    Nt, Nx = 16, 24
+    tol = 1e-8 if tolmode == 0 else 1e-10
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.1, g=0.5, cgTol=tol, cgTolMode=tolmode, cgMaxIter=1000)
+    rng = np.random.default_rng(11)
+    phi = rng.uniform(-1, 1, (Nt, Nx))
+    b = rng.standard_normal((Nt * Nx, 4)) + 1j * rng.standard_normal((Nt * Nx, 4))
+    with Y.Context(p) as c:
+        x, st = c.cg_solve(b, phi=phi, mtype=mtype)
+    xr, sr = O.cg(orc, phi, b, mtype)
+    assert abs(st["iters"] - sr["iters"]) <= 1, (st, sr)
+    assert st["converged"] == 1
+    assert rel(x, xr) < 1e-9  # solution agrees to the solver tolerance
+    if tolmode == 1:
+        assert st["true_rel_resid"] <= 1e-10
+
+
+def test_cg_fixed_iterations(gpu):
+    """Same recurrence: with the iteration count fixed both sides give the same iterate."""
+    Y = gpu
+    Nt, Nx = 16, 16
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.1, g=0.5, cgTol=0.0, cgTolMode=0, cgMaxIter=37)
+    rng = np.random.default_rng(3)
+    phi = rng.uniform(-1, 1, (Nt, Nx))
+    b = rng.standard_normal((Nt * Nx, 4)) + 0j
+    with Y.Context(p) as c:
+        x, st = c.cg_solve(b, phi=phi, mtype=1)
+    xr, sr = O.cg(orc, phi, b, 1)
+    assert st["iters"] == sr["iters"] == 37
+    assert rel(x, xr) < RTOL
+
+
+@pytest.mark.parametrize("driftMode", [0, 1])
+def test_fermion_force_fixed_iterations(gpu, driftMode):
+    """Same eta, same recurrence run for a fixed number of CG iterations: 1e-12."""
+    Y = gpu
+    Nt, Nx = 16, 16
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, driftMode=driftMode, seed=77, cgTol=0.0, cgTolMode=0, cgMaxIter=60)
+    phi = O.hot_start(7, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        d, st = c.fermion_force(5)
+    dr, sr = O.fermion_drift(orc, phi, 5)
+    assert st["iters"] == sr["iters"] == 60
+    assert rel(d, dr) < RTOL
+
+
+def test_fermion_force_converged(gpu):
+    """Both solves stop at true residual <= 1e-10; the drift is then fixed only to the
+    solver's accuracy (kappa * tol), so the bar here is 1e-9 relative."""
+    Y = gpu
+    Nt, Nx = 16, 16
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, seed=77)
+    phi = O.hot_start(7, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        d, st = c.fermion_force(5)
+    dr, sr = O.fermion_drift(orc, phi, 5)
+    assert abs(st["iters"] - sr["iters"]) <= 1
+    assert st["true_rel_resid"] <= 1e-10
+    assert rel(d, dr) < 1e-9
+
+
+def test_hot_start(gpu):
+    Y = gpu
+    p, _ = mk(Y, Nt=32, Nx=64)
+    with Y.Context(p) as c:
+        c.hot_start(7)
+        phi = c.get_field()
+    assert np.array_equal(phi, O.hot_start(7, 32, 64))
+
+
+@pytest.mark.parametrize("adaptive", [0, 1])
+@pytest.mark.parametrize("Nt,Nx", [(16, 16), (32, 32), (6, 32)])
+def test_langevin_step(gpu, adaptive, Nt, Nx):
+    Y = gpu
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, adaptive=adaptive, epsBar=0.01)
+    phi0 = O.hot_start(7, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi0)
+        phi = phi0.copy()
+        for step in range(3):
+            r = c.langevin_step(step)
+            phi, eps, st = O.langevin_step(orc, phi, step)
+            g = c.get_field()
+            assert rel(g, phi) < RTOL, step
+            assert abs(r.eps[0] - eps) <= 1e-13 * eps
+
+
+def test_batched_replicas(gpu):
+    Y = gpu
+    Nt, Nx, R = 16, 16, 4
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, nReplicas=R, adaptive=0)
+    m2 = np.linspace(-1.0, 0.5, R)
+    seeds = np.arange(R, dtype=np.uint64)
+    phi0 = np.stack([O.hot_start(10 + k, Nt, Nx) for k in range(R)])
+    with Y.Context(p) as c:
+        c.set_replicas(m2, seeds)
+        c.set_field(phi0)
+        c.langevin_step(0)
+        c.langevin_step(1)
+        g = c.get_field()
+    for k in range(R):
+        o = O.params(Nt=Nt, Nx=Nx, m2=m2[k], lam=0.5, g=0.3, M=0.3, cutFraction=0.5, epsBar=0.01,
+                     adaptive=0, seed=int(seeds[k]), cgTol=1e-10, cgTolMode=1, cgMaxIter=10000)
+        ph, _, _ = O.langevin_step(o, phi0[k], 0)
+        ph, _, _ = O.langevin_step(o, ph, 1)
+        assert rel(g[k], ph) < RTOL, k
+
+
+def test_multi_rhs_noise(gpu):
+    Y = gpu
+    Nt, Nx = 16, 16
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, nNoise=3, cgTol=0.0, cgTolMode=0, cgMaxIter=50)
+    phi = O.hot_start(3, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        d, st = c.fermion_force(9)
+    dr, _ = O.fermion_drift(orc, phi, 9)
+    assert st["systems"] == 3
+    assert rel(d, dr) < RTOL
+
+
+def test_measure(gpu):
+    Y = gpu
+    Nt, Nx = 16, 16
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.5, g=0.3)
+    phi = O.hot_start(4, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        ob, sl, cr = c.measure(0, 3)
+    assert rel(sl, O.time_slices(phi)) < RTOL
+    assert abs(ob["magnetisation"] - phi.sum()) < 1e-12 * np.abs(phi).sum()
+    assert abs(ob["phi2"] - (phi ** 2).sum()) < 1e-12 * (phi ** 2).sum()
+    cref, _ = O.correlator(orc, phi)
+    assert rel(cr, cref) < 1e-9

@@ -1,0 +1,469 @@
+// ycn_abi.hip -- the C-ABI of libycn (include/ycn.h): context, buffers, stream,
+// profiling and the step sequence of code/src/main.cu:305-315.
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstring>
+#include <new>
+#include <string>
+#include <vector>
+
+#include "ycn_internal.hpp"
+
+using namespace ycn;
+
+struct ycn_ctx : public Ctx {};
+
+namespace ycn {
+
+void prof_before(Ctx &c, int k, double bytes) {
+    if (c.profTarget != k) return;
+    if (c.evCount >= (int)c.evStart.size()) return;
+    c.profBytes = bytes;
+    (void)hipEventRecord(c.evStart[c.evCount], c.stream);
+}
+void prof_after(Ctx &c, int k) {
+    if (c.profTarget != k) return;
+    if (c.evCount >= (int)c.evStart.size()) return;
+    (void)hipEventRecord(c.evStop[c.evCount], c.stream);
+    ++c.evCount;
+}
+
+}  // namespace ycn
+
+#define CHECK_CTX(ctx)                                  \
+    do {                                                \
+        if (!(ctx)) return YCN_ERR_INVALID;             \
+    } while (0)
+
+static int fail(Ctx &c, int code, const std::string &msg) {
+    c.err = msg;
+    return code;
+}
+
+// Copy a host array into a temporary device buffer owned by the caller of this helper.
+struct DevTmp {
+    void *p = nullptr;
+    ~DevTmp() {
+        if (p) (void)hipFree(p);
+    }
+};
+
+extern "C" {
+
+int ycn_abi_version(void) { return YCN_ABI_VERSION; }
+
+void ycn_default_params(ycn_params *p) {
+    if (!p) return;
+    std::memset(p, 0, sizeof(*p));
+    p->Nt = 16;             // code/src/params.h:29
+    p->Nx = 16;
+    p->nReplicas = 1;
+    p->nNoise = 1;
+    p->m2 = 0.0;
+    p->lambda = 0.0;
+    p->g = 0.0;
+    p->M = 1.0;             // code/input.toml:26
+    p->r = 1.0;             // code/input.toml:28
+    p->cutFraction = 1.0;   // code/input.toml:7
+    p->driftMode = 0;
+    p->epsBar = 0.02;       // code/input.toml:10
+    p->adaptive = 1;        // langevin_gpu_v2.cu:277
+    p->seed = 1234;         // code/input.toml:22
+    p->cgTol = 1e-8;        // params.h:14
+    p->cgTolMode = YCN_TOL_ABS_RECURSIVE;
+    p->cgMaxIter = 1000;    // params.h:15
+    p->cgPrecision = YCN_CG_FP64;
+    p->reliableEvery = 50;
+    p->precond = YCN_PRECOND_NONE;
+    p->device = 0;
+}
+
+const char *ycn_last_error(const ycn_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+
+void ycn_destroy(ycn_ctx *ctx) {
+    if (!ctx) return;
+    Ctx &c = *ctx;
+    (void)hipSetDevice(c.p.device);
+    if (c.stream) (void)hipStreamSynchronize(c.stream);
+    noise_free(c);
+    cg_free(c);
+    double **bufs[] = {&c.phi[0], &c.phi[1], &c.noise, &c.drift, &c.m2_d};
+    for (double **b : bufs)
+        if (*b) (void)hipFree(*b);
+    if (c.seed_d) (void)hipFree(c.seed_d);
+    if (c.step_d) (void)hipFree(c.step_d);
+    for (auto e : c.evStart) (void)hipEventDestroy(e);
+    for (auto e : c.evStop) (void)hipEventDestroy(e);
+    if (c.stream) (void)hipStreamDestroy(c.stream);
+    delete ctx;
+}
+
+int ycn_create(const ycn_params *p, ycn_ctx **out) {
+    if (!p || !out) return YCN_ERR_INVALID;
+    *out = nullptr;
+    if (p->Nt < 2 || p->Nx < 2 || p->nReplicas < 1 || p->nNoise < 1 || p->cgMaxIter < 0 || p->cgTol < 0 ||
+        (p->cgTolMode != YCN_TOL_ABS_RECURSIVE && p->cgTolMode != YCN_TOL_REL_TRUE) ||
+        (p->driftMode != 0 && p->driftMode != 1))
+        return YCN_ERR_INVALID;
+    if ((int64_t)p->Nt * p->Nx > (int64_t)1 << 28) return YCN_ERR_INVALID;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return YCN_ERR_NO_DEVICE;
+    if (p->device < 0 || p->device >= ndev) return YCN_ERR_INVALID;
+    ycn_ctx *ctx = new (std::nothrow) ycn_ctx();
+    if (!ctx) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    c.p = *p;
+    c.V = p->Nt * p->Nx;
+    c.H = p->Nx / 2 + 1;
+    c.nRep = p->nReplicas;
+    c.nNoise = p->nNoise;
+    c.nSys = c.nRep * c.nNoise;
+    c.epsBar = p->epsBar;
+    c.m2.assign(c.nRep, p->m2);
+    c.seeds.assign(c.nRep, p->seed);
+    int rc = YCN_OK;
+    auto bail = [&](int code) {
+        *out = nullptr;
+        std::string e = c.err;
+        ycn_destroy(ctx);
+        return code;
+    };
+    if (hipSetDevice(p->device) != hipSuccess) return bail(YCN_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return bail(YCN_ERR_HIP);
+    const size_t fb = sizeof(double) * (size_t)c.nRep * c.V;
+    double **bufs[] = {&c.phi[0], &c.phi[1], &c.noise, &c.drift};
+    for (double **b : bufs)
+        if (hipMalloc(b, fb) != hipSuccess) return bail(YCN_ERR_HIP);
+    if (hipMemset(c.phi[0], 0, fb) != hipSuccess || hipMemset(c.phi[1], 0, fb) != hipSuccess)
+        return bail(YCN_ERR_HIP);
+    if (hipMalloc(&c.m2_d, sizeof(double) * c.nRep) != hipSuccess) return bail(YCN_ERR_HIP);
+    if (hipMalloc(&c.seed_d, sizeof(uint64_t) * c.nRep) != hipSuccess) return bail(YCN_ERR_HIP);
+    if (hipMalloc(&c.step_d, sizeof(StepState) * c.nRep) != hipSuccess) return bail(YCN_ERR_HIP);
+    if (hipMemset(c.step_d, 0, sizeof(StepState) * c.nRep) != hipSuccess) return bail(YCN_ERR_HIP);
+    if (hipMemcpy(c.m2_d, c.m2.data(), sizeof(double) * c.nRep, hipMemcpyHostToDevice) != hipSuccess ||
+        hipMemcpy(c.seed_d, c.seeds.data(), sizeof(uint64_t) * c.nRep, hipMemcpyHostToDevice) != hipSuccess)
+        return bail(YCN_ERR_HIP);
+    if ((rc = noise_init(c))) return bail(rc);
+    if ((rc = cg_alloc(c))) return bail(rc);
+    c.evStart.resize(8192);
+    c.evStop.resize(8192);
+    for (size_t i = 0; i < c.evStart.size(); ++i)
+        if (hipEventCreate(&c.evStart[i]) != hipSuccess || hipEventCreate(&c.evStop[i]) != hipSuccess)
+            return bail(YCN_ERR_HIP);
+    if (hipDeviceSynchronize() != hipSuccess) return bail(YCN_ERR_HIP);
+    *out = ctx;
+    return YCN_OK;
+}
+
+int ycn_set_replicas(ycn_ctx *ctx, const double *m2, const uint64_t *seeds) {
+    CHECK_CTX(ctx);
+    Ctx &c = *ctx;
+    if (m2) c.m2.assign(m2, m2 + c.nRep);
+    if (seeds) c.seeds.assign(seeds, seeds + c.nRep);
+    YCN_HIP(hipMemcpyAsync(c.m2_d, c.m2.data(), sizeof(double) * c.nRep, hipMemcpyHostToDevice, c.stream));
+    YCN_HIP(hipMemcpyAsync(c.seed_d, c.seeds.data(), sizeof(uint64_t) * c.nRep, hipMemcpyHostToDevice, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int ycn_set_eps_bar(ycn_ctx *ctx, double epsBar) {
+    CHECK_CTX(ctx);
+    if (!(epsBar > 0)) return YCN_ERR_INVALID;
+    ctx->epsBar = epsBar;
+    return YCN_OK;
+}
+
+int ycn_set_field(ycn_ctx *ctx, const double *phi) {
+    CHECK_CTX(ctx);
+    if (!phi) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    YCN_HIP(hipMemcpyAsync(c.phi[c.cur], phi, sizeof(double) * (size_t)c.nRep * c.V, hipMemcpyHostToDevice, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int ycn_get_field(ycn_ctx *ctx, double *phi) {
+    CHECK_CTX(ctx);
+    if (!phi) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    YCN_HIP(hipMemcpyAsync(phi, c.phi[c.cur], sizeof(double) * (size_t)c.nRep * c.V, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int ycn_hot_start(ycn_ctx *ctx, uint64_t initSeed) {
+    CHECK_CTX(ctx);
+    Ctx &c = *ctx;
+    int rc = hot_start(c, initSeed);
+    if (rc) return rc;
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int ycn_coloured_noise(ycn_ctx *ctx, uint64_t step, double *noise) {
+    CHECK_CTX(ctx);
+    Ctx &c = *ctx;
+    int rc = noise_generate(c, step, nullptr, c.noise);
+    if (rc) return rc;
+    if (noise)
+        YCN_HIP(hipMemcpyAsync(noise, c.noise, sizeof(double) * (size_t)c.nRep * c.V, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int ycn_coloured_filter(ycn_ctx *ctx, const double *xi, double *out) {
+    CHECK_CTX(ctx);
+    if (!xi || !out) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    const size_t nb = sizeof(double) * (size_t)c.nRep * c.V;
+    DevTmp d;
+    YCN_HIP(hipMalloc(&d.p, nb));
+    YCN_HIP(hipMemcpyAsync(d.p, xi, nb, hipMemcpyHostToDevice, c.stream));
+    int rc = noise_generate(c, 0, (const double *)d.p, c.noise);
+    if (rc) return rc;
+    YCN_HIP(hipMemcpyAsync(out, c.noise, nb, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+// phi for single-system entry points: replica 0 of the context or a host array
+static int stage_phi(Ctx &c, const double *phi_host, DevTmp &tmp, const double **phi_dev) {
+    if (!phi_host) {
+        *phi_dev = c.phi[c.cur];
+        return YCN_OK;
+    }
+    YCN_HIP(hipMalloc(&tmp.p, sizeof(double) * c.V));
+    YCN_HIP(hipMemcpyAsync(tmp.p, phi_host, sizeof(double) * c.V, hipMemcpyHostToDevice, c.stream));
+    *phi_dev = (const double *)tmp.p;
+    return YCN_OK;
+}
+
+int ycn_dirac_apply(ycn_ctx *ctx, const double *phi, const double *in, double *out, int mtype) {
+    CHECK_CTX(ctx);
+    if (!in || !out || (mtype != YCN_NORMAL && mtype != YCN_DAGGER)) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    const size_t nb = sizeof(double2) * 4 * (size_t)c.V;
+    DevTmp ph, aos;
+    const double *phi_dev = nullptr;
+    int rc = stage_phi(c, phi, ph, &phi_dev);
+    if (rc) return rc;
+    YCN_HIP(hipMalloc(&aos.p, nb));
+    YCN_HIP(hipMemcpyAsync(aos.p, in, nb, hipMemcpyHostToDevice, c.stream));
+    if ((rc = aos_to_soa(c, (const double *)aos.p, c.t, 1))) return rc;
+    const int save = c.nNoise;
+    c.nNoise = 1;  // one system, replica-0 field
+    rc = dirac_apply(c, phi_dev, c.t, c.Ap, 1, mtype == YCN_DAGGER);
+    c.nNoise = save;
+    if (rc) return rc;
+    if ((rc = soa_to_aos(c, c.Ap, (double *)aos.p, 1))) return rc;
+    YCN_HIP(hipMemcpyAsync(out, aos.p, nb, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+static void fill_stats(const SolveStats &s, ycn_cg_stats *o) {
+    if (!o) return;
+    o->iters = s.iters_max;
+    o->restarts = s.restarts;
+    o->converged = s.converged;
+    o->systems = s.systems;
+    o->rr = s.rr_max;
+    o->true_rel_resid = s.true_rel_max;
+    o->iters_mean = s.iters_mean;
+}
+
+int ycn_cg_solve(ycn_ctx *ctx, const double *phi, const double *b, double *x, int mtype, ycn_cg_stats *stats) {
+    CHECK_CTX(ctx);
+    if (!b || !x || (mtype != YCN_NORMAL && mtype != YCN_DAGGER)) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    const size_t nb = sizeof(double2) * 4 * (size_t)c.V;
+    DevTmp ph, aos;
+    const double *phi_dev = nullptr;
+    int rc = stage_phi(c, phi, ph, &phi_dev);
+    if (rc) return rc;
+    YCN_HIP(hipMalloc(&aos.p, nb));
+    YCN_HIP(hipMemcpyAsync(aos.p, b, nb, hipMemcpyHostToDevice, c.stream));
+    if ((rc = aos_to_soa(c, (const double *)aos.p, c.eta, 1))) return rc;
+    const int save = c.nNoise;
+    c.nNoise = 1;
+    SolveStats st;
+    rc = cg_solve(c, phi_dev, 1, mtype, false, 0, 0, &st);
+    c.nNoise = save;
+    if (rc) return rc;
+    if ((rc = soa_to_aos(c, c.x, (double *)aos.p, 1))) return rc;
+    YCN_HIP(hipMemcpyAsync(x, aos.p, nb, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    fill_stats(st, stats);
+    return st.converged ? YCN_OK : YCN_ERR_NOT_CONVERGED;
+}
+
+static int force_impl(Ctx &c, uint64_t step, uint32_t streamBase, SolveStats *st) {
+    int rc = cg_solve(c, c.phi[c.cur], c.nSys, YCN_DAGGER, true, streamBase, step, st);
+    if (rc) return rc;
+    return fermion_drift(c, c.phi[c.cur], c.drift);
+}
+
+int ycn_fermion_force(ycn_ctx *ctx, uint64_t step, double *drift, ycn_cg_stats *stats) {
+    CHECK_CTX(ctx);
+    Ctx &c = *ctx;
+    SolveStats st;
+    int rc = force_impl(c, step, 1u, &st);
+    if (rc) return rc;
+    if (drift)
+        YCN_HIP(hipMemcpyAsync(drift, c.drift, sizeof(double) * (size_t)c.nRep * c.V, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    fill_stats(st, stats);
+    return st.converged ? YCN_OK : YCN_ERR_NOT_CONVERGED;
+}
+
+static int step_impl(Ctx &c, uint64_t step, SolveStats *st) {
+    int rc = noise_generate(c, step, nullptr, c.noise);     // cn()
+    if (rc) return rc;
+    rc = force_impl(c, step, 1u, st);                        // fDrift.getForce(drift)
+    if (rc) return rc;
+    return bosonic_update(c, c.p.adaptive ? 0 : 1);          // kli.Run(kAll)
+}
+
+int ycn_langevin_step(ycn_ctx *ctx, uint64_t step, double *eps, ycn_cg_stats *stats) {
+    CHECK_CTX(ctx);
+    Ctx &c = *ctx;
+    SolveStats st;
+    int rc = step_impl(c, step, &st);
+    if (rc) return rc;
+    if (eps) {
+        std::vector<StepState> S(c.nRep);
+        YCN_HIP(hipMemcpyAsync(S.data(), c.step_d, sizeof(StepState) * c.nRep, hipMemcpyDeviceToHost, c.stream));
+        YCN_HIP(hipStreamSynchronize(c.stream));
+        for (int i = 0; i < c.nRep; ++i) eps[i] = S[i].eps;
+    }
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    fill_stats(st, stats);
+    return st.converged ? YCN_OK : YCN_ERR_NOT_CONVERGED;
+}
+
+int ycn_langevin_run(ycn_ctx *ctx, uint64_t step0, int nsteps, ycn_cg_stats *stats) {
+    CHECK_CTX(ctx);
+    if (nsteps < 0) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    SolveStats tot;
+    tot.systems = c.nSys;
+    double meanSum = 0;
+    for (int i = 0; i < nsteps; ++i) {
+        SolveStats st;
+        int rc = step_impl(c, step0 + (uint64_t)i, &st);
+        if (rc) return rc;
+        tot.iters_max += st.iters_max;
+        tot.restarts += st.restarts;
+        tot.converged &= st.converged;
+        tot.rr_max = std::max(tot.rr_max, st.rr_max);
+        tot.true_rel_max = std::max(tot.true_rel_max, st.true_rel_max);
+        meanSum += st.iters_mean;
+    }
+    tot.iters_mean = nsteps ? meanSum / nsteps : 0;
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    fill_stats(tot, stats);
+    return tot.converged ? YCN_OK : YCN_ERR_NOT_CONVERGED;
+}
+
+int ycn_measure(ycn_ctx *ctx, int replica, uint64_t step, ycn_observables *obs, double *slices, double *corr) {
+    CHECK_CTX(ctx);
+    Ctx &c = *ctx;
+    if (replica < 0 || replica >= c.nRep) return YCN_ERR_INVALID;
+    const int Nt = c.p.Nt;
+    DevTmp dsl;
+    YCN_HIP(hipMalloc(&dsl.p, sizeof(double) * (3 * Nt + 8)));
+    double *sl = (double *)dsl.p, *sq = sl + Nt, *cr = sl + 2 * Nt;
+    int rc = field_slices(c, replica, sl, sq);
+    if (rc) return rc;
+    std::vector<double> hs(2 * Nt);
+    YCN_HIP(hipMemcpyAsync(hs.data(), sl, sizeof(double) * 2 * Nt, hipMemcpyDeviceToHost, c.stream));
+    std::vector<StepState> S(c.nRep);
+    YCN_HIP(hipMemcpyAsync(S.data(), c.step_d, sizeof(StepState) * c.nRep, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    ycn_observables o{};
+    for (int t = 0; t < Nt; ++t) { o.magnetisation += hs[t]; o.phi2 += hs[Nt + t]; }
+    if (slices) std::copy(hs.begin(), hs.begin() + Nt, slices);
+    o.eps = S[replica].eps;
+    o.langevin_time = S[replica].tL;
+    o.clamp_events = S[replica].clamps;
+    const double *phi_rep = c.phi[c.cur] + (size_t)replica * c.V;
+    if (corr) {
+        // point source (main.cu:441-469): delta_{x,0} (1,1,0,0); CG on D D^dag, then D^dag
+        std::vector<double> src(8 * (size_t)c.V, 0.0);
+        src[0] = 1.0;
+        src[2] = 1.0;
+        DevTmp aos;
+        const size_t nb = sizeof(double) * src.size();
+        YCN_HIP(hipMalloc(&aos.p, nb));
+        YCN_HIP(hipMemcpyAsync(aos.p, src.data(), nb, hipMemcpyHostToDevice, c.stream));
+        if ((rc = aos_to_soa(c, (const double *)aos.p, c.eta, 1))) return rc;
+        const int save = c.nNoise;
+        c.nNoise = 1;
+        SolveStats st;
+        rc = cg_solve(c, phi_rep, 1, YCN_NORMAL, false, 0, 0, &st);
+        if (!rc) rc = dirac_apply(c, phi_rep, c.x, c.t, 1, 1);
+        c.nNoise = save;
+        if (rc) return rc;
+        if ((rc = spinor_slices(c, c.t, cr))) return rc;
+        YCN_HIP(hipMemcpyAsync(corr, cr, sizeof(double) * Nt, hipMemcpyDeviceToHost, c.stream));
+        YCN_HIP(hipStreamSynchronize(c.stream));
+    }
+    // condensate: fresh drift with measurement noise (main.cu:472-481)
+    {
+        SolveStats st;
+        if ((rc = force_impl(c, step, 0x4D00u, &st))) return rc;
+        std::vector<double> d(c.V);
+        YCN_HIP(hipMemcpyAsync(d.data(), c.drift + (size_t)replica * c.V, sizeof(double) * c.V, hipMemcpyDeviceToHost,
+                               c.stream));
+        YCN_HIP(hipStreamSynchronize(c.stream));
+        double s = 0;
+        for (double v : d) s += v;
+        o.trace = (c.p.g != 0.0) ? s / c.p.g : 0.0;
+    }
+    if (obs) *obs = o;
+    return YCN_OK;
+}
+
+int ycn_sync(ycn_ctx *ctx) {
+    CHECK_CTX(ctx);
+    Ctx &c = *ctx;
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int ycn_profile_begin(ycn_ctx *ctx, const char *kernel) {
+    CHECK_CTX(ctx);
+    if (!kernel) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    static const char *names[PK_COUNT] = {"cg_normal_op", "cg_update", "dirac", "noise_cols", "bosonic"};
+    int k = PK_NONE;
+    for (int i = 0; i < PK_COUNT; ++i)
+        if (std::strcmp(kernel, names[i]) == 0) k = i;
+    if (k == PK_NONE) return fail(c, YCN_ERR_INVALID, std::string("unknown kernel ") + kernel);
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    c.profTarget = k;
+    c.evCount = 0;
+    c.profBytes = 0;
+    return YCN_OK;
+}
+
+int ycn_profile_end(ycn_ctx *ctx, double *total_ms, int64_t *launches, double *bytes_per_launch) {
+    CHECK_CTX(ctx);
+    Ctx &c = *ctx;
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    double tot = 0;
+    for (int i = 0; i < c.evCount; ++i) {
+        float ms = 0;
+        YCN_HIP(hipEventElapsedTime(&ms, c.evStart[i], c.evStop[i]));
+        tot += ms;
+    }
+    if (total_ms) *total_ms = tot;
+    if (launches) *launches = c.evCount;
+    if (bytes_per_launch) *bytes_per_launch = c.profBytes;
+    c.profTarget = PK_NONE;
+    return YCN_OK;
+}
+
+}  // extern "C"

@@ -1,0 +1,143 @@
+// ycn_internal.hpp -- context and internal launch interfaces of libycn (host side).
+#pragma once
+#include <hip/hip_runtime.h>
+
+#include <stdint.h>
+#include <string>
+#include <vector>
+
+#include "../../include/ycn.h"
+
+namespace ycn {
+
+// ---------------------------------------------------------------------------
+// FFT plan for one transform length (mixed radix, Stockham in LDS).
+// ---------------------------------------------------------------------------
+constexpr int kMaxFactors = 24;
+struct FftPlan {
+    int n;
+    int nf;
+    int radix[kMaxFactors];
+    const double2 *tw;  // device: tw[e] = exp(-2 pi i e / n), e < n
+};
+
+// Per-system CG state (device resident; written by the last block of a kernel).
+struct CgState {
+    double rr;         // recursive |r|^2
+    double bnorm2;     // |b|^2
+    double stop;       // stop when sqrt(rr) <= stop
+    double beta;
+    double2 alpha;
+    double true_rel;   // |b - A x| / |b| of the last true-residual check
+    double tol;
+    int iter;
+    int done;
+    int restarts;
+    int converged;
+    int maxIter;
+    int mode;          // YCN_TOL_*
+    int maxRestarts;
+    int pad;
+};
+
+// Per-replica Langevin state (device resident).
+struct StepState {
+    double eps;        // last step size
+    double tL;         // accumulated Langevin time
+    double maxDrift;   // last (clamped) max |drift|
+    int clamps;        // clamp events
+    int pad;
+};
+
+enum ProfKernel { PK_NONE = -1, PK_NORMAL_OP = 0, PK_CG_UPDATE, PK_DIRAC, PK_NOISE_COLS, PK_BOSONIC, PK_COUNT };
+
+struct Ctx {
+    ycn_params p;
+    int V = 0, H = 0, nRep = 0, nNoise = 0, nSys = 0;
+    hipStream_t stream = nullptr;
+    std::string err;
+
+    // host copies of per-replica parameters
+    std::vector<double> m2;
+    std::vector<uint64_t> seeds;
+    double epsBar = 0;
+
+    // device buffers
+    double *phi[2] = {nullptr, nullptr};  // ping-pong for the fused fixed-eps update
+    int cur = 0;
+    double *noise = nullptr, *drift = nullptr;
+    double2 *spec = nullptr;              // half spectrum [rep][t][H]
+    int *kmax = nullptr;                  // mask: keep kx <= kmax[kt]
+    int kmaxAll = -1;
+    double *m2_d = nullptr;
+    uint64_t *seed_d = nullptr;
+    StepState *step_d = nullptr;
+    FftPlan planX{}, planT{};
+
+    // CG (FP64): [nSys][4][V] double2 each
+    double2 *eta = nullptr, *x = nullptr, *r = nullptr, *pv[2] = {nullptr, nullptr}, *t = nullptr, *Ap = nullptr;
+    CgState *cg_d = nullptr;
+    double2 *partials = nullptr;          // [nSys][maxBlocks]
+    unsigned *counters = nullptr;         // [nSys + nRep] last-block counters
+    int *active = nullptr;                // systems not yet converged
+    int *active_h = nullptr;              // pinned host mirror
+    int maxBlocks = 0;
+    int lastIters = 32;                   // predictor for the CG launch chunk
+
+    // mixed precision: [nSys][4][V] float2
+    float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *ts = nullptr, *Aps = nullptr, *xs = nullptr;
+    float *phis = nullptr;
+
+    // profiling
+    int profTarget = PK_NONE;
+    std::vector<hipEvent_t> evStart, evStop;
+    int evCount = 0;
+    double profBytes = 0;
+};
+
+// error helpers
+#define YCN_HIP(call)                                                                   \
+    do {                                                                                \
+        hipError_t e_ = (call);                                                         \
+        if (e_ != hipSuccess) {                                                         \
+            c.err = std::string(#call) + ": " + hipGetErrorString(e_);                  \
+            return YCN_ERR_HIP;                                                         \
+        }                                                                               \
+    } while (0)
+
+// profiling brackets around a launch of kernel `k`
+void prof_before(Ctx &c, int k, double bytes);
+void prof_after(Ctx &c, int k);
+
+// noise.hip
+int noise_init(Ctx &c);
+void noise_free(Ctx &c);
+// xi == nullptr: Philox field of (seeds, stream 0, step); else filter xi (device, nRep*V)
+int noise_generate(Ctx &c, uint64_t step, const double *xi_dev, double *out_dev);
+
+// cg.hip
+struct SolveStats {
+    int iters_max = 0, restarts = 0, converged = 1, systems = 0;
+    double rr_max = 0, true_rel_max = 0, iters_mean = 0;
+};
+int cg_alloc(Ctx &c);
+void cg_free(Ctx &c);
+int dirac_apply(Ctx &c, const double *phi_dev, const double2 *in, double2 *out, int nsys, int dagger);
+// b: if gen, eta is generated from (seeds, streamBase + j, step); else eta must hold b.
+int cg_solve(Ctx &c, const double *phi_dev, int nsys, int mtype, bool gen, uint32_t streamBase,
+             uint64_t step, SolveStats *st);
+int fermion_drift(Ctx &c, const double *phi_dev, double *drift_dev);
+// correlator / trace helpers
+int spinor_slices(Ctx &c, const double2 *psi, double *out_dev);
+
+// update.hip
+int bosonic_update(Ctx &c, int fixedEps);
+int field_slices(Ctx &c, int rep, double *slices_dev, double *sums_dev);
+int hot_start(Ctx &c, uint64_t initSeed);
+int set_int(Ctx &c, int *ptr, int v);
+
+// layout conversion (abi)
+int aos_to_soa(Ctx &c, const double *aos_dev, double2 *soa, int nsys);
+int soa_to_aos(Ctx &c, const double2 *soa, double *aos_dev, int nsys);
+
+}  // namespace ycn
