@@ -1,0 +1,165 @@
+#!/usr/bin/env python3
+"""Langevin steps/s of the 2D lattice Yukawa model with coloured noise on one MI355X.
+
+Workload (the headline configuration the metric is quoted on): Nt = Nx = 1024,
+M = 0.05, g = 0.5, m^2 = -0.4, lambda = 1.0, r = 1, cutoff s = 0.5, eps = 0.005 fixed,
+one Gaussian noise vector per drift estimate, every CG solve run to true relative
+residual <= 1e-10 (recomputed in FP64, restarted while above).  Start: hot start
+phi ~ U[-1, 1] (Philox, seed 7), then `warmup` untimed steps; the timed region is
+`steps` full Langevin steps (coloured noise, fermion force, bosonic drift, update),
+bracketed by device synchronisation.  Inputs and state stay resident in HBM.
+
+Prints one JSON line.  `roofline` is measured live: every launch of the dominant kernel
+in the timed region is bracketed by HIP events on the library's own stream.
+`cpu_baseline` times the FP64 CPU oracle on a bounded sample of the same workload.
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import platform
+import sys
+import time
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0
+
+WORKLOAD = dict(Nt=1024, Nx=1024, m2=-0.4, lam=1.0, g=0.5, M=0.05, r=1.0, cutFraction=0.5,
+                driftMode=0, epsBar=0.005, adaptive=0, seed=1234, cgTol=1e-10, cgTolMode=1,
+                cgMaxIter=20000)
+
+
+def cpu_baseline(gpu_iters_per_solve: float, budget_s: float = 20.0):
+    """FP64 oracle on the host cores: a bounded sample of the 1024^2 step.
+
+    The sample: one coloured-noise field, one bosonic update, and a fixed number of
+    CG iterations on D^dag D (each = two oracle Dirac applications + BLAS-1), timed
+    separately; a step is then priced at the GPU's measured iterations per solve
+    plus the true-residual check."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+
+    cores = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    p = O.params(**WORKLOAD)
+    Nt, Nx = p.Nt, p.Nx
+    phi = O.hot_start(7, Nt, Nx)
+    t0 = time.perf_counter()
+    noise = O.coloured_noise(p, 0)
+    t_noise = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    O.bosonic_update(p, phi, np.zeros((Nt, Nx)), noise)
+    t_upd = time.perf_counter() - t0
+    # CG iterations: fixed-count solves until the budget is used
+    eta = O.fermion_noise(p.seed, 0, 0, Nt * Nx)
+    q = O.params(**{**WORKLOAD, "cgTol": 0.0, "cgTolMode": 0, "cgMaxIter": 4})
+    iters, t_cg = 0, 0.0
+    while t_cg < budget_s and iters < 400:
+        t0 = time.perf_counter()
+        _, st = O.cg(q, phi, eta, 1)
+        t_cg += time.perf_counter() - t0
+        iters += st["iters"]
+    # each fixed-count solve also pays one true-residual check (~ one iteration)
+    solves = iters / 4
+    t_iter = t_cg / (iters + solves)
+    t_step = t_noise + t_upd + (gpu_iters_per_solve + 1.0) * t_iter
+    return {
+        "value": 1.0 / t_step,
+        "unit": "steps/s",
+        "cores": cores,
+        "kind": "port",
+        "sample": (f"oracle FP64 at 1024^2 on {cores} host threads: 1 coloured-noise field "
+                   f"({t_noise:.2f} s), 1 bosonic update ({t_upd:.3f} s), {iters} CG iterations "
+                   f"({t_iter * 1e3:.1f} ms/iter incl. its share of true-residual checks); step priced "
+                   f"at the GPU's {gpu_iters_per_solve:.0f} iterations/solve + 1"),
+        "cpu": platform.processor() or platform.machine(),
+    }
+
+
+def main():
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--size", type=int, default=1024, help="Nt = Nx (default: the 1024^2 workload)")
+    ap.add_argument("--profile-kernel", default="cg_update")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1:
+        print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
+              "(the multi-GPU decomposition is design-only)", file=sys.stderr)
+        sys.exit(2)
+
+    import yukawacolourednoise_amd as Y
+
+    wl = dict(WORKLOAD)
+    wl["Nt"] = wl["Nx"] = args.size
+    p = Y.default_params(**wl)
+    ctx = Y.Context(p)
+    ctx.hot_start(7)
+    step = 0
+    warm_iters = []
+    for _ in range(args.warmup):
+        r = ctx.langevin_step(step)
+        warm_iters.append(r.stats["iters"])
+        step += 1
+    ctx.sync()
+
+    ctx.profile_begin(args.profile_kernel)
+    t0 = time.perf_counter()
+    st = ctx.langevin_run(step, args.steps)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    k_ms, k_n, k_bytes = ctx.profile_end()
+    step += args.steps
+    ctx.close()
+
+    steps_per_s = args.steps / dt
+    iters_per_solve = st["iters"] / max(1, args.steps)
+    avg_ms = k_ms / max(1, k_n)
+    achieved = (k_bytes / 1e9) / (avg_ms / 1e3) if k_n else 0.0
+    out = {
+        "metric": "Langevin steps/s @1024^2 (true resid <= 1e-10)",
+        "value": steps_per_s,
+        "unit": "steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": 1e3 * dt / args.steps,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f64",
+        "data": "synthetic (Philox hot start, seed 7)",
+        "config": {
+            "workload": f"{args.size}x{args.size} near-critical Langevin step, plain FP64 CG on D^dag D "
+                        "to true relative residual 1e-10, 1 noise vector, fixed eps 0.005",
+            "Nt": args.size, "Nx": args.size, "M": wl["M"], "g": wl["g"], "m2": wl["m2"],
+            "lambda": wl["lam"], "s": wl["cutFraction"], "eps": wl["epsBar"],
+            "cg": "fp64", "precond": "none", "parallelism": "single GPU",
+        },
+        "cg_iters_per_solve": iters_per_solve,
+        "cg_true_rel_resid_max": st["true_rel_resid"],
+        "roofline": {
+            "kernel": args.profile_kernel,
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_us": avg_ms * 1e3,
+            "launches": k_n,
+            "algorithmic_bytes_per_launch": k_bytes,
+        },
+    }
+    if not args.no_cpu_baseline:
+        out["cpu_baseline"] = cpu_baseline(iters_per_solve)
+    print(json.dumps(out), flush=True)
+
+
+if __name__ == "__main__":
+    main()

@@ -300,9 +300,8 @@ void orc_dirac(const orc_params *p, const double *phi, const double *in_, double
 
 /* solve_kernel's operator (CGsolver.cu:162-182): mtype 1 (Dagger): D then D^dag,
  * i.e. D^dag D; mtype 0 (Normal): D^dag then D, i.e. D D^dag. */
-void orc_normal_op(const orc_params *p, const double *phi, const double *in, double *out, int mtype) {
-    size_t n = (size_t)8 * p->Nt * p->Nx;
-    double *tmp = (double *)malloc(sizeof(double) * n);
+static void normal_op_tmp(const orc_params *p, const double *phi, const double *in, double *out, int mtype,
+                          double *tmp) {
     if (mtype == 1) {
         orc_dirac(p, phi, in, tmp, 0);
         orc_dirac(p, phi, tmp, out, 1);
@@ -310,6 +309,12 @@ void orc_normal_op(const orc_params *p, const double *phi, const double *in, dou
         orc_dirac(p, phi, in, tmp, 1);
         orc_dirac(p, phi, tmp, out, 0);
     }
+}
+
+void orc_normal_op(const orc_params *p, const double *phi, const double *in, double *out, int mtype) {
+    size_t n = (size_t)8 * p->Nt * p->Nx;
+    double *tmp = (double *)malloc(sizeof(double) * n);
+    normal_op_tmp(p, phi, in, out, mtype, tmp);
     free(tmp);
 }
 
@@ -367,7 +372,7 @@ int orc_cg(const orc_params *p, const double *phi, const double *b_, double *x_,
     const cplx *b = (const cplx *)b_;
     cplx *x = (cplx *)x_;
     cplx *r = (cplx *)malloc(sizeof(cplx) * n), *pp = (cplx *)malloc(sizeof(cplx) * n);
-    cplx *Ap = (cplx *)malloc(sizeof(cplx) * n);
+    cplx *Ap = (cplx *)malloc(sizeof(cplx) * n), *tmp = (cplx *)malloc(sizeof(cplx) * n);
     int maxIter = maxIterOverride > 0 ? maxIterOverride : p->cgMaxIter;
     double bnorm = sqrt(cabs_(cdot(n, b, b)));
     double stopAbs = p->cgTolMode == 0 ? p->cgTol : p->cgTol * bnorm;
@@ -378,7 +383,7 @@ int orc_cg(const orc_params *p, const double *phi, const double *b_, double *x_,
     for (;;) {
         rr = cabs_(cdot(n, r, r));
         while (k < maxIter && sqrt(rr) > stopAbs) {
-            orc_normal_op(p, phi, (const double *)pp, (double *)Ap, mtype);
+            normal_op_tmp(p, phi, (const double *)pp, (double *)Ap, mtype, (double *)tmp);
             cplx dot = cdot(n, pp, Ap);
             cplx rrc = {rr, 0.0};
             cplx alpha = c_div(rrc, dot);
@@ -396,7 +401,7 @@ int orc_cg(const orc_params *p, const double *phi, const double *b_, double *x_,
         }
         converged = sqrt(rr) <= stopAbs;
         /* true residual b - A x, recomputed in FP64 */
-        orc_normal_op(p, phi, (const double *)x, (double *)Ap, mtype);
+        normal_op_tmp(p, phi, (const double *)x, (double *)Ap, mtype, (double *)tmp);
         for (size_t i = 0; i < n; ++i) Ap[i] = c_sub(b[i], Ap[i]);
         double tr = sqrt(cabs_(cdot(n, Ap, Ap)));
         trueRel = bnorm > 0 ? tr / bnorm : tr;
@@ -410,7 +415,7 @@ int orc_cg(const orc_params *p, const double *phi, const double *b_, double *x_,
         break;
     }
     if (st) { st->iters = k; st->restarts = restarts; st->converged = converged; st->rr = rr; st->true_rel_resid = trueRel; }
-    free(r); free(pp); free(Ap);
+    free(r); free(pp); free(Ap); free(tmp);
     return converged ? 0 : 1;
 }
 
