@@ -103,6 +103,11 @@ int ycn_create(const ycn_params *p, ycn_ctx **out);
 void ycn_destroy(ycn_ctx *ctx);
 const char *ycn_last_error(const ycn_ctx *ctx);
 int ycn_abi_version(void);
+/* Stencil-tile autotuner result of ycn_create: the measured time per launch of the
+ * fused normal operator for each tile shape that divides the lattice, and the choice.
+ * No reference counterpart (the reference sizes launches with
+ * cudaOccupancyMaxPotentialBlockSize, code/src/device_info.h:18-66). */
+const char *ycn_tuning_report(const ycn_ctx *ctx);
 
 /* Per-replica m^2 and seeds for batched chains (nReplicas entries each; either may
  * be NULL to keep the ycn_params value).  No reference counterpart: the reference

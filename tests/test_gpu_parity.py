@@ -210,3 +210,64 @@ def test_measure(gpu):
     assert abs(ob["phi2"] - (phi ** 2).sum()) < 1e-12 * (phi ** 2).sum()
     cref, _ = O.correlator(orc, phi)
     assert rel(cr, cref) < 1e-9
+
+
+@pytest.mark.parametrize("tile", [-1, 0, 1, 2, 3])
+@pytest.mark.parametrize("mtype", [0, 1])
+def test_cg_every_tile(gpu, tile, mtype, monkeypatch):
+    """Each fused normal-operator tile shape (and the two-stencil pair), forced through
+    YCN_TILE, gives the oracle's iterate at a fixed iteration count."""
+    Y = gpu
+    monkeypatch.setenv("YCN_TILE", str(tile))
+    Nt, Nx = 32, 64
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.1, g=0.5, cgTol=0.0, cgTolMode=0, cgMaxIter=25)
+    rng = np.random.default_rng(7)
+    phi = rng.uniform(-1, 1, (Nt, Nx))
+    b = rng.standard_normal((Nt * Nx, 4)) + 1j * rng.standard_normal((Nt * Nx, 4))
+    with Y.Context(p) as c:
+        rep = c.tuning_report()
+        x, st = c.cg_solve(b, phi=phi, mtype=mtype)
+    xr, sr = O.cg(orc, phi, b, mtype)
+    assert st["iters"] == sr["iters"] == 25, rep
+    assert rel(x, xr) < RTOL, rep
+
+
+# ---------------------------------------------------------------------------
+# mixed precision: FP32 inner CG, FP64 reliable updates, FP64 true-residual stop
+# ---------------------------------------------------------------------------
+@pytest.mark.parametrize("mtype", [0, 1])
+def test_mixed_cg(gpu, mtype):
+    Y = gpu
+    Nt, Nx = 32, 64
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.05, g=0.5, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    p.cgPrecision = Y.YCN_CG_MIXED
+    p.reliableEvery = 50
+    rng = np.random.default_rng(8)
+    phi = rng.uniform(-1, 1, (Nt, Nx))
+    b = rng.standard_normal((Nt * Nx, 4)) + 1j * rng.standard_normal((Nt * Nx, 4))
+    with Y.Context(p) as c:
+        x, st = c.cg_solve(b, phi=phi, mtype=mtype)
+    xr, sr = O.cg(orc, phi, b, mtype)
+    assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+    # the FP64 true residual of the mixed solution, recomputed by the oracle
+    Ax = O.normal_op(orc, phi, x, mtype)
+    assert np.linalg.norm(b - Ax) / np.linalg.norm(b) <= 1.5e-10
+    assert rel(x, xr) < 1e-7   # both solutions are within kappa * tol of the exact one
+    # iteration overhead of FP32 inner iterations stays modest
+    assert st["iters"] <= 1.5 * sr["iters"] + 100, (st, sr)
+
+
+def test_mixed_langevin_step(gpu):
+    """Field error of one mixed-precision step against the FP64 oracle step."""
+    Y = gpu
+    Nt, Nx = 32, 32
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.05, g=0.5, m2=-0.4, lam=1.0, epsBar=0.005)
+    p.cgPrecision = Y.YCN_CG_MIXED
+    phi0 = O.hot_start(7, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi0)
+        r = c.langevin_step(0)
+        g = c.get_field()
+    ref, _, _ = O.langevin_step(orc, phi0, 0)
+    assert r.stats["true_rel_resid"] <= 1e-10
+    assert rel(g, ref) < 1e-10

@@ -34,6 +34,7 @@ YCN_PRECOND_NONE, YCN_PRECOND_CNN = 0, 1
 # every entry point declared in include/ycn.h
 EXPORTED = [
     "ycn_default_params", "ycn_create", "ycn_destroy", "ycn_last_error", "ycn_abi_version",
+    "ycn_tuning_report",
     "ycn_set_replicas", "ycn_set_eps_bar", "ycn_set_field", "ycn_get_field", "ycn_hot_start",
     "ycn_coloured_noise", "ycn_coloured_filter", "ycn_dirac_apply", "ycn_cg_solve",
     "ycn_fermion_force", "ycn_langevin_step", "ycn_langevin_run", "ycn_measure", "ycn_sync",
@@ -91,6 +92,8 @@ def _load():
     L.ycn_destroy.restype = None
     L.ycn_last_error.argtypes = [vp]
     L.ycn_last_error.restype = C.c_char_p
+    L.ycn_tuning_report.argtypes = [vp]
+    L.ycn_tuning_report.restype = C.c_char_p
     L.ycn_set_replicas.argtypes = [vp, dp, C.POINTER(C.c_uint64)]
     L.ycn_set_eps_bar.argtypes = [vp, C.c_double]
     L.ycn_set_field.argtypes = [vp, dp]
@@ -257,6 +260,9 @@ class Context:
         self._check(lib.ycn_measure(self._h, replica, step, C.byref(ob), None if sl is None else _ptr(sl),
                                     None if cr is None else _ptr(cr)))
         return ob.as_dict(), sl, cr
+
+    def tuning_report(self):
+        return lib.ycn_tuning_report(self._h).decode()
 
     def sync(self):
         self._check(lib.ycn_sync(self._h))

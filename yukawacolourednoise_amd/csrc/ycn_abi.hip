@@ -82,6 +82,8 @@ void ycn_default_params(ycn_params *p) {
 
 const char *ycn_last_error(const ycn_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
 
+const char *ycn_tuning_report(const ycn_ctx *ctx) { return ctx ? ctx->tuneReport.c_str() : ""; }
+
 void ycn_destroy(ycn_ctx *ctx) {
     if (!ctx) return;
     Ctx &c = *ctx;
@@ -89,6 +91,7 @@ void ycn_destroy(ycn_ctx *ctx) {
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     noise_free(c);
     cg_free(c);
+    mp_free(c);
     double **bufs[] = {&c.phi[0], &c.phi[1], &c.noise, &c.drift, &c.m2_d};
     for (double **b : bufs)
         if (*b) (void)hipFree(*b);
@@ -105,9 +108,12 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
     *out = nullptr;
     if (p->Nt < 2 || p->Nx < 2 || p->nReplicas < 1 || p->nNoise < 1 || p->cgMaxIter < 0 || p->cgTol < 0 ||
         (p->cgTolMode != YCN_TOL_ABS_RECURSIVE && p->cgTolMode != YCN_TOL_REL_TRUE) ||
-        (p->driftMode != 0 && p->driftMode != 1))
+        (p->driftMode != 0 && p->driftMode != 1) ||
+        (p->cgPrecision != YCN_CG_FP64 && p->cgPrecision != YCN_CG_MIXED) || p->reliableEvery < 1 ||
+        (p->precond != YCN_PRECOND_NONE && p->precond != YCN_PRECOND_CNN))
         return YCN_ERR_INVALID;
     if ((int64_t)p->Nt * p->Nx > (int64_t)1 << 28) return YCN_ERR_INVALID;
+    if (p->precond == YCN_PRECOND_CNN) return YCN_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return YCN_ERR_NO_DEVICE;
     if (p->device < 0 || p->device >= ndev) return YCN_ERR_INVALID;
@@ -147,6 +153,12 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         return bail(YCN_ERR_HIP);
     if ((rc = noise_init(c))) return bail(rc);
     if ((rc = cg_alloc(c))) return bail(rc);
+    if (c.p.cgPrecision == YCN_CG_MIXED && (rc = mp_alloc(c))) return bail(rc);
+    if ((rc = cg_autotune(c))) return bail(rc);
+    if (c.p.cgPrecision == YCN_CG_MIXED && c.tile32 < 0) {
+        c.err = "mixed-precision CG needs Nt % 4 == 0 and Nx % 16 == 0";
+        return bail(YCN_ERR_UNSUPPORTED);
+    }
     c.evStart.resize(8192);
     c.evStop.resize(8192);
     for (size_t i = 0; i < c.evStart.size(); ++i)

@@ -37,7 +37,8 @@ struct CgState {
     int maxIter;
     int mode;          // YCN_TOL_*
     int maxRestarts;
-    int pad;
+    int final_;        // mixed precision: converged or gave up (no further reliable updates)
+    double rr_prev;    // |r|^2 before the last update (mixed precision: beta after a reliable update)
 };
 
 // Per-replica Langevin state (device resident).
@@ -84,8 +85,12 @@ struct Ctx {
     int maxBlocks = 0;
     int lastIters = 32;                   // predictor for the CG launch chunk
 
+    // fused normal-operator tiles chosen by the autotuner (-1: three-kernel path)
+    int tile64 = -1, tile32 = -1;
+    std::string tuneReport;
+
     // mixed precision: [nSys][4][V] float2
-    float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *ts = nullptr, *Aps = nullptr, *xs = nullptr;
+    float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *Aps = nullptr, *xs = nullptr;
     float *phis = nullptr;
 
     // profiling
@@ -122,6 +127,12 @@ struct SolveStats {
 };
 int cg_alloc(Ctx &c);
 void cg_free(Ctx &c);
+int cg_autotune(Ctx &c);
+// mixed precision (cg_mixed.hip)
+int mp_alloc(Ctx &c);
+void mp_free(Ctx &c);
+int mp_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t streamBase, uint64_t step,
+             SolveStats *st);
 int dirac_apply(Ctx &c, const double *phi_dev, const double2 *in, double2 *out, int nsys, int dagger);
 // b: if gen, eta is generated from (seeds, streamBase + j, step); else eta must hold b.
 int cg_solve(Ctx &c, const double *phi_dev, int nsys, int mtype, bool gen, uint32_t streamBase,
