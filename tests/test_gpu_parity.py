@@ -271,3 +271,52 @@ def test_mixed_langevin_step(gpu):
     ref, _, _ = O.langevin_step(orc, phi0, 0)
     assert r.stats["true_rel_resid"] <= 1e-10
     assert rel(g, ref) < 1e-10
+
+
+# ---------------------------------------------------------------------------
+# committed golden fixtures (tests/golden/)
+# ---------------------------------------------------------------------------
+import json  # noqa: E402
+import os  # noqa: E402
+
+GOLD = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden")
+GOLD_MAN = json.load(open(os.path.join(GOLD, "manifest.json")))
+
+
+def _gold(name):
+    return dict(np.load(os.path.join(GOLD, name), allow_pickle=False))
+
+
+@pytest.mark.parametrize("name", sorted(n for n in GOLD_MAN if n.startswith("step_")))
+def test_gpu_golden_step(gpu, name):
+    Y = gpu
+    f = _gold(name)
+    keys = ["Nt", "Nx", "m2", "lam", "g", "M", "r", "cutFraction", "driftMode", "epsBar", "adaptive", "seed",
+            "cgTol", "cgTolMode", "cgMaxIter"]
+    p = Y.default_params(**{k: f[k].item() for k in keys})
+    step = int(f["step"])
+    with Y.Context(p) as c:
+        c.hot_start(int(f["init_seed"]))
+        assert np.array_equal(c.get_field(), f["phi0"])
+        assert rel(c.coloured_noise(step), f["noise"]) < RTOL
+        r = c.langevin_step(step)
+        assert rel(c.get_field(), f["phi1"]) < RTOL
+        assert abs(r.eps[0] - float(f["eps"])) <= 1e-13 * float(f["eps"])
+        assert abs(r.stats["iters"] - int(f["cg_iters"])) <= 1
+
+
+@pytest.mark.parametrize("name", sorted(n for n in GOLD_MAN if n.startswith("filter_")))
+def test_gpu_golden_filter(gpu, name):
+    Y = gpu
+    f = _gold(name)
+    Nt, Nx = f["xi"].shape
+    with Y.Context(Y.default_params(Nt=Nt, Nx=Nx, cutFraction=float(f["cutFraction"]))) as c:
+        assert rel(c.coloured_filter(f["xi"]), f["out"]) < RTOL
+
+
+def test_gpu_golden_dirac(gpu):
+    Y = gpu
+    f = _gold("dirac_8x12.npz")
+    with Y.Context(Y.default_params(Nt=8, Nx=12, M=float(f["M"]), g=float(f["g"]), r=float(f["r"]))) as c:
+        assert rel(c.dirac_apply(f["psi"], phi=f["phi"]), f["Dpsi"]) < RTOL
+        assert rel(c.dirac_apply(f["psi"], phi=f["phi"], dagger=True), f["Ddagpsi"]) < RTOL
