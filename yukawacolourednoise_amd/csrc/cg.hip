@@ -30,6 +30,7 @@
 namespace ycn {
 
 constexpr int kNT = 256;  // threads per block for the stencil and BLAS-1 kernels
+constexpr int kGraphIters = 8;  // CG iterations per captured graph (even: keeps the p parity)
 
 __device__ __forceinline__ WilsonCoef make_coef(double r, double M, double g) {
     WilsonCoef w;
@@ -436,6 +437,8 @@ int cg_alloc(Ctx &c) {
 }
 
 void cg_free(Ctx &c) {
+    for (const Ctx::GraphEntry &e : c.graphs) (void)hipGraphExecDestroy(e.exec);
+    c.graphs.clear();
     double2 **bufs[] = {&c.eta, &c.x, &c.r, &c.pv[0], &c.pv[1], &c.t, &c.Ap};
     for (double2 **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
@@ -515,6 +518,42 @@ static int launch_iteration(Ctx &c, const LatticeArgs &L, const double *phi, int
     launch_normal_op(c, L, phi, nsys, mtype, parity, c.tile64);
     launch_cg_c(c, nsys, c.pv[parity ^ 1]);
     YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+
+// Launch n CG iterations (n even keeps the p ping-pong parity).  When graphs are on and
+// no kernel is being profiled, the n iterations are captured once into a hipGraph per
+// (field, systems, operator, n, precision, tile) and replayed; every kernel reads its
+// scalars from device state and exits early for converged systems, so replays need no
+// host-side parameters.
+template <typename Launch>
+static int run_chunk(Ctx &c, const void *phi, int nsys, int mtype, int n, int precision, int tile, Launch launch) {
+    if (!c.useGraphs || c.profTarget != PK_NONE) {
+        for (int i = 0; i < n; ++i) launch(i & 1);
+        YCN_HIP(hipGetLastError());
+        return YCN_OK;
+    }
+    for (const Ctx::GraphEntry &e : c.graphs)
+        if (e.phi == phi && e.nsys == nsys && e.mtype == mtype && e.n == n && e.precision == precision &&
+            e.tile == tile) {
+            YCN_HIP(hipGraphLaunch(e.exec, c.stream));
+            return YCN_OK;
+        }
+    hipGraph_t g;
+    YCN_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    for (int i = 0; i < n; ++i) launch(i & 1);
+    const hipError_t le = hipGetLastError();
+    YCN_HIP(hipStreamEndCapture(c.stream, &g));
+    if (le != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        c.err = std::string("launch during capture: ") + hipGetErrorString(le);
+        return YCN_ERR_HIP;
+    }
+    hipGraphExec_t exec;
+    YCN_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
+    (void)hipGraphDestroy(g);
+    c.graphs.push_back({phi, nsys, mtype, n, precision, tile, exec});
+    YCN_HIP(hipGraphLaunch(exec, c.stream));
     return YCN_OK;
 }
 
@@ -623,21 +662,23 @@ int cg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
                            c.p.cgTol, c.p.cgTolMode, maxIter, maxRestarts);
     YCN_HIP(hipGetLastError());
 
-    int parity = 0, launched = 0, active = nsys;
-    int chunk = std::max(4, std::min(c.lastIters, maxIter));
+    int launched = 0, active = nsys;
+    int chunk = std::max(kGraphIters, std::min(c.lastIters, maxIter));
     const int hardCap = maxIter * (maxRestarts + 1) + 8;
     for (int guard = 0; guard < 64 + hardCap; ++guard) {
         while (active > 0 && launched < hardCap) {
+            // launch ~chunk iterations as graphs of kGraphIters, then poll
             const int n = std::min(chunk, hardCap - launched);
-            for (int i = 0; i < n; ++i) {
-                rc = launch_iteration(c, L, phi, nsys, mtype, parity);
+            for (int done_n = 0; done_n < n; done_n += kGraphIters) {
+                rc = run_chunk(c, phi, nsys, mtype, kGraphIters, YCN_CG_FP64, c.tile64, [&](int par) {
+                    (void)launch_iteration(c, L, phi, nsys, mtype, par);
+                });
                 if (rc) return rc;
-                parity ^= 1;
             }
-            launched += n;
+            launched += ((n + kGraphIters - 1) / kGraphIters) * kGraphIters;
             rc = poll_active(c, active);
             if (rc) return rc;
-            chunk = std::max(4, std::min(64, chunk / 2));
+            chunk = std::max(kGraphIters, std::min(64, chunk / 2));
         }
         // true residual check (may restart systems in YCN_TOL_REL_TRUE mode)
         if (mtype == YCN_DAGGER) {
@@ -653,7 +694,7 @@ int cg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
         rc = poll_active(c, active);
         if (rc) return rc;
         if (active == 0 || launched >= hardCap) break;
-        chunk = 8;
+        chunk = kGraphIters;
     }
     std::vector<CgState> S(nsys);
     YCN_HIP(hipMemcpyAsync(S.data(), c.cg_d, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream));
@@ -976,15 +1017,15 @@ int mp_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
     YCN_HIP(hipGetLastError());
     const int every = std::max(1, c.p.reliableEvery);
     const int hardCap = maxIter + every + 8;
-    int parity = 0, launched = 0, active = nsys;
+    int launched = 0, active = nsys;
     rc = poll_active(c, active);
     if (rc) return rc;
     while (active > 0 && launched < hardCap) {
-        for (int i = 0; i < every; ++i) {
-            launch_mp_iteration(c, nsys, mtype, parity, c.phis);
-            parity ^= 1;
-        }
-        launched += every;
+        const int nIt = every + (every & 1);   // even: keeps the p parity
+        rc = run_chunk(c, phi, nsys, mtype, nIt, YCN_CG_MIXED, c.tile32,
+                       [&](int par) { launch_mp_iteration(c, nsys, mtype, par, c.phis); });
+        if (rc) return rc;
+        launched += nIt;
         // reliable update: x += xs; r = b - A x (FP64)
         hipLaunchKernelGGL(k_mp_accum, g, dim3(kNT), 0, c.stream, c.V, (const CgState *)c.cg_d, c.x, c.xs);
         if (mtype == YCN_DAGGER) {

@@ -93,6 +93,15 @@ struct Ctx {
     float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *Aps = nullptr, *xs = nullptr;
     float *phis = nullptr;
 
+    // captured CG iteration chunks: key -> executable graph
+    struct GraphEntry {
+        const void *phi;
+        int nsys, mtype, n, precision, tile;
+        hipGraphExec_t exec;
+    };
+    std::vector<GraphEntry> graphs;
+    bool useGraphs = true;
+
     // profiling
     int profTarget = PK_NONE;
     std::vector<hipEvent_t> evStart, evStop;
