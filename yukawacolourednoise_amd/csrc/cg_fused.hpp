@@ -2,7 +2,8 @@
 //
 //   p' = r + beta p_old                          (over the tile plus a 2-site halo)
 //   q  = D1 p'                                   (tile plus a 1-site halo, in LDS)
-//   Ap = D2 q, <p', Ap> -> alpha = rr / <p', Ap> (tile; last block finishes the dot)
+//   Ap = D2 q, <p', Ap>                          (tile; one partial per block, a
+//                                                 finalize kernel forms alpha)
 // with (D1, D2) = (D, D^dag) for A = D^dag D and (D^dag, D) for A = D D^dag.
 //
 // One block owns a TT x TX tile of one system; p' is staged once in LDS (SoA planes,
@@ -59,7 +60,7 @@ __global__ void __launch_bounds__(TT *TX)
 k_cg_normal_fused(FusedArgs A, const R *__restrict__ phi, CgState *st,
                   const typename Vec2<R>::T *__restrict__ r, const typename Vec2<R>::T *__restrict__ pold,
                   typename Vec2<R>::T *__restrict__ pnew, typename Vec2<R>::T *__restrict__ Ap,
-                  double2 *__restrict__ part, unsigned *__restrict__ counters) {
+                  double2 *__restrict__ part) {
     using V2 = typename Vec2<R>::T;
     constexpr int NT = TT * TX;
     constexpr int W2 = TX + 4, H2 = TT + 4, N2 = W2 * H2;
@@ -68,7 +69,6 @@ k_cg_normal_fused(FusedArgs A, const R *__restrict__ phi, CgState *st,
     __shared__ V2 Q[4][N1];
     __shared__ R F[N1];
     __shared__ double2 red[NT / kWave > 0 ? NT / kWave : 1];
-    __shared__ bool last;
 
     const int sys = blockIdx.y;
     if (st[sys].done) return;
@@ -145,26 +145,7 @@ k_cg_normal_fused(FusedArgs A, const R *__restrict__ phi, CgState *st,
         acc.y += (double)(p0.x * o0.y - p0.y * o0.x) + (double)(p1.x * o1.y - p1.y * o1.x);
     }
     acc = block_sum2<NT>(acc, red);
-    if (threadIdx.x == 0) {
-        part[(size_t)sys * nb + blockIdx.x] = acc;
-        last = arrive_last(&counters[sys], nb);
-    }
-    __syncthreads();
-    if (!last) return;
-    double2 tot = make_double2(0.0, 0.0);
-    for (int i = threadIdx.x; i < nb; i += NT) {
-        const double *pp = reinterpret_cast<const double *>(part + (size_t)sys * nb + i);
-        tot.x += load_coherent(pp);
-        tot.y += load_coherent(pp + 1);
-    }
-    __syncthreads();
-    tot = block_sum2<NT>(tot, red);
-    if (threadIdx.x == 0) {
-        const double rr = st[sys].rr;
-        const double den = tot.x * tot.x + tot.y * tot.y;
-        st[sys].alpha = make_double2(rr * tot.x / den, -rr * tot.y / den);
-        counters[sys] = 0u;
-    }
+    if (threadIdx.x == 0) part[(size_t)sys * nb + blockIdx.x] = acc;
 }
 
 }  // namespace ycn
@@ -184,14 +165,14 @@ constexpr int kNumTiles32 = sizeof(kTiles32) / sizeof(kTiles32[0]);
 template <typename R, int TT, int TX>
 inline void launch_fused_tt(hipStream_t s, int nsys, bool dag1, const FusedArgs &A, const R *phi, CgState *st,
                             const typename Vec2<R>::T *r, const typename Vec2<R>::T *pold, typename Vec2<R>::T *pnew,
-                            typename Vec2<R>::T *Ap, double2 *part, unsigned *counters) {
+                            typename Vec2<R>::T *Ap, double2 *part) {
     dim3 grid(A.tilesPerSys, nsys), block(TT * TX);
     if (dag1)
         hipLaunchKernelGGL((k_cg_normal_fused<R, TT, TX, true>), grid, block, 0, s, A, phi, st, r, pold, pnew, Ap,
-                           part, counters);
+                           part);
     else
         hipLaunchKernelGGL((k_cg_normal_fused<R, TT, TX, false>), grid, block, 0, s, A, phi, st, r, pold, pnew, Ap,
-                           part, counters);
+                           part);
 }
 
 inline FusedArgs fused_args(const Ctx &c, int TT, int TX) {
@@ -215,35 +196,33 @@ inline bool tile_fits(const Ctx &c, TileShape t) { return c.p.Nt % t.TT == 0 && 
 template <typename R>
 inline void launch_fused(const Ctx &c, int ti, int nsys, bool dag1, const R *phi, CgState *st,
                          const typename Vec2<R>::T *r, const typename Vec2<R>::T *pold, typename Vec2<R>::T *pnew,
-                         typename Vec2<R>::T *Ap, double2 *part, unsigned *counters);
+                         typename Vec2<R>::T *Ap, double2 *part);
 
 template <>
 inline void launch_fused<double>(const Ctx &c, int ti, int nsys, bool dag1, const double *phi, CgState *st,
-                                 const double2 *r, const double2 *pold, double2 *pnew, double2 *Ap, double2 *part,
-                                 unsigned *counters) {
+                                 const double2 *r, const double2 *pold, double2 *pnew, double2 *Ap, double2 *part) {
     const TileShape t = kTiles64[ti];
     const FusedArgs A = fused_args(c, t.TT, t.TX);
     switch (ti) {
-    case 0: launch_fused_tt<double, 8, 32>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
-    case 1: launch_fused_tt<double, 4, 64>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
-    case 2: launch_fused_tt<double, 16, 16>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
-    default: launch_fused_tt<double, 4, 32>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
+    case 0: launch_fused_tt<double, 8, 32>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
+    case 1: launch_fused_tt<double, 4, 64>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
+    case 2: launch_fused_tt<double, 16, 16>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
+    default: launch_fused_tt<double, 4, 32>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
     }
 }
 
 template <>
 inline void launch_fused<float>(const Ctx &c, int ti, int nsys, bool dag1, const float *phi, CgState *st,
-                                const float2 *r, const float2 *pold, float2 *pnew, float2 *Ap, double2 *part,
-                                unsigned *counters) {
+                                const float2 *r, const float2 *pold, float2 *pnew, float2 *Ap, double2 *part) {
     const TileShape t = kTiles32[ti];
     const FusedArgs A = fused_args(c, t.TT, t.TX);
     switch (ti) {
-    case 0: launch_fused_tt<float, 8, 32>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
-    case 1: launch_fused_tt<float, 4, 64>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
-    case 2: launch_fused_tt<float, 16, 16>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
-    case 3: launch_fused_tt<float, 8, 64>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
-    case 4: launch_fused_tt<float, 16, 32>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
-    default: launch_fused_tt<float, 4, 16>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part, counters); break;
+    case 0: launch_fused_tt<float, 8, 32>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
+    case 1: launch_fused_tt<float, 4, 64>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
+    case 2: launch_fused_tt<float, 16, 16>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
+    case 3: launch_fused_tt<float, 8, 64>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
+    case 4: launch_fused_tt<float, 16, 32>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
+    default: launch_fused_tt<float, 4, 16>(c.stream, nsys, dag1, A, phi, st, r, pold, pnew, Ap, part); break;
     }
 }
 

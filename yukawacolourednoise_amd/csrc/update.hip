@@ -23,13 +23,11 @@ __device__ __forceinline__ double bosonic_drift(const double *__restrict__ ph, i
     return pot - lap;
 }
 
-// adaptive path, pass 1: total drift + max |drift| -> eps (last block per replica)
+// adaptive path, pass 1: total drift, one partial max |drift| per block
 __global__ void __launch_bounds__(kUT)
 k_bosonic_max(const double *__restrict__ phi, double *__restrict__ drift, const double *__restrict__ m2, double lam6,
-              int Nt, int Nx, double epsBar, int adaptive, double *__restrict__ part, unsigned *__restrict__ counters,
-              StepState *__restrict__ st) {
+              int Nt, int Nx, double *__restrict__ part) {
     __shared__ double lds[kUT / kWave];
-    __shared__ bool last;
     const int rep = blockIdx.y;
     const int V = Nt * Nx;
     const int s = blockIdx.x * kUT + threadIdx.x;
@@ -41,28 +39,26 @@ k_bosonic_max(const double *__restrict__ phi, double *__restrict__ drift, const 
         mx = fabs(d);
     }
     mx = block_max<kUT>(mx, lds);
-    const int nb = gridDim.x;
-    if (threadIdx.x == 0) {
-        part[(size_t)rep * nb + blockIdx.x] = mx;
-        last = arrive_last(&counters[rep], nb);
-    }
-    __syncthreads();
-    if (!last) return;
+    if (threadIdx.x == 0) part[(size_t)rep * gridDim.x + blockIdx.x] = mx;
+}
+
+// adaptive path, finalize (one block per replica): clamp max |drift| to [1e-5, 1e5]
+// (langevin_gpu_v2.cu:248-256), eps = epsBar * 30 / max (:277), Langevin time
+__global__ void __launch_bounds__(kUT)
+k_fin_eps(const double *__restrict__ part, int nb, double epsBar, int adaptive, StepState *__restrict__ st) {
+    __shared__ double lds[kUT / kWave];
+    const int rep = blockIdx.x;
     double m = 0.0;
-    for (int i = threadIdx.x; i < nb; i += kUT) m = fmax(m, load_coherent(part + (size_t)rep * nb + i));
-    __syncthreads();
+    for (int i = threadIdx.x; i < nb; i += kUT) m = fmax(m, part[(size_t)rep * nb + i]);
     m = block_max<kUT>(m, lds);
-    if (threadIdx.x == 0) {
-        StepState &S = st[rep];
-        // clamp (langevin_gpu_v2.cu:248-256)
-        if (m > 1e5) { m = 1e5; S.clamps += 1; }
-        else if (m < 1e-5) { m = 1e-5; S.clamps += 1; }
-        const double e = adaptive ? epsBar * 30.0 / m : epsBar;
-        S.maxDrift = m;
-        S.eps = e;
-        S.tL += e;
-        counters[rep] = 0u;
-    }
+    if (threadIdx.x) return;
+    StepState &S = st[rep];
+    if (m > 1e5) { m = 1e5; S.clamps += 1; }
+    else if (m < 1e-5) { m = 1e-5; S.clamps += 1; }
+    const double e = adaptive ? epsBar * 30.0 / m : epsBar;
+    S.maxDrift = m;
+    S.eps = e;
+    S.tL += e;
 }
 
 // adaptive path, pass 2: phi -= eps drift + sqrt(2 eps) noise (Evolve, :29-37)
@@ -157,13 +153,15 @@ int bosonic_update(Ctx &c, int fixedEps) {
         c.cur ^= 1;
         return YCN_OK;
     }
-    // partials: reuse the CG partial buffer (nRep * blocks doubles fit: nSys >= nRep)
+    // partials: the CG partial buffer holds >= nRep * blocks doubles
     double *part = reinterpret_cast<double *>(c.partials);
+    const int nb = (int)g.x;
     prof_before(c, PK_BOSONIC, (double)c.nRep * V * 24);
     hipLaunchKernelGGL(k_bosonic_max, g, dim3(kUT), 0, c.stream, (const double *)c.phi[c.cur], c.drift,
-                       (const double *)c.m2_d, lam6, Nt, Nx, c.epsBar, c.p.adaptive, part, c.counters + c.nSys,
-                       c.step_d);
+                       (const double *)c.m2_d, lam6, Nt, Nx, part);
     prof_after(c, PK_BOSONIC);
+    hipLaunchKernelGGL(k_fin_eps, dim3(c.nRep), dim3(kUT), 0, c.stream, (const double *)part, nb, c.epsBar,
+                       c.p.adaptive, c.step_d);
     hipLaunchKernelGGL(k_evolve, g, dim3(kUT), 0, c.stream, c.phi[c.cur], (const double *)c.drift,
                        (const double *)c.noise, V, (const StepState *)c.step_d);
     YCN_HIP(hipGetLastError());

@@ -110,21 +110,6 @@ __device__ __forceinline__ double block_max(double v, double *lds) {
     return v;
 }
 
-// "Last block" completion: thread 0 publishes its partial, bumps the per-group
-// counter with acquire-release semantics at agent scope and learns whether it was
-// the last of `nblocks`.  The last block then reads all partials (coherent loads)
-// and must reset the counter.
-__device__ __forceinline__ bool arrive_last(unsigned *counter, unsigned nblocks) {
-    const unsigned prev = __hip_atomic_fetch_add(counter, 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-    return prev == nblocks - 1;
-}
-__device__ __forceinline__ double load_coherent(const double *p) {
-    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void store_coherent(double *p, double v) {
-    __hip_atomic_store(p, v, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-
 // ---------------------------------------------------------------------------
 // Wilson-Dirac hopping on one site (code/src/Dirac.cu:49-70,144-190).
 //   D   : T_b = s1 psi(t-1), T_f = s0 psi(t+1), X_f = psi(x+1), X_b = psi(x-1)

@@ -79,7 +79,6 @@ struct Ctx {
     double2 *eta = nullptr, *x = nullptr, *r = nullptr, *pv[2] = {nullptr, nullptr}, *t = nullptr, *Ap = nullptr;
     CgState *cg_d = nullptr;
     double2 *partials = nullptr;          // [nSys][maxBlocks]
-    unsigned *counters = nullptr;         // [nSys + nRep] last-block counters
     int *active = nullptr;                // systems not yet converged
     int *active_h = nullptr;              // pinned host mirror
     int maxBlocks = 0;
