@@ -79,13 +79,42 @@ def cpu_baseline(gpu_iters_per_solve: float, budget_s: float = 20.0):
     }
 
 
+def run_method(Y, size, method, steps, warmup, profile_kernel):
+    """Time `steps` Langevin steps of one CG method on the workload; returns a dict."""
+    wl = dict(WORKLOAD)
+    wl["Nt"] = wl["Nx"] = size
+    p = Y.default_params(**wl)
+    p.cgPrecision = Y.YCN_CG_MIXED if method == "mixed" else Y.YCN_CG_FP64
+    p.reliableEvery = 50
+    ctx = Y.Context(p)
+    tuning = ctx.tuning_report()
+    ctx.hot_start(7)
+    step = 0
+    for _ in range(warmup):
+        ctx.langevin_step(step)
+        step += 1
+    ctx.sync()
+    if profile_kernel:
+        ctx.profile_begin(profile_kernel)
+    t0 = time.perf_counter()
+    st = ctx.langevin_run(step, steps)
+    ctx.sync()
+    dt = time.perf_counter() - t0
+    prof = ctx.profile_end() if profile_kernel else (0.0, 0, 0.0)
+    ctx.close()
+    return {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
+            "cg_iters_per_solve": st["iters"] / max(1, steps), "true_rel_resid_max": st["true_rel_resid"],
+            "reliable_updates_or_restarts": st["restarts"], "tuning": tuning, "prof": prof}
+
+
 def main():
     ap = argparse.ArgumentParser()
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=1024, help="Nt = Nx (default: the 1024^2 workload)")
-    ap.add_argument("--profile-kernel", default="cg_update")
+    ap.add_argument("--method", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--profile-kernel", default="cg_normal_op")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     args = ap.parse_args()
     if args.gpus != 1:
@@ -95,56 +124,45 @@ def main():
 
     import yukawacolourednoise_amd as Y
 
-    wl = dict(WORKLOAD)
-    wl["Nt"] = wl["Nx"] = args.size
-    p = Y.default_params(**wl)
-    ctx = Y.Context(p)
-    ctx.hot_start(7)
-    step = 0
-    warm_iters = []
-    for _ in range(args.warmup):
-        r = ctx.langevin_step(step)
-        warm_iters.append(r.stats["iters"])
-        step += 1
-    ctx.sync()
-
-    ctx.profile_begin(args.profile_kernel)
-    t0 = time.perf_counter()
-    st = ctx.langevin_run(step, args.steps)
-    ctx.sync()
-    dt = time.perf_counter() - t0
-    k_ms, k_n, k_bytes = ctx.profile_end()
-    step += args.steps
-    ctx.close()
-
-    steps_per_s = args.steps / dt
-    iters_per_solve = st["iters"] / max(1, args.steps)
+    methods = ["fp64", "mixed"] if args.method == "both" else [args.method]
+    res = {}
+    for m in methods:
+        # the dominant kernel is profiled in the headline (last) method only
+        res[m] = run_method(Y, args.size, m, args.steps, args.warmup,
+                            args.profile_kernel if m == methods[-1] else None)
+        print(f"# {m}: {res[m]['steps_per_s']:.3f} steps/s, {res[m]['cg_iters_per_solve']:.0f} CG iters/solve",
+              file=sys.stderr, flush=True)
+    head = methods[-1]
+    h = res[head]
+    k_ms, k_n, k_bytes = h["prof"]
     avg_ms = k_ms / max(1, k_n)
     achieved = (k_bytes / 1e9) / (avg_ms / 1e3) if k_n else 0.0
+    cg_desc = {"fp64": "plain FP64 CG", "mixed": "mixed-precision CG (FP32 inner, FP64 reliable update every 50)"}
     out = {
         "metric": "Langevin steps/s @1024^2 (true resid <= 1e-10)",
-        "value": steps_per_s,
+        "value": h["steps_per_s"],
         "unit": "steps/s",
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": 1e3 * dt / args.steps,
+        "ms_per_step": h["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64",
+        "dtype": "f64" if head == "fp64" else "f32+f64",
         "data": "synthetic (Philox hot start, seed 7)",
         "config": {
-            "workload": f"{args.size}x{args.size} near-critical Langevin step, plain FP64 CG on D^dag D "
-                        "to true relative residual 1e-10, 1 noise vector, fixed eps 0.005",
-            "Nt": args.size, "Nx": args.size, "M": wl["M"], "g": wl["g"], "m2": wl["m2"],
-            "lambda": wl["lam"], "s": wl["cutFraction"], "eps": wl["epsBar"],
-            "cg": "fp64", "precond": "none", "parallelism": "single GPU",
+            "workload": f"{args.size}x{args.size} near-critical Langevin step, {cg_desc[head]} on D^dag D "
+                        "to FP64 true relative residual 1e-10, 1 noise vector, fixed eps 0.005",
+            "Nt": args.size, "Nx": args.size, "M": WORKLOAD["M"], "g": WORKLOAD["g"], "m2": WORKLOAD["m2"],
+            "lambda": WORKLOAD["lam"], "s": WORKLOAD["cutFraction"], "eps": WORKLOAD["epsBar"],
+            "cg": head, "precond": "none", "parallelism": "single GPU",
         },
-        "cg_iters_per_solve": iters_per_solve,
-        "cg_true_rel_resid_max": st["true_rel_resid"],
+        "cg_iters_per_solve": h["cg_iters_per_solve"],
+        "cg_true_rel_resid_max": h["true_rel_resid_max"],
+        "methods": {m: {k: v for k, v in r.items() if k != "prof"} for m, r in res.items()},
         "roofline": {
-            "kernel": args.profile_kernel,
+            "kernel": args.profile_kernel + (" (fp32)" if head == "mixed" else " (fp64)"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -156,8 +174,10 @@ def main():
             "algorithmic_bytes_per_launch": k_bytes,
         },
     }
+    if "fp64" in res and head != "fp64":
+        out["speedup_vs_plain_fp64"] = h["steps_per_s"] / res["fp64"]["steps_per_s"]
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(iters_per_solve)
+        out["cpu_baseline"] = cpu_baseline(res.get("fp64", h)["cg_iters_per_solve"])
     print(json.dumps(out), flush=True)
 
 

@@ -164,11 +164,40 @@ int ycn_measure(ycn_ctx *ctx, int replica, uint64_t step, ycn_observables *obs,
 /* Wait for all work on the context stream. */
 int ycn_sync(ycn_ctx *ctx);
 
+/* ---- learned preconditioner (north-star addition; no reference counterpart) ----
+ * A 3-layer periodic 3x3 CNN, 8 -> 16 -> 16 -> 8 channels (channel 2c / 2c+1 = Re / Im
+ * of spinor component c), no bias, tanh between layers, BF16 weights and activations
+ * with FP32 accumulation on MFMA.  z = s * net(r / s), s = RMS of r's 8V real entries.
+ * Weights (YCN_CNN_NWEIGHTS floats): W1[9][8][16], W2[9][16][16], W3[9][16][8] as
+ * [tap][in][out], tap = (dt+1)*3 + (dx+1), out(site) = sum W * in(site + (dt,dx)).
+ * With precond = YCN_PRECOND_CNN the fermion-force solves run flexible CG with this
+ * network (FP32 inner iterations, FP64 reliable updates, plain CG on stall); the FP64
+ * true residual with the plain operator decides convergence.  Needs Nt, Nx % 16 == 0. */
+#define YCN_CNN_NWEIGHTS 4608
+typedef struct {
+    int pairs;             /* (residual, error) pairs trained on                        */
+    double loss_first;     /* energy-norm loss of the first pair (before its update)    */
+    double loss_last;      /* loss of the last pair                                     */
+} ycn_train_stats;
+int ycn_precond_set_weights(ycn_ctx *ctx, const float *w);
+int ycn_precond_get_weights(ycn_ctx *ctx, float *w);
+/* z = CNN(r) for one AoS spinor (host arrays, 8V doubles each) */
+int ycn_precond_apply(ycn_ctx *ctx, const double *r, double *z);
+/* loss L = <e - z, A (e - z)> / <e, r> (A = D^dag D at phi, or the replica-0 field if
+ * phi is NULL) for z = CNN(r), and dL/dW (YCN_CNN_NWEIGHTS floats); no update */
+int ycn_precond_loss_grad(ycn_ctx *ctx, const double *phi, const double *r, const double *e, double *loss,
+                          float *grad);
+/* Online training: nsteps Langevin steps from counter step0 (they advance the chain);
+ * each fermion solve leaves pairsPerSolve (r_k, e_k = x* - x_k) pairs, each one Adam
+ * step with learning rate lr.  nReplicas = nNoise = 1. */
+int ycn_precond_train(ycn_ctx *ctx, uint64_t step0, int nsteps, int pairsPerSolve, double lr,
+                      ycn_train_stats *stats);
+
 /* Kernel timing with HIP events on the context stream (for roofline figures):
  * between begin and end every launch of the named kernel is bracketed by events;
  * end returns the summed duration, the launch count and the algorithmic bytes
  * moved per launch.  Kernel names: "cg_normal_op", "cg_update", "dirac",
- * "noise_cols", "bosonic". */
+ * "noise_cols", "bosonic", "cnn". */
 int ycn_profile_begin(ycn_ctx *ctx, const char *kernel);
 int ycn_profile_end(ycn_ctx *ctx, double *total_ms, int64_t *launches, double *bytes_per_launch);
 

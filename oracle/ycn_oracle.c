@@ -576,3 +576,133 @@ int orc_correlator(const orc_params *p, const double *phi, double *corr, orc_cg_
     free(in); free(out);
     return rc;
 }
+
+/* ========================================================================== */
+/* learned preconditioner (CPU FP64 oracle of the CNN; no reference counterpart) */
+/* ========================================================================== */
+double orc_bf16_round(double v) {
+    float f = (float)v;
+    uint32_t u;
+    memcpy(&u, &f, 4);
+    if ((u & 0x7f800000u) == 0x7f800000u) return (double)f;   /* inf / nan */
+    uint32_t lsb = (u >> 16) & 1u;
+    u = (u + 0x7fffu + lsb) & 0xffff0000u;
+    memcpy(&f, &u, 4);
+    return (double)f;
+}
+
+static const int cnn_woff[3] = {0, 9 * 8 * 16, 9 * 8 * 16 + 9 * 16 * 16};
+
+/* y[site][co] = sum_tap sum_ci W[tap][ci][co] u[site + off(tap)][ci], periodic */
+static void conv3x3(int Nt, int Nx, const double *W, int cin, int cout, const double *u, double *y) {
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < Nt * Nx; ++s) {
+        int t = s / Nx, x = s % Nx;
+        for (int co = 0; co < cout; ++co) y[(size_t)s * cout + co] = 0.0;
+        for (int tap = 0; tap < 9; ++tap) {
+            int dt = tap / 3 - 1, dx = tap % 3 - 1;
+            int s2 = ((t + dt + Nt) % Nt) * Nx + (x + dx + Nx) % Nx;
+            for (int ci = 0; ci < cin; ++ci) {
+                double a = u[(size_t)s2 * cin + ci];
+                const double *w = W + ((size_t)tap * cin + ci) * cout;
+                for (int co = 0; co < cout; ++co) y[(size_t)s * cout + co] += w[co] * a;
+            }
+        }
+    }
+}
+
+/* transposed: gu[site][ci] = sum_tap sum_co W[tap][ci][co] gy[site - off(tap)][co] */
+static void conv3x3_t(int Nt, int Nx, const double *W, int cin, int cout, const double *gy, double *gu) {
+#pragma omp parallel for schedule(static)
+    for (int s = 0; s < Nt * Nx; ++s) {
+        int t = s / Nx, x = s % Nx;
+        for (int ci = 0; ci < cin; ++ci) gu[(size_t)s * cin + ci] = 0.0;
+        for (int tap = 0; tap < 9; ++tap) {
+            int dt = tap / 3 - 1, dx = tap % 3 - 1;
+            int s2 = ((t - dt + Nt) % Nt) * Nx + (x - dx + Nx) % Nx;
+            for (int ci = 0; ci < cin; ++ci) {
+                const double *w = W + ((size_t)tap * cin + ci) * cout;
+                double acc = 0.0;
+                for (int co = 0; co < cout; ++co) acc += w[co] * gy[(size_t)s2 * cout + co];
+                gu[(size_t)s * cin + ci] += acc;
+            }
+        }
+    }
+}
+
+/* gW[tap][ci][co] += sum_site u[site + off(tap)][ci] gy[site][co] */
+static void conv3x3_wgrad(int Nt, int Nx, int cin, int cout, const double *u, const double *gy, double *gW) {
+    for (int tap = 0; tap < 9; ++tap) {
+        int dt = tap / 3 - 1, dx = tap % 3 - 1;
+        for (int ci = 0; ci < cin; ++ci)
+            for (int co = 0; co < cout; ++co) {
+                double acc = 0.0;
+                for (int s = 0; s < Nt * Nx; ++s) {
+                    int t = s / Nx, x = s % Nx;
+                    int s2 = ((t + dt + Nt) % Nt) * Nx + (x + dx + Nx) % Nx;
+                    acc += u[(size_t)s2 * cin + ci] * gy[(size_t)s * cout + co];
+                }
+                gW[((size_t)tap * cin + ci) * cout + co] += acc;
+            }
+    }
+}
+
+/* forward pass keeping the activations; returns the input scale s */
+static double cnn_fwd(int Nt, int Nx, const float *Wf, int bf16, const double *r, double *u0, double *y1,
+                      double *u1, double *y2, double *u2, double *y3) {
+    const size_t V = (size_t)Nt * Nx;
+    double W[ORC_CNN_NW];
+    for (int i = 0; i < ORC_CNN_NW; ++i) W[i] = bf16 ? orc_bf16_round(Wf[i]) : (double)Wf[i];
+    double ss = 0.0;
+    for (size_t i = 0; i < 8 * V; ++i) ss += r[i] * r[i];
+    double s = sqrt(ss / (double)(8 * V));
+    double inv = s > 0.0 ? 1.0 / s : 0.0;
+    for (size_t i = 0; i < 8 * V; ++i) u0[i] = bf16 ? orc_bf16_round(r[i] * inv) : r[i] * inv;
+    conv3x3(Nt, Nx, W + cnn_woff[0], 8, 16, u0, y1);
+    for (size_t i = 0; i < 16 * V; ++i) u1[i] = bf16 ? orc_bf16_round(tanh(y1[i])) : tanh(y1[i]);
+    conv3x3(Nt, Nx, W + cnn_woff[1], 16, 16, u1, y2);
+    for (size_t i = 0; i < 16 * V; ++i) u2[i] = bf16 ? orc_bf16_round(tanh(y2[i])) : tanh(y2[i]);
+    conv3x3(Nt, Nx, W + cnn_woff[2], 16, 8, u2, y3);
+    return s;
+}
+
+void orc_cnn_forward(int Nt, int Nx, const float *W, int bf16, const double *r, double *z) {
+    const size_t V = (size_t)Nt * Nx;
+    double *u0 = malloc(sizeof(double) * 8 * V), *y1 = malloc(sizeof(double) * 16 * V);
+    double *u1 = malloc(sizeof(double) * 16 * V), *y2 = malloc(sizeof(double) * 16 * V);
+    double *u2 = malloc(sizeof(double) * 16 * V), *y3 = malloc(sizeof(double) * 8 * V);
+    double s = cnn_fwd(Nt, Nx, W, bf16, r, u0, y1, u1, y2, u2, y3);
+    for (size_t i = 0; i < 8 * V; ++i) z[i] = s * y3[i];
+    free(u0); free(y1); free(u1); free(y2); free(u2); free(y3);
+}
+
+double orc_cnn_loss_grad(const orc_params *p, const double *phi, const float *W, const double *r,
+                         const double *e, double *grad) {
+    const int Nt = p->Nt, Nx = p->Nx;
+    const size_t V = (size_t)Nt * Nx;
+    double *u0 = malloc(sizeof(double) * 8 * V), *y1 = malloc(sizeof(double) * 16 * V);
+    double *u1 = malloc(sizeof(double) * 16 * V), *y2 = malloc(sizeof(double) * 16 * V);
+    double *u2 = malloc(sizeof(double) * 16 * V), *y3 = malloc(sizeof(double) * 8 * V);
+    double *z = malloc(sizeof(double) * 8 * V), *Az = malloc(sizeof(double) * 8 * V);
+    double *g3 = malloc(sizeof(double) * 8 * V), *g2 = malloc(sizeof(double) * 16 * V);
+    double *g1 = malloc(sizeof(double) * 16 * V);
+    double s = cnn_fwd(Nt, Nx, W, 0, r, u0, y1, u1, y2, u2, y3);
+    /* d = e - z; L = <d, A d> / <e, r>; dL/dz = -2 A d / <e, r>; dL/dy3 = s dL/dz */
+    for (size_t i = 0; i < 8 * V; ++i) z[i] = e[i] - s * y3[i];
+    orc_normal_op(p, phi, z, Az, 1);
+    double er = 0.0, L = 0.0;
+    for (size_t i = 0; i < 8 * V; ++i) { er += e[i] * r[i]; L += z[i] * Az[i]; }
+    L /= er;
+    double Wd[ORC_CNN_NW];
+    for (int i = 0; i < ORC_CNN_NW; ++i) { Wd[i] = (double)W[i]; grad[i] = 0.0; }
+    for (size_t i = 0; i < 8 * V; ++i) g3[i] = -2.0 * s * Az[i] / er;
+    conv3x3_wgrad(Nt, Nx, 16, 8, u2, g3, grad + cnn_woff[2]);
+    conv3x3_t(Nt, Nx, Wd + cnn_woff[2], 16, 8, g3, g2);
+    for (size_t i = 0; i < 16 * V; ++i) g2[i] *= 1.0 - u2[i] * u2[i];
+    conv3x3_wgrad(Nt, Nx, 16, 16, u1, g2, grad + cnn_woff[1]);
+    conv3x3_t(Nt, Nx, Wd + cnn_woff[1], 16, 16, g2, g1);
+    for (size_t i = 0; i < 16 * V; ++i) g1[i] *= 1.0 - u1[i] * u1[i];
+    conv3x3_wgrad(Nt, Nx, 8, 16, u0, g1, grad + cnn_woff[0]);
+    free(u0); free(y1); free(u1); free(y2); free(u2); free(y3); free(z); free(Az); free(g3); free(g2); free(g1);
+    return L;
+}

@@ -1,0 +1,67 @@
+"""Pins the CPU FP64 oracle of the learned preconditioner: the analytic gradient of the
+energy-norm loss against central finite differences, scale equivariance, the identity
+network, and bfloat16 round-to-nearest-even."""
+import numpy as np
+import pytest
+
+import oracle as O
+
+
+def test_bf16_rounding():
+    L = O.lib()
+    assert L.orc_bf16_round(1.0) == 1.0
+    assert L.orc_bf16_round(1.0 + 2 ** -8) == 1.0            # tie -> even
+    assert L.orc_bf16_round(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -6 or L.orc_bf16_round(1.0 + 3 * 2 ** -8) == 1.0 + 2 ** -7
+    assert L.orc_bf16_round(1.0 + 2 ** -7) == 1.0 + 2 ** -7   # representable
+    assert abs(L.orc_bf16_round(np.pi) - np.pi) / np.pi < 2 ** -8
+
+
+def test_identity_network():
+    """Centre taps as identity and tiny weights elsewhere: net(u) ~ u in the linear
+    range of tanh (W1 = a I, W2 = I, W3 = I/a)."""
+    Nt, Nx = 4, 6
+    W = np.zeros(O.CNN_NW, np.float32)
+    a = 1e-3
+    for ci in range(8):
+        W[(4 * 8 + ci) * 16 + ci] = a
+    for ci in range(16):
+        W[9 * 8 * 16 + (4 * 16 + ci) * 16 + ci] = 1.0
+    for ci in range(8):
+        W[9 * 8 * 16 + 9 * 16 * 16 + (4 * 16 + ci) * 8 + ci] = 1.0 / a
+    r = np.random.default_rng(0).standard_normal((Nt * Nx, 4)) + 0j
+    z = O.cnn_forward(Nt, Nx, W, r, bf16=False)
+    assert np.abs(z - r).max() < 1e-5 * np.abs(r).max()
+
+
+def test_scale_equivariance():
+    Nt, Nx = 4, 6
+    rng = np.random.default_rng(1)
+    W = (0.3 * rng.standard_normal(O.CNN_NW)).astype(np.float32)
+    r = rng.standard_normal((Nt * Nx, 4)) + 1j * rng.standard_normal((Nt * Nx, 4))
+    z1 = O.cnn_forward(Nt, Nx, W, r, bf16=False)
+    z2 = O.cnn_forward(Nt, Nx, W, 1e-6 * r, bf16=False)
+    assert np.abs(z2 - 1e-6 * z1).max() < 1e-12 * np.abs(z1).max()
+
+
+@pytest.mark.parametrize("seed", [0, 1])
+def test_gradient_finite_differences(seed):
+    Nt, Nx = 4, 6
+    V = Nt * Nx
+    rng = np.random.default_rng(seed)
+    W = (0.2 * rng.standard_normal(O.CNN_NW)).astype(np.float32)
+    p = O.params(Nt=Nt, Nx=Nx, M=0.3, g=0.5)
+    phi = rng.uniform(-1, 1, (Nt, Nx))
+    e = rng.standard_normal((V, 4)) + 1j * rng.standard_normal((V, 4))
+    r = O.normal_op(p, phi, e, 1)
+    L, g = O.cnn_loss_grad(p, phi, W, r, e)
+    # one weight in every layer, including the first tap and channel edges
+    for i in [0, 5, 1151, 1152, 2000, 3455, 3456, 4000, 4607] + list(rng.choice(O.CNN_NW, 6, replace=False)):
+        h = np.float32(1e-3)
+        Wp, Wm = W.copy(), W.copy()
+        Wp[i] += h
+        Wm[i] -= h
+        fd = (O.cnn_loss_grad(p, phi, Wp, r, e)[0] - O.cnn_loss_grad(p, phi, Wm, r, e)[0]) / (float(Wp[i]) - float(Wm[i]))
+        assert abs(g[i] - fd) <= 1e-4 * max(1.0, abs(fd)), (i, g[i], fd)
+    # with z = 0 (all weights zero) the loss is exactly 1 for a consistent pair
+    L0, _ = O.cnn_loss_grad(p, phi, np.zeros(O.CNN_NW, np.float32), r, e)
+    assert abs(L0 - 1.0) < 1e-12

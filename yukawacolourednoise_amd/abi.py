@@ -39,7 +39,10 @@ EXPORTED = [
     "ycn_coloured_noise", "ycn_coloured_filter", "ycn_dirac_apply", "ycn_cg_solve",
     "ycn_fermion_force", "ycn_langevin_step", "ycn_langevin_run", "ycn_measure", "ycn_sync",
     "ycn_profile_begin", "ycn_profile_end",
+    "ycn_precond_set_weights", "ycn_precond_get_weights", "ycn_precond_apply", "ycn_precond_loss_grad",
+    "ycn_precond_train",
 ]
+YCN_CNN_NWEIGHTS = 4608
 
 
 class Params(C.Structure):
@@ -66,6 +69,13 @@ class CgStats(C.Structure):
 class Observables(C.Structure):
     _fields_ = [("magnetisation", C.c_double), ("phi2", C.c_double), ("trace", C.c_double),
                 ("eps", C.c_double), ("langevin_time", C.c_double), ("clamp_events", C.c_int)]
+
+    def as_dict(self):
+        return {k: getattr(self, k) for k, _ in self._fields_}
+
+
+class TrainStats(C.Structure):
+    _fields_ = [("pairs", C.c_int), ("loss_first", C.c_double), ("loss_last", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}
@@ -110,6 +120,12 @@ def _load():
     L.ycn_sync.argtypes = [vp]
     L.ycn_profile_begin.argtypes = [vp, C.c_char_p]
     L.ycn_profile_end.argtypes = [vp, dp, C.POINTER(C.c_int64), dp]
+    fp = C.POINTER(C.c_float)
+    L.ycn_precond_set_weights.argtypes = [vp, fp]
+    L.ycn_precond_get_weights.argtypes = [vp, fp]
+    L.ycn_precond_apply.argtypes = [vp, dp, dp]
+    L.ycn_precond_loss_grad.argtypes = [vp, dp, dp, dp, dp, fp]
+    L.ycn_precond_train.argtypes = [vp, C.c_uint64, C.c_int, C.c_int, C.c_double, C.POINTER(TrainStats)]
     return L
 
 
@@ -260,6 +276,37 @@ class Context:
         self._check(lib.ycn_measure(self._h, replica, step, C.byref(ob), None if sl is None else _ptr(sl),
                                     None if cr is None else _ptr(cr)))
         return ob.as_dict(), sl, cr
+
+    # -- learned preconditioner ------------------------------------------------
+    def precond_set_weights(self, w):
+        w = np.ascontiguousarray(w, dtype=np.float32)
+        assert w.size == YCN_CNN_NWEIGHTS
+        self._check(lib.ycn_precond_set_weights(self._h, w.ctypes.data_as(C.POINTER(C.c_float))))
+
+    def precond_get_weights(self):
+        w = np.empty(YCN_CNN_NWEIGHTS, np.float32)
+        self._check(lib.ycn_precond_get_weights(self._h, w.ctypes.data_as(C.POINTER(C.c_float))))
+        return w
+
+    def precond_apply(self, r):
+        r = _spin(r)
+        out = np.empty(8 * self.V, np.float64)
+        self._check(lib.ycn_precond_apply(self._h, _ptr(r), _ptr(out)))
+        return out.view(np.complex128).reshape(self.V, 4)
+
+    def precond_loss_grad(self, r, e, phi=None):
+        r, e = _spin(r), _spin(e)
+        ph = None if phi is None else _f64(phi).reshape(-1)
+        loss = C.c_double()
+        g = np.empty(YCN_CNN_NWEIGHTS, np.float32)
+        self._check(lib.ycn_precond_loss_grad(self._h, None if ph is None else _ptr(ph), _ptr(r), _ptr(e),
+                                              C.byref(loss), g.ctypes.data_as(C.POINTER(C.c_float))))
+        return loss.value, g
+
+    def precond_train(self, step0, nsteps, pairs_per_solve=4, lr=1e-3):
+        st = TrainStats()
+        self._check(lib.ycn_precond_train(self._h, step0, nsteps, pairs_per_solve, lr, C.byref(st)))
+        return st.as_dict()
 
     def tuning_report(self):
         return lib.ycn_tuning_report(self._h).decode()

@@ -122,6 +122,9 @@ k_fin_setup(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, 
     S.iter = 0;
     S.restarts = 0;
     S.true_rel = 0.0;
+    S.rz = 0.0;
+    S.true_prev = 1.0;
+    S.precOff = 0;
     const bool run = (0 < maxIter) && (sqrt(tot.x) > S.stop);
     S.done = run ? 0 : 1;
     S.final_ = run ? 0 : 1;
@@ -189,7 +192,10 @@ k_fin_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, i
 }
 
 // mixed precision reliable update decision: converged on the FP64 true residual, or
-// resume the FP32 loop keeping the direction (beta = |r_true|^2 / |r_prev|^2)
+// resume the FP32 loop.  Mixed CG keeps the direction (beta = |r_true|^2 / |r_prev|^2);
+// flexible CG restarts it (beta = 0, z recomputed from the new residual) and switches
+// the preconditioner off when a reliable interval did not reduce the true residual.
+template <bool FCG>
 __global__ void __launch_bounds__(kNT)
 k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active) {
     __shared__ double2 lds[kNT / kWave];
@@ -207,11 +213,55 @@ k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
         S.converged = conv ? 1 : 0;
         S.final_ = 1;
         if (!S.done) { S.done = 1; atomicSub(active, 1); }
+        return;
+    }
+    if (FCG) {
+        if (!(S.true_rel < S.true_prev)) S.precOff = 1;   // stall: plain CG from here on
+        S.true_prev = S.true_rel;
+        S.beta = 0.0;
     } else {
         S.beta = rrt / S.rr_prev;
-        S.rr = rrt;
-        if (S.done) { S.done = 0; atomicAdd(active, 1); }
     }
+    S.rr = rrt;
+    if (S.done) { S.done = 0; atomicAdd(active, 1); }
+}
+
+// flexible CG: alpha = <r, z> / Re <p', Ap> (real, the system seen as real 8V-vectors)
+__global__ void __launch_bounds__(kNT) k_fin_alpha_fcg(const double2 *__restrict__ part, int nb, CgState *__restrict__ st) {
+    __shared__ double2 lds[kNT / kWave];
+    const int sys = blockIdx.x;
+    if (st[sys].done) return;
+    const double2 dot = fin_sum(part + (size_t)sys * nb, nb, lds);
+    if (threadIdx.x) return;
+    CgState &S = st[sys];
+    if (!(dot.x > 0.0) || !isfinite(S.rz) || !(S.rz > 0.0)) {
+        // the preconditioned direction is not a descent direction: no step, and plain
+        // CG from the next reliable update on
+        S.alpha = make_double2(0.0, 0.0);
+        S.precOff = 1;
+        return;
+    }
+    S.alpha = make_double2(S.rz / dot.x, 0.0);
+}
+
+// flexible CG after z = CNN(r'): partials hold (<z, r'>, <z, Ap>).  Polak-Ribiere form
+// (Notay, SIAM J. Sci. Comput. 22 (2000)): beta = <z, r' - r> / <z_old, r> with
+// r = r' + alpha Ap, i.e. beta = -alpha <z, Ap> / rz_old.  RESTART: beta = 0.
+template <bool RESTART>
+__global__ void __launch_bounds__(kNT) k_fin_fcg(const double2 *__restrict__ part, int nb, CgState *__restrict__ st) {
+    __shared__ double2 lds[kNT / kWave];
+    const int sys = blockIdx.x;
+    if (st[sys].done) return;
+    const double2 tot = fin_sum(part + (size_t)sys * nb, nb, lds);
+    if (threadIdx.x) return;
+    CgState &S = st[sys];
+    if (!isfinite(tot.x) || !isfinite(tot.y)) {
+        S.precOff = 1;
+        S.beta = 0.0;
+        return;
+    }
+    S.beta = (RESTART || !(S.rz > 0.0)) ? 0.0 : -S.alpha.x * tot.y / S.rz;
+    S.rz = tot.x;
 }
 
 // ---------------------------------------------------------------------------
@@ -750,7 +800,7 @@ int cg_autotune(Ctx &c) {
         c.tuneReport += " (forced by YCN_TILE)";
     }
     c.tile64 = bestTile;
-    if (c.p.cgPrecision == YCN_CG_MIXED) {
+    if (c.rs) {
         c.tuneReport += " | fp32 tiles (us/launch):";
         best = 1e30;
         int best32 = -1;
@@ -801,7 +851,10 @@ static int read_stats(Ctx &c, int nsys, SolveStats *st) {
 
 int cg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t streamBase, uint64_t step,
              SolveStats *st) {
-    if (c.p.cgPrecision == YCN_CG_MIXED) return mp_solve(c, phi, nsys, mtype, gen, streamBase, step, st);
+    if (c.p.precond == YCN_PRECOND_CNN && mtype == YCN_DAGGER)
+        return fcg_solve(c, phi, nsys, mtype, gen, streamBase, step, st);
+    if (c.p.cgPrecision == YCN_CG_MIXED || c.p.precond == YCN_PRECOND_CNN)
+        return mp_solve(c, phi, nsys, mtype, gen, streamBase, step, st);
     const LatticeArgs L = largs(c);
     const dim3 g = site_grid(c, nsys);
     const int nb = site_blocks(c);
@@ -950,8 +1003,8 @@ int mp_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
             hipLaunchKernelGGL((k_true_b<false, true>), g, dim3(kNT), 0, c.stream, L, phi, (const CgState *)c.cg_d,
                                (const double2 *)c.eta, (const double2 *)c.t, c.r, c.rs, c.partials);
         }
-        hipLaunchKernelGGL(k_fin_mp_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
-                           c.cg_d, c.active);
+        hipLaunchKernelGGL(k_fin_mp_true<false>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials,
+                           nb, c.cg_d, c.active);
         YCN_HIP(hipGetLastError());
         rc = poll_active(c, active);
         if (rc) return rc;
@@ -971,6 +1024,172 @@ int fermion_drift(Ctx &c, const double *phi, double *drift) {
 int spinor_slices(Ctx &c, const double2 *psi, double *out_dev) {
     hipLaunchKernelGGL(k_spinor_slices, dim3(c.p.Nt), dim3(kNT), 0, c.stream, psi, c.p.Nt, c.p.Nx, out_dev);
     YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+
+// ===========================================================================
+// Flexible CG with the learned preconditioner (north-star addition; no reference
+// counterpart): FP32 inner iterations as in the mixed solver, z = CNN(r_s) each
+// iteration, FP64 reliable updates every `reliableEvery` iterations that restart the
+// direction, and plain CG (identity preconditioner) once a reliable interval stalls.
+// The FP64 true residual with the plain operator decides convergence.
+// ===========================================================================
+static void launch_fcg_iteration(Ctx &c, int nsys, int mtype, int parity) {
+    const float2 *pold = c.ps[parity];
+    float2 *pnew = c.ps[parity ^ 1];
+    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (4 * 32 + 4));
+    launch_fused<float>(c, c.tile32, nsys, mtype != YCN_DAGGER, c.phis, c.cg_d, (const float2 *)c.zs, pold, pnew,
+                        c.Aps, c.partials);
+    prof_after(c, PK_NORMAL_OP);
+    const int nbF = fused_args(c, kTiles32[c.tile32].TT, kTiles32[c.tile32].TX).tilesPerSys;
+    hipLaunchKernelGGL(k_fin_alpha_fcg, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbF, c.cg_d);
+    prof_before(c, PK_CG_UPDATE, (double)nsys * c.V * 6 * 32);
+    hipLaunchKernelGGL(k_mp_update, site_grid(c, nsys), dim3(kNT), 0, c.stream, c.V, (const CgState *)c.cg_d, c.xs,
+                       c.rs, (const float2 *)pnew, (const float2 *)c.Aps, c.partials);
+    prof_after(c, PK_CG_UPDATE);
+    hipLaunchKernelGGL(k_fin_beta<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials,
+                       site_blocks(c), c.cg_d, c.active);
+    const int nbC = cnn_apply(c, nsys, c.cg_d, c.rs, c.Aps, c.zs, c.partials);
+    hipLaunchKernelGGL(k_fin_fcg<false>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC, c.cg_d);
+}
+
+int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t streamBase, uint64_t step,
+              SolveStats *st) {
+    if (c.tile32 < 0 || !c.zs) {
+        c.err = "the learned preconditioner needs Nt % 16 == 0 and Nx % 16 == 0";
+        return YCN_ERR_UNSUPPORTED;
+    }
+    const LatticeArgs L = largs(c);
+    const dim3 g = site_grid(c, nsys);
+    const int nb = site_blocks(c);
+    const int nrepUsed = (nsys + c.nNoise - 1) / c.nNoise;
+    const int64_t nconv = std::min<int64_t>((int64_t)c.nRep * c.V, (int64_t)nrepUsed * c.V);
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((nconv + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, phi, c.phis,
+                       nconv);
+    int rc = set_int(c, c.active, nsys);
+    if (rc) return rc;
+    const int maxIter = c.p.cgMaxIter;
+    if (gen)
+        hipLaunchKernelGGL(k_mp_setup<true>, g, dim3(kNT), 0, c.stream, L, (const uint64_t *)c.seed_d, streamBase,
+                           step, c.eta, c.x, c.xs, c.rs, c.ps[0], c.ps[1], c.partials);
+    else
+        hipLaunchKernelGGL(k_mp_setup<false>, g, dim3(kNT), 0, c.stream, L, (const uint64_t *)c.seed_d, streamBase,
+                           step, c.eta, c.x, c.xs, c.rs, c.ps[0], c.ps[1], c.partials);
+    hipLaunchKernelGGL(k_fin_setup, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb, c.cg_d,
+                       c.active, c.p.cgTol, c.p.cgTolMode, maxIter, 0);
+    // z = CNN(b), rz = <b, z>, p' = z on the first iteration
+    int nbC = cnn_apply(c, nsys, c.cg_d, c.rs, c.Aps, c.zs, c.partials);
+    if (nbC < 0) return nbC;
+    hipLaunchKernelGGL(k_fin_fcg<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC, c.cg_d);
+    YCN_HIP(hipGetLastError());
+    const int every = std::max(1, c.p.reliableEvery);
+    const int nIt = every + (every & 1);
+    const int hardCap = maxIter + nIt + 8;
+    int launched = 0, active = nsys;
+    rc = poll_active(c, active);
+    if (rc) return rc;
+    while (active > 0 && launched < hardCap) {
+        rc = run_chunk(c, phi, nsys, mtype, nIt, 2 /* flexible */, c.tile32,
+                       [&](int par) { launch_fcg_iteration(c, nsys, mtype, par); });
+        if (rc) return rc;
+        launched += nIt;
+        hipLaunchKernelGGL(k_mp_accum, g, dim3(kNT), 0, c.stream, c.V, (const CgState *)c.cg_d, c.x, c.xs);
+        hipLaunchKernelGGL(k_true_a<false>, g, dim3(kNT), 0, c.stream, L, phi, (const double2 *)c.x, c.t);
+        hipLaunchKernelGGL((k_true_b<true, true>), g, dim3(kNT), 0, c.stream, L, phi, (const CgState *)c.cg_d,
+                           (const double2 *)c.eta, (const double2 *)c.t, c.r, c.rs, c.partials);
+        hipLaunchKernelGGL(k_fin_mp_true<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
+                           c.cg_d, c.active);
+        nbC = cnn_apply(c, nsys, c.cg_d, c.rs, c.Aps, c.zs, c.partials);
+        if (nbC < 0) return nbC;
+        hipLaunchKernelGGL(k_fin_fcg<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC,
+                           c.cg_d);
+        YCN_HIP(hipGetLastError());
+        rc = poll_active(c, active);
+        if (rc) return rc;
+    }
+    return read_stats(c, nsys, st);
+}
+
+// e = x_final - x_snapshot
+__global__ void __launch_bounds__(kNT) k_sub(const double2 *__restrict__ a, double2 *__restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x;
+    if (i < n) b[i] = csub(a[i], b[i]);
+}
+
+// Online training on thermalisation: `nsteps` Langevin steps from counter step0 whose
+// fermion solves (FP64 CG, one system) leave `pairsPerSolve` snapshots (r_k, x_k) at
+// evenly spaced iterations; once a solve has converged each pair (r_k, e_k = x* - x_k)
+// drives one Adam step of the network.  The chain itself advances as usual.
+int train_preconditioner(Ctx &c, uint64_t step0, int nsteps, int pairsPerSolve, float lr, int *pairsUsed) {
+    if (c.nSys != 1) { c.err = "training runs on a single system (nReplicas = nNoise = 1)"; return YCN_ERR_UNSUPPORTED; }
+    if (!c.cnnW || c.tile32 < 0) { c.err = "the learned preconditioner is not available on this lattice"; return YCN_ERR_UNSUPPORTED; }
+    const size_t sp = (size_t)4 * c.V;
+    if (c.pairCap < pairsPerSolve) {
+        if (c.pairR) (void)hipFree(c.pairR);
+        if (c.pairX) (void)hipFree(c.pairX);
+        YCN_HIP(hipMalloc(&c.pairR, sizeof(double2) * sp * pairsPerSolve));
+        YCN_HIP(hipMalloc(&c.pairX, sizeof(double2) * sp * pairsPerSolve));
+        c.pairCap = pairsPerSolve;
+    }
+    const LatticeArgs L = largs(c);
+    const dim3 g = site_grid(c, 1);
+    const int nb = site_blocks(c);
+    int used = 0;
+    c.trainLossFirst = -1;
+    for (int k = 0; k < nsteps; ++k) {
+        const uint64_t step = step0 + (uint64_t)k;
+        int rc = noise_generate(c, step, nullptr, c.noise);
+        if (rc) return rc;
+        const double *phi = c.phi[c.cur];
+        hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((c.V + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, phi, c.phis,
+                           (int64_t)c.V);
+        // FP64 CG (the reference recurrence) with snapshots
+        rc = set_int(c, c.active, 1);
+        if (rc) return rc;
+        hipLaunchKernelGGL(k_cg_setup<true>, g, dim3(kNT), 0, c.stream, L, (const uint64_t *)c.seed_d, 1u, step, c.eta,
+                           c.x, c.r, c.pv[0], c.pv[1], c.partials);
+        hipLaunchKernelGGL(k_fin_setup, dim3(1), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb, c.cg_d,
+                           c.active, c.p.cgTol, c.p.cgTolMode, c.p.cgMaxIter, 4);
+        const int pred = std::max(kGraphIters * (pairsPerSolve + 1), c.lastIters);
+        int launched = 0, active = 1, taken = 0;
+        while (active > 0 && launched < c.p.cgMaxIter) {
+            rc = run_chunk(c, phi, 1, YCN_DAGGER, kGraphIters, YCN_CG_FP64, c.tile64,
+                           [&](int par) { launch_iteration(c, L, phi, 1, YCN_DAGGER, par); });
+            if (rc) return rc;
+            launched += kGraphIters;
+            if (taken < pairsPerSolve && launched >= (taken + 1) * pred / (pairsPerSolve + 1)) {
+                YCN_HIP(hipMemcpyAsync(c.pairR + taken * sp, c.r, sizeof(double2) * sp, hipMemcpyDeviceToDevice, c.stream));
+                YCN_HIP(hipMemcpyAsync(c.pairX + taken * sp, c.x, sizeof(double2) * sp, hipMemcpyDeviceToDevice, c.stream));
+                ++taken;
+            }
+            if (launched % (4 * kGraphIters) == 0) {
+                rc = poll_active(c, active);
+                if (rc) return rc;
+            }
+        }
+        SolveStats st;
+        rc = read_stats(c, 1, &st);
+        if (rc) return rc;
+        // fermion drift and update: the chain advances exactly as a plain step
+        rc = fermion_drift(c, phi, c.drift);
+        if (rc) return rc;
+        rc = bosonic_update(c, c.p.adaptive ? 0 : 1);
+        if (rc) return rc;
+        for (int i = 0; i < taken; ++i) {
+            hipLaunchKernelGGL(k_sub, dim3((unsigned)((sp + kNT - 1) / kNT)), dim3(kNT), 0, c.stream,
+                               (const double2 *)c.x, c.pairX + i * sp, (int64_t)sp);
+            double loss = 0;
+            rc = cnn_train_step(c, c.pairR + i * sp, c.pairX + i * sp, c.phis, lr, &loss, true);
+            if (rc) return rc;
+            if (c.trainLossFirst < 0) c.trainLossFirst = loss;
+            c.trainLossLast = loss;
+            ++used;
+        }
+    }
+    YCN_HIP(hipMemsetAsync(c.cg_d, 0, sizeof(CgState), c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    c.trainPairs = used;
+    if (pairsUsed) *pairsUsed = used;
     return YCN_OK;
 }
 

@@ -1,0 +1,515 @@
+// cnn.hip -- the learned preconditioner on gfx950: a 3-layer periodic 3x3 CNN
+// (8 -> 16 -> 16 -> 8 channels, no bias, tanh between layers) with BF16 weights and
+// activations and FP32 accumulation on MFMA, trained online with Adam on (residual,
+// error) pairs the CG solver produces.  No reference counterpart.
+//
+// Channel 2c / 2c+1 = Re / Im of spinor component c.  z = s * net(r / s) with
+// s = RMS of the 8V real entries of r, so the preconditioner is scale-equivariant.
+// Weight layout (FP32 master copy): W1[9][8][16], W2[9][16][16], W3[9][16][8],
+// [tap][in][out], tap = (dt+1)*3 + (dx+1), out(site) = sum W * in(site + (dt, dx)).
+//
+// Inference (inside flexible CG): k_cnn_fused keeps all three layers of a 16x16 tile
+// (plus its 3-site halo) in LDS as BF16 and fuses the two dot products the flexible
+// CG recurrence needs.  Compulsory HBM traffic per site: read r_s, Ap (32 + 32 B),
+// write z (32 B).
+// Training: layer-wise MFMA kernels over channels-last FP32 activations in HBM,
+// forward, data gradient (a forward convolution with mirrored, transposed weights),
+// weight gradient (an MFMA GEMM whose reduction runs over lattice sites), then Adam.
+#include <algorithm>
+#include <cmath>
+#include <vector>
+
+#include "cg_fused.hpp"
+#include "cnn_mfma.hpp"
+#include "ycn_device.hpp"
+#include "ycn_internal.hpp"
+
+namespace ycn {
+
+constexpr int kCT = 256;   // threads per CNN block (4 waves)
+
+__device__ __forceinline__ int wrapn(int a, int n) { return a < 0 ? a + n : (a >= n ? a - n : a); }
+
+// ---------------------------------------------------------------------------
+// weight fragments
+// ---------------------------------------------------------------------------
+// Forward fragment of a layer: B[k][n] = W[k][n] (k = tap*cin + ci), zero padded.
+// Data-gradient fragment: the transposed convolution equals a forward convolution with
+// W'[tap'][co][ci] = W[8 - tap'][ci][co] (mirrored taps), i.e. cin' = cout.
+__global__ void k_make_frags(const float *__restrict__ W, bf16x8 *__restrict__ f1, bf16x8 *__restrict__ f2,
+                             bf16x8 *__restrict__ f3, bf16x8 *__restrict__ d3, bf16x8 *__restrict__ d2) {
+    const int lane = threadIdx.x & 63;
+    const int job = blockIdx.x;   // 0..4
+    const float *w;
+    int cin, cout, ks;
+    bool dgrad;
+    bf16x8 *dst;
+    switch (job) {
+    case 0: w = W; cin = 8; cout = 16; dgrad = false; dst = f1; break;
+    case 1: w = W + kCnnNW1; cin = 16; cout = 16; dgrad = false; dst = f2; break;
+    case 2: w = W + kCnnNW1 + kCnnNW2; cin = 16; cout = 8; dgrad = false; dst = f3; break;
+    case 3: w = W + kCnnNW1 + kCnnNW2; cin = 8; cout = 16; dgrad = true; dst = d3; break;   // layer 3: 16 -> 8
+    default: w = W + kCnnNW1; cin = 16; cout = 16; dgrad = true; dst = d2; break;          // layer 2
+    }
+    ks = ksteps(cin);
+    const int n = lane % 16, kg = lane / 16;
+    for (int s = threadIdx.x / 64; s < ks; s += blockDim.x / 64) {
+        bf16x8 v;
+#pragma unroll
+        for (int i = 0; i < 8; ++i) {
+            const int k = s * 32 + kg * 8 + i;
+            float val = 0.f;
+            if (k < 9 * cin && n < cout) {
+                const int tap = k / cin, ci = k % cin;
+                // dgrad: this "layer" maps cin (= original cout) to cout (= original cin)
+                val = dgrad ? w[((8 - tap) * cout + n) * cin + ci] : w[(tap * cin + ci) * cout + n];
+            }
+            v[i] = (__bf16)val;
+        }
+        dst[s * 64 + lane] = v;
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused inference: z = s * net(r_s / s) on a 16x16 tile, plus partial <z, r_s>, <z, Ap>
+// ---------------------------------------------------------------------------
+template <int CIN, int COUT, int WOUT, int WIN, int KS, bool TANH, typename Store>
+__device__ __forceinline__ void conv_lds(const __bf16 *__restrict__ in, const bf16x8 *__restrict__ frag, Store store) {
+    constexpr int N = WOUT * WOUT;
+    constexpr int G = (N + 15) / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kg = lane >> 4, col = lane & 15;
+    for (int g = wave; g < G; g += kCT / 64) {
+        const int m = g * 16 + col;
+        const bool valid = m < N;
+        const int a = valid ? m / WOUT : 0, b = valid ? m % WOUT : 0;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+        for (int ks = 0; ks < KS; ++ks) {
+            const int k0 = ks * 32 + kg * 8;
+            const int tap = k0 / CIN, ci0 = k0 % CIN;
+            bf16x8 A = bf16x8_zero();
+            if (tap < 9 && valid) {
+                const int dt = tap / 3 - 1, dx = tap % 3 - 1;
+                const int site = (a + 1 + dt) * WIN + (b + 1 + dx);
+                A = *reinterpret_cast<const bf16x8 *>(in + site * CIN + ci0);
+            }
+            acc = mfma_bf16(A, frag[ks * 64 + lane], acc);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int mm = g * 16 + kg * 4 + i;
+            if (mm < N && col < COUT) store(mm, col, TANH ? fast_tanh(acc[i]) : acc[i]);
+        }
+    }
+}
+
+__global__ void __launch_bounds__(kCT)
+k_cnn_fused(int Nt, int Nx, int tilesX, const CgState *__restrict__ st, const float2 *__restrict__ rs,
+            const float2 *__restrict__ Ap, float2 *__restrict__ z, const bf16x8 *__restrict__ gf1,
+            const bf16x8 *__restrict__ gf2, const bf16x8 *__restrict__ gf3, double2 *__restrict__ part) {
+    constexpr int W0 = 22, W1 = 20, W2 = 18, W3 = 16;
+    __shared__ __attribute__((aligned(16))) __bf16 U0[W0 * W0 * 8];
+    __shared__ __attribute__((aligned(16))) __bf16 U1[W1 * W1 * 16];
+    __shared__ __attribute__((aligned(16))) __bf16 U2[W2 * W2 * 16];
+    __shared__ bf16x8 F1[kFrag1], F2[kFrag2], F3[kFrag3];
+    __shared__ double2 red[kCT / kWave];
+    const int sys = blockIdx.y;
+    if (st[sys].done) return;
+    const int V = Nt * Nx;
+    const size_t off = (size_t)sys * 4 * V;
+    const int tile = blockIdx.x;
+    const int ty = tile / tilesX, tx = tile - ty * tilesX;
+    const int t0 = ty * 16, x0 = tx * 16;
+    const double s = sqrt(st[sys].rr / (8.0 * (double)V));
+    const float inv = s > 0.0 ? (float)(1.0 / s) : 0.f, sf = (float)s;
+    double2 acc = make_double2(0.0, 0.0);
+    if (st[sys].precOff) {
+        // plain CG fallback: identity preconditioner z = r_s
+        const int lt = threadIdx.x / 16, lx = threadIdx.x % 16;
+        const size_t gs = (size_t)(t0 + lt) * Nx + x0 + lx;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float2 rv = rs[off + c * V + gs], av = Ap[off + c * V + gs];
+            z[off + c * V + gs] = rv;
+            acc.x += (double)(rv.x * rv.x + rv.y * rv.y);
+            acc.y += (double)(rv.x * av.x + rv.y * av.y);
+        }
+    } else {
+        for (int w = threadIdx.x; w < W0 * W0; w += kCT) {
+            const int a = w / W0, b = w - a * W0;
+            const size_t gs = (size_t)wrapn(t0 - 3 + a, Nt) * Nx + wrapn(x0 - 3 + b, Nx);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const float2 v = rs[off + c * V + gs];
+                U0[w * 8 + 2 * c] = (__bf16)(v.x * inv);
+                U0[w * 8 + 2 * c + 1] = (__bf16)(v.y * inv);
+            }
+        }
+        for (int i = threadIdx.x; i < kFrag1; i += kCT) F1[i] = gf1[i];
+        for (int i = threadIdx.x; i < kFrag2; i += kCT) F2[i] = gf2[i];
+        for (int i = threadIdx.x; i < kFrag3; i += kCT) F3[i] = gf3[i];
+        __syncthreads();
+        conv_lds<8, 16, W1, W0, ksteps(8), true>(U0, F1, [&](int m, int n, float v) { U1[m * 16 + n] = (__bf16)v; });
+        __syncthreads();
+        conv_lds<16, 16, W2, W1, ksteps(16), true>(U1, F2, [&](int m, int n, float v) { U2[m * 16 + n] = (__bf16)v; });
+        __syncthreads();
+        conv_lds<16, 8, W3, W2, ksteps(16), false>(U2, F3, [&](int m, int n, float v) {
+            const int a = m / W3, b = m - a * W3;
+            const size_t gs = (size_t)(t0 + a) * Nx + x0 + b;
+            const size_t idx = off + (size_t)(n >> 1) * V + gs;
+            const float zv = sf * v;
+            reinterpret_cast<float *>(z + idx)[n & 1] = zv;
+            const float rv = reinterpret_cast<const float *>(rs + idx)[n & 1];
+            const float av = reinterpret_cast<const float *>(Ap + idx)[n & 1];
+            acc.x += (double)(zv * rv);
+            acc.y += (double)(zv * av);
+        });
+    }
+    acc = block_sum2<kCT>(acc, red);
+    if (threadIdx.x == 0) part[(size_t)sys * gridDim.x + blockIdx.x] = acc;
+}
+
+// ---------------------------------------------------------------------------
+// training kernels (layer-wise, channels-last FP32 activations [site][C])
+// ---------------------------------------------------------------------------
+// out[site][co] = act(sum_k in[site + off(tap)][ci] W[k][co]); one wave = 16 sites
+template <int CIN, int COUT, bool TANH>
+__global__ void __launch_bounds__(kCT)
+k_conv_fwd(int Nt, int Nx, const float *__restrict__ in, const bf16x8 *__restrict__ frag, float *__restrict__ out) {
+    constexpr int KS = ksteps(CIN);
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kg = lane >> 4, col = lane & 15;
+    const int V = Nt * Nx;
+    const int g = blockIdx.x * (kCT / 64) + wave;
+    if (g * 16 >= V) return;
+    const int site = g * 16 + col;
+    const int t = site / Nx, x = site - t * Nx;
+    f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) {
+        const int k0 = ks * 32 + kg * 8;
+        const int tap = k0 / CIN, ci0 = k0 % CIN;
+        bf16x8 A = bf16x8_zero();
+        if (tap < 9) {
+            const int dt = tap / 3 - 1, dx = tap % 3 - 1;
+            const int s2 = wrapn(t + dt, Nt) * Nx + wrapn(x + dx, Nx);
+            const float4 lo = *reinterpret_cast<const float4 *>(in + (size_t)s2 * CIN + ci0);
+            const float4 hi = *reinterpret_cast<const float4 *>(in + (size_t)s2 * CIN + ci0 + 4);
+            A[0] = (__bf16)lo.x; A[1] = (__bf16)lo.y; A[2] = (__bf16)lo.z; A[3] = (__bf16)lo.w;
+            A[4] = (__bf16)hi.x; A[5] = (__bf16)hi.y; A[6] = (__bf16)hi.z; A[7] = (__bf16)hi.w;
+        }
+        acc = mfma_bf16(A, frag[ks * 64 + lane], acc);
+    }
+#pragma unroll
+    for (int i = 0; i < 4; ++i) {
+        const int so = g * 16 + kg * 4 + i;
+        if (col < COUT) out[(size_t)so * COUT + col] = TANH ? fast_tanh(acc[i]) : acc[i];
+    }
+}
+
+// partial dW[m][n] = sum_{sites of this block} in[site + off(tap_m)][ci_m] * gy[site][n],
+// m = tap * CIN + ci; MFMA with the lattice sites as the reduction dimension.
+// Each block reduces kWgSites sites; partials go to wpart[block][9*CIN][16].
+constexpr int kWgSites = 1024;
+template <int CIN, int COUT>
+__global__ void __launch_bounds__(kCT)
+k_conv_wgrad(int Nt, int Nx, const float *__restrict__ in, const float *__restrict__ gy, float *__restrict__ wpart) {
+    constexpr int M = 9 * CIN, MT = (M + 15) / 16;
+    const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
+    const int kg = lane >> 4, col = lane & 15;
+    const int V = Nt * Nx;
+    const int s0 = blockIdx.x * kWgSites;
+    float *dst = wpart + (size_t)blockIdx.x * M * 16;
+    for (int mt = wave; mt < MT; mt += kCT / 64) {
+        const int m = mt * 16 + col;          // A row held by this lane
+        const bool mvalid = m < M;
+        const int tap = mvalid ? m / CIN : 0, ci = mvalid ? m % CIN : 0;
+        const int dt = tap / 3 - 1, dx = tap % 3 - 1;
+        f32x4 acc = {0.f, 0.f, 0.f, 0.f};
+        for (int sb = s0; sb < s0 + kWgSites && sb < V; sb += 32) {
+            bf16x8 A, B;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) {
+                const int site = sb + kg * 8 + i;
+                float av = 0.f, bv = 0.f;
+                if (site < V) {
+                    const int t = site / Nx, x = site - t * Nx;
+                    if (mvalid) av = in[(size_t)(wrapn(t + dt, Nt) * Nx + wrapn(x + dx, Nx)) * CIN + ci];
+                    if (col < COUT) bv = gy[(size_t)site * COUT + col];
+                }
+                A[i] = (__bf16)av;
+                B[i] = (__bf16)bv;
+            }
+            acc = mfma_bf16(A, B, acc);
+        }
+#pragma unroll
+        for (int i = 0; i < 4; ++i) {
+            const int mm = mt * 16 + kg * 4 + i;
+            if (mm < M) dst[mm * 16 + col] = acc[i];
+        }
+    }
+}
+
+// grad[w0 + m*COUT + n] = sum_blocks wpart[b][m][n] (block order, deterministic)
+__global__ void k_wgrad_reduce(const float *__restrict__ wpart, int nblocks, int M, int COUT, float *__restrict__ grad) {
+    const int idx = blockIdx.x * blockDim.x + threadIdx.x;
+    if (idx >= M * COUT) return;
+    const int m = idx / COUT, n = idx - m * COUT;
+    double acc = 0.0;
+    for (int b = 0; b < nblocks; ++b) acc += (double)wpart[((size_t)b * M + m) * 16 + n];
+    grad[idx] = (float)acc;
+}
+
+// x0[site][2c+k] = r (FP64 SoA) / s
+__global__ void __launch_bounds__(kCT) k_cnn_prep(const double2 *__restrict__ r, int V, double inv, float *__restrict__ x0) {
+    const int s = blockIdx.x * kCT + threadIdx.x;
+    if (s >= V) return;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double2 v = r[(size_t)c * V + s];
+        x0[(size_t)s * 8 + 2 * c] = (float)(v.x * inv);
+        x0[(size_t)s * 8 + 2 * c + 1] = (float)(v.y * inv);
+    }
+}
+
+// d = e - s * y3 (FP32 SoA for the FP32 normal operator)
+__global__ void __launch_bounds__(kCT)
+k_cnn_resid(const double2 *__restrict__ e, const float *__restrict__ y3, int V, double s, float2 *__restrict__ d) {
+    const int i = blockIdx.x * kCT + threadIdx.x;
+    if (i >= V) return;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const double2 ev = e[(size_t)c * V + i];
+        d[(size_t)c * V + i] = make_float2((float)(ev.x - s * y3[(size_t)i * 8 + 2 * c]),
+                                           (float)(ev.y - s * y3[(size_t)i * 8 + 2 * c + 1]));
+    }
+}
+
+// g3[site][2c+k] = fac * (A d)[c][site].{x,y}
+__global__ void __launch_bounds__(kCT)
+k_cnn_g3(const float2 *__restrict__ Ad, int V, float fac, float *__restrict__ g3) {
+    const int i = blockIdx.x * kCT + threadIdx.x;
+    if (i >= V) return;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const float2 v = Ad[(size_t)c * V + i];
+        g3[(size_t)i * 8 + 2 * c] = fac * v.x;
+        g3[(size_t)i * 8 + 2 * c + 1] = fac * v.y;
+    }
+}
+
+// g *= 1 - u^2 (tanh derivative)
+__global__ void k_tanh_grad(float *__restrict__ g, const float *__restrict__ u, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) g[i] *= 1.f - u[i] * u[i];
+}
+
+// <a, b> and <a, a> partials of FP64 SoA spinors (for s and <e, r>)
+__global__ void __launch_bounds__(kCT)
+k_dot2(const double2 *__restrict__ a, const double2 *__restrict__ b, int n, double2 *__restrict__ part) {
+    __shared__ double2 red[kCT / kWave];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = blockIdx.x * kCT + threadIdx.x; i < n; i += gridDim.x * kCT) {
+        const double2 u = a[i], v = b[i];
+        acc.x += u.x * v.x + u.y * v.y;
+        acc.y += u.x * u.x + u.y * u.y;
+    }
+    acc = block_sum2<kCT>(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
+}
+
+// Adam (Kingma & Ba 2015) on the FP32 master weights
+__global__ void k_adam(float *__restrict__ W, float *__restrict__ m, float *__restrict__ v,
+                       const float *__restrict__ g, int n, float lr, float b1, float b2, float eps, float c1, float c2) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    const float gi = g[i];
+    if (!isfinite(gi)) return;
+    const float mi = b1 * m[i] + (1.f - b1) * gi, vi = b2 * v[i] + (1.f - b2) * gi * gi;
+    m[i] = mi;
+    v[i] = vi;
+    W[i] -= lr * (mi / c1) / (sqrtf(vi / c2) + eps);
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+static bool cnn_fits(const Ctx &c) { return c.p.Nt % 16 == 0 && c.p.Nx % 16 == 0; }
+
+// Default weights: a near-linear, Jacobi-like start: z ~ r / (2r + M) through the
+// centre taps, small enough that tanh stays in its linear range.
+static std::vector<float> default_weights(const Ctx &c) {
+    std::vector<float> W(kCnnNW, 0.f);
+    const float d = (float)(1.0 / ((2.0 * c.p.r + c.p.M) * (2.0 * c.p.r + c.p.M) + 1.0));
+    for (int ci = 0; ci < 8; ++ci) W[(4 * 8 + ci) * 16 + ci] = 0.25f;                          // W1 centre
+    for (int ci = 0; ci < 16; ++ci) W[kCnnNW1 + (4 * 16 + ci) * 16 + ci] = 1.0f;             // W2 centre
+    for (int ci = 0; ci < 8; ++ci) W[kCnnNW1 + kCnnNW2 + (4 * 16 + ci) * 8 + ci] = 4.0f * d;  // W3 centre
+    return W;
+}
+
+int cnn_make_frags(Ctx &c) {
+    hipLaunchKernelGGL(k_make_frags, dim3(5), dim3(256), 0, c.stream, (const float *)c.cnnW, c.cnnF1, c.cnnF2,
+                       c.cnnF3, c.cnnD3, c.cnnD2);
+    YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+
+int cnn_alloc(Ctx &c) {
+    if (!cnn_fits(c)) {
+        c.err = "the learned preconditioner needs Nt % 16 == 0 and Nx % 16 == 0";
+        return YCN_ERR_UNSUPPORTED;
+    }
+    YCN_HIP(hipMalloc(&c.cnnW, sizeof(float) * kCnnNW * 4));   // W, Adam m, Adam v, grad
+    YCN_HIP(hipMemset(c.cnnW, 0, sizeof(float) * kCnnNW * 4));
+    YCN_HIP(hipMalloc(&c.cnnF1, sizeof(bf16x8) * (kFrag1 + kFrag2 + kFrag3 + ksteps(8) * 64 + ksteps(16) * 64)));
+    c.cnnF2 = c.cnnF1 + kFrag1;
+    c.cnnF3 = c.cnnF2 + kFrag2;
+    c.cnnD3 = c.cnnF3 + kFrag3;
+    c.cnnD2 = c.cnnD3 + ksteps(8) * 64;
+    YCN_HIP(hipMalloc(&c.zs, sizeof(float2) * (size_t)c.nSys * 4 * c.V));
+    YCN_HIP(hipMemset(c.zs, 0, sizeof(float2) * (size_t)c.nSys * 4 * c.V));
+    const std::vector<float> W = default_weights(c);
+    YCN_HIP(hipMemcpy(c.cnnW, W.data(), sizeof(float) * kCnnNW, hipMemcpyHostToDevice));
+    c.adamStep = 0;
+    return cnn_make_frags(c);
+}
+
+void cnn_free(Ctx &c) {
+    void **bufs[] = {(void **)&c.cnnW, (void **)&c.cnnF1, (void **)&c.zs, (void **)&c.cnnAct};
+    for (void **b : bufs)
+        if (*b) { (void)hipFree(*b); *b = nullptr; }
+    c.cnnActV = 0;
+}
+
+int cnn_set_weights(Ctx &c, const float *w) {
+    YCN_HIP(hipMemcpyAsync(c.cnnW, w, sizeof(float) * kCnnNW, hipMemcpyHostToDevice, c.stream));
+    YCN_HIP(hipMemsetAsync(c.cnnW + kCnnNW, 0, sizeof(float) * kCnnNW * 2, c.stream));   // reset Adam moments
+    c.adamStep = 0;
+    int rc = cnn_make_frags(c);
+    if (rc) return rc;
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int cnn_get_weights(Ctx &c, float *w) {
+    YCN_HIP(hipMemcpyAsync(w, c.cnnW, sizeof(float) * kCnnNW, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int cnn_apply(Ctx &c, int nsys, const CgState *st, const float2 *rs, const float2 *Ap, float2 *z, double2 *part) {
+    const int tilesX = c.p.Nx / 16, tiles = (c.p.Nt / 16) * tilesX;
+    prof_before(c, PK_CNN, (double)nsys * c.V * 96.0);
+    hipLaunchKernelGGL(k_cnn_fused, dim3(tiles, nsys), dim3(kCT), 0, c.stream, c.p.Nt, c.p.Nx, tilesX, st, rs, Ap, z,
+                       (const bf16x8 *)c.cnnF1, (const bf16x8 *)c.cnnF2, (const bf16x8 *)c.cnnF3, part);
+    prof_after(c, PK_CNN);
+    YCN_HIP(hipGetLastError());
+    return tiles;
+}
+
+// activations for training at lattice volume V: x0[8V] u1[16V] u2[16V] y3[8V] g3[8V]
+// ga[16V] gb[16V] wpart[blocks*144*16] red[256]
+static int ensure_act(Ctx &c) {
+    const size_t V = c.V;
+    const size_t nwb = (V + kWgSites - 1) / kWgSites;
+    const size_t need = V * (8 + 16 + 16 + 8 + 8 + 16 + 16) + nwb * 144 * 16 + 512;
+    if (c.cnnActV == need) return YCN_OK;
+    if (c.cnnAct) (void)hipFree(c.cnnAct);
+    YCN_HIP(hipMalloc(&c.cnnAct, sizeof(float) * need));
+    c.cnnActV = need;
+    return YCN_OK;
+}
+
+// One training step on the pair (r, e) (FP64 SoA, system layout of one system) at the
+// field phis (FP32): forward, loss L = <d, A d> / <e, r> with d = e - z, backward,
+// Adam.  Uses FP32 normal operator scratch (rs/ps/Aps of system 0) and the CgState
+// slot cg_d[0].  Returns the loss (before the update) in *loss.
+int cnn_train_step(Ctx &c, const double2 *r, const double2 *e, const float *phis, float lr, double *loss,
+                   bool update) {
+    int rc = ensure_act(c);
+    if (rc) return rc;
+    const int Nt = c.p.Nt, Nx = c.p.Nx, V = c.V;
+    float *x0 = c.cnnAct, *u1 = x0 + (size_t)8 * V, *u2 = u1 + (size_t)16 * V, *y3 = u2 + (size_t)16 * V;
+    float *g3 = y3 + (size_t)8 * V, *ga = g3 + (size_t)8 * V, *gb = ga + (size_t)16 * V;
+    float *wpart = gb + (size_t)16 * V;
+    const int nwb = (V + kWgSites - 1) / kWgSites;
+    double2 *red = reinterpret_cast<double2 *>(wpart + (size_t)nwb * 144 * 16);
+    float *grad = c.cnnW + 3 * kCnnNW;
+    // s = RMS(r), <e, r>
+    const int nred = 64;
+    hipLaunchKernelGGL(k_dot2, dim3(nred), dim3(kCT), 0, c.stream, e, r, 4 * V, red);
+    hipLaunchKernelGGL(k_dot2, dim3(nred), dim3(kCT), 0, c.stream, r, r, 4 * V, red + nred);
+    std::vector<double2> hr(2 * nred);
+    YCN_HIP(hipMemcpyAsync(hr.data(), red, sizeof(double2) * 2 * nred, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    double er = 0, rr = 0;
+    for (int i = 0; i < nred; ++i) { er += hr[i].x; rr += hr[nred + i].y; }
+    if (!(rr > 0) || !(er > 0)) { *loss = 0; return YCN_OK; }
+    const double s = std::sqrt(rr / (8.0 * V));
+    const dim3 gs((V + kCT - 1) / kCT), gc((V / 16 + 3) / 4);
+    // forward
+    hipLaunchKernelGGL(k_cnn_prep, gs, dim3(kCT), 0, c.stream, r, V, 1.0 / s, x0);
+    hipLaunchKernelGGL((k_conv_fwd<8, 16, true>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)x0,
+                       (const bf16x8 *)c.cnnF1, u1);
+    hipLaunchKernelGGL((k_conv_fwd<16, 16, true>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u1,
+                       (const bf16x8 *)c.cnnF2, u2);
+    hipLaunchKernelGGL((k_conv_fwd<16, 8, false>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u2,
+                       (const bf16x8 *)c.cnnF3, y3);
+    // d = e - s y3; A d with the FP32 fused normal operator (beta = 0: p' = d)
+    hipLaunchKernelGGL(k_cnn_resid, gs, dim3(kCT), 0, c.stream, e, (const float *)y3, V, s, c.rs);
+    CgState S0{};
+    S0.beta = 0.0;
+    S0.done = 0;
+    YCN_HIP(hipMemcpyAsync(c.cg_d, &S0, sizeof(CgState), hipMemcpyHostToDevice, c.stream));
+    launch_fused<float>(c, c.tile32, 1, false, phis, c.cg_d, (const float2 *)c.rs, c.ps[0], c.ps[1], c.Aps,
+                        c.partials);
+    const int nbF = fused_args(c, kTiles32[c.tile32].TT, kTiles32[c.tile32].TX).tilesPerSys;
+    std::vector<double2> hp(nbF);
+    YCN_HIP(hipMemcpyAsync(hp.data(), c.partials, sizeof(double2) * nbF, hipMemcpyDeviceToHost, c.stream));
+    // backward
+    hipLaunchKernelGGL(k_cnn_g3, gs, dim3(kCT), 0, c.stream, (const float2 *)c.Aps, V, (float)(-2.0 * s / er), g3);
+    hipLaunchKernelGGL((k_conv_wgrad<16, 8>), dim3(nwb), dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u2,
+                       (const float *)g3, wpart);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((144 * 8 + 255) / 256), dim3(256), 0, c.stream, (const float *)wpart, nwb,
+                       144, 8, grad + kCnnNW1 + kCnnNW2);
+    hipLaunchKernelGGL((k_conv_fwd<8, 16, false>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)g3,
+                       (const bf16x8 *)c.cnnD3, ga);
+    hipLaunchKernelGGL(k_tanh_grad, dim3((16 * V + 255) / 256), dim3(256), 0, c.stream, ga, (const float *)u2,
+                       (int64_t)16 * V);
+    hipLaunchKernelGGL((k_conv_wgrad<16, 16>), dim3(nwb), dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u1,
+                       (const float *)ga, wpart);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((144 * 16 + 255) / 256), dim3(256), 0, c.stream, (const float *)wpart, nwb,
+                       144, 16, grad + kCnnNW1);
+    hipLaunchKernelGGL((k_conv_fwd<16, 16, false>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)ga,
+                       (const bf16x8 *)c.cnnD2, gb);
+    hipLaunchKernelGGL(k_tanh_grad, dim3((16 * V + 255) / 256), dim3(256), 0, c.stream, gb, (const float *)u1,
+                       (int64_t)16 * V);
+    hipLaunchKernelGGL((k_conv_wgrad<8, 16>), dim3(nwb), dim3(kCT), 0, c.stream, Nt, Nx, (const float *)x0,
+                       (const float *)gb, wpart);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((72 * 16 + 255) / 256), dim3(256), 0, c.stream, (const float *)wpart, nwb,
+                       72, 16, grad);
+    YCN_HIP(hipGetLastError());
+    if (update) {
+        ++c.adamStep;
+        const float b1 = 0.9f, b2 = 0.999f;
+        const float c1 = 1.f - std::pow(b1, (float)c.adamStep), c2 = 1.f - std::pow(b2, (float)c.adamStep);
+        hipLaunchKernelGGL(k_adam, dim3((kCnnNW + 255) / 256), dim3(256), 0, c.stream, c.cnnW, c.cnnW + kCnnNW,
+                           c.cnnW + 2 * kCnnNW, (const float *)grad, kCnnNW, lr, b1, b2, 1e-8f, c1, c2);
+        rc = cnn_make_frags(c);
+        if (rc) return rc;
+    }
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    double dad = 0;
+    for (const double2 &v : hp) dad += v.x;
+    *loss = dad / er;
+    return YCN_OK;
+}
+
+int cnn_get_grad(Ctx &c, float *g) {
+    YCN_HIP(hipMemcpyAsync(g, c.cnnW + 3 * kCnnNW, sizeof(float) * kCnnNW, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+}  // namespace ycn

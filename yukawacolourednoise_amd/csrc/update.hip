@@ -138,7 +138,39 @@ __global__ void __launch_bounds__(kUT) k_soa_to_aos(const double2 *__restrict__ 
     for (int c = 0; c < 4; ++c) aos[o + 4 * (size_t)s + c] = soa[o + (size_t)c * V + s];
 }
 
+__global__ void __launch_bounds__(kUT) k_d2f(const double2 *__restrict__ a, float2 *__restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kUT + threadIdx.x;
+    if (i < n) b[i] = make_float2((float)a[i].x, (float)a[i].y);
+}
+__global__ void __launch_bounds__(kUT) k_f2d(const float2 *__restrict__ a, double2 *__restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kUT + threadIdx.x;
+    if (i < n) b[i] = make_double2((double)a[i].x, (double)a[i].y);
+}
+__global__ void __launch_bounds__(kUT) k_field_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+    const int64_t i = (int64_t)blockIdx.x * kUT + threadIdx.x;
+    if (i < n) b[i] = (float)a[i];
+}
+
 // ---------------------------------------------------------------------------
+int spinor_to_f32(Ctx &c, const double2 *a, float2 *b, int nsys) {
+    const int64_t n = (int64_t)nsys * 4 * c.V;
+    hipLaunchKernelGGL(k_d2f, dim3((unsigned)((n + kUT - 1) / kUT)), dim3(kUT), 0, c.stream, a, b, n);
+    YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+int spinor_to_f64(Ctx &c, const float2 *a, double2 *b, int nsys) {
+    const int64_t n = (int64_t)nsys * 4 * c.V;
+    hipLaunchKernelGGL(k_f2d, dim3((unsigned)((n + kUT - 1) / kUT)), dim3(kUT), 0, c.stream, a, b, n);
+    YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+int field_to_f32(Ctx &c, const double *a, float *b) {
+    hipLaunchKernelGGL(k_field_f32, dim3((unsigned)((c.V + kUT - 1) / kUT)), dim3(kUT), 0, c.stream, a, b,
+                       (int64_t)c.V);
+    YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+
 int bosonic_update(Ctx &c, int fixedEps) {
     const int Nt = c.p.Nt, Nx = c.p.Nx, V = c.V;
     dim3 g((V + kUT - 1) / kUT, c.nRep);

@@ -92,6 +92,9 @@ void ycn_destroy(ycn_ctx *ctx) {
     noise_free(c);
     cg_free(c);
     mp_free(c);
+    cnn_free(c);
+    if (c.pairR) (void)hipFree(c.pairR);
+    if (c.pairX) (void)hipFree(c.pairX);
     double **bufs[] = {&c.phi[0], &c.phi[1], &c.noise, &c.drift, &c.m2_d};
     for (double **b : bufs)
         if (*b) (void)hipFree(*b);
@@ -113,7 +116,6 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         (p->precond != YCN_PRECOND_NONE && p->precond != YCN_PRECOND_CNN))
         return YCN_ERR_INVALID;
     if ((int64_t)p->Nt * p->Nx > (int64_t)1 << 28) return YCN_ERR_INVALID;
-    if (p->precond == YCN_PRECOND_CNN) return YCN_ERR_UNSUPPORTED;
     int ndev = 0;
     if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0) return YCN_ERR_NO_DEVICE;
     if (p->device < 0 || p->device >= ndev) return YCN_ERR_INVALID;
@@ -153,7 +155,15 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         return bail(YCN_ERR_HIP);
     if ((rc = noise_init(c))) return bail(rc);
     if ((rc = cg_alloc(c))) return bail(rc);
-    if (c.p.cgPrecision == YCN_CG_MIXED && (rc = mp_alloc(c))) return bail(rc);
+    // FP32 and CNN buffers whenever the lattice admits the FP32 tiles / CNN tiles
+    const bool f32ok = (p->Nt % 4 == 0) && (p->Nx % 16 == 0);
+    const bool cnnok = (p->Nt % 16 == 0) && (p->Nx % 16 == 0);
+    if (f32ok && (rc = mp_alloc(c))) return bail(rc);
+    if (cnnok && (rc = cnn_alloc(c))) return bail(rc);
+    if (c.p.precond == YCN_PRECOND_CNN && !cnnok) {
+        c.err = "the learned preconditioner needs Nt % 16 == 0 and Nx % 16 == 0";
+        return bail(YCN_ERR_UNSUPPORTED);
+    }
     if ((rc = cg_autotune(c))) return bail(rc);
     if (c.p.cgPrecision == YCN_CG_MIXED && c.tile32 < 0) {
         c.err = "mixed-precision CG needs Nt % 4 == 0 and Nx % 16 == 0";
@@ -438,6 +448,105 @@ int ycn_measure(ycn_ctx *ctx, int replica, uint64_t step, ycn_observables *obs, 
     return YCN_OK;
 }
 
+static int need_cnn(Ctx &c) {
+    if (!c.cnnW || !c.rs || c.tile32 < 0)
+        return fail(c, YCN_ERR_UNSUPPORTED, "the learned preconditioner needs Nt % 16 == 0 and Nx % 16 == 0");
+    return YCN_OK;
+}
+
+int ycn_precond_set_weights(ycn_ctx *ctx, const float *w) {
+    CHECK_CTX(ctx);
+    if (!w) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    int rc = need_cnn(c);
+    return rc ? rc : cnn_set_weights(c, w);
+}
+
+int ycn_precond_get_weights(ycn_ctx *ctx, float *w) {
+    CHECK_CTX(ctx);
+    if (!w) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    int rc = need_cnn(c);
+    return rc ? rc : cnn_get_weights(c, w);
+}
+
+int ycn_precond_apply(ycn_ctx *ctx, const double *r, double *z) {
+    CHECK_CTX(ctx);
+    if (!r || !z) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    int rc = need_cnn(c);
+    if (rc) return rc;
+    const size_t n = (size_t)8 * c.V;
+    double rr = 0;
+    for (size_t i = 0; i < n; ++i) rr += r[i] * r[i];
+    DevTmp aos;
+    YCN_HIP(hipMalloc(&aos.p, sizeof(double) * n));
+    YCN_HIP(hipMemcpyAsync(aos.p, r, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    if ((rc = aos_to_soa(c, (const double *)aos.p, c.t, 1))) return rc;
+    if ((rc = spinor_to_f32(c, c.t, c.rs, 1))) return rc;
+    CgState S{};
+    S.rr = rr;
+    YCN_HIP(hipMemcpyAsync(c.cg_d, &S, sizeof(CgState), hipMemcpyHostToDevice, c.stream));
+    rc = cnn_apply(c, 1, c.cg_d, c.rs, c.Aps, c.zs, c.partials);
+    if (rc < 0) return rc;
+    if ((rc = spinor_to_f64(c, c.zs, c.t, 1))) return rc;
+    if ((rc = soa_to_aos(c, c.t, (double *)aos.p, 1))) return rc;
+    YCN_HIP(hipMemcpyAsync(z, aos.p, sizeof(double) * n, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipMemsetAsync(c.cg_d, 0, sizeof(CgState), c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int ycn_precond_loss_grad(ycn_ctx *ctx, const double *phi, const double *r, const double *e, double *loss,
+                          float *grad) {
+    CHECK_CTX(ctx);
+    if (!r || !e) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    int rc = need_cnn(c);
+    if (rc) return rc;
+    const size_t n = (size_t)8 * c.V;
+    if (c.pairCap < 1) {
+        if (c.pairR) (void)hipFree(c.pairR);
+        if (c.pairX) (void)hipFree(c.pairX);
+        YCN_HIP(hipMalloc(&c.pairR, sizeof(double) * n));
+        YCN_HIP(hipMalloc(&c.pairX, sizeof(double) * n));
+        c.pairCap = 1;
+    }
+    DevTmp ph, aos;
+    const double *phi_dev = nullptr;
+    if ((rc = stage_phi(c, phi, ph, &phi_dev))) return rc;
+    YCN_HIP(hipMalloc(&aos.p, sizeof(double) * n));
+    YCN_HIP(hipMemcpyAsync(aos.p, r, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    if ((rc = aos_to_soa(c, (const double *)aos.p, c.pairR, 1))) return rc;
+    YCN_HIP(hipMemcpyAsync(aos.p, e, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
+    if ((rc = aos_to_soa(c, (const double *)aos.p, c.pairX, 1))) return rc;
+    if ((rc = field_to_f32(c, phi_dev, c.phis))) return rc;
+    double L = 0;
+    if ((rc = cnn_train_step(c, c.pairR, c.pairX, c.phis, 0.f, &L, false))) return rc;
+    if (loss) *loss = L;
+    if (grad && (rc = cnn_get_grad(c, grad))) return rc;
+    YCN_HIP(hipMemsetAsync(c.cg_d, 0, sizeof(CgState), c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+int ycn_precond_train(ycn_ctx *ctx, uint64_t step0, int nsteps, int pairsPerSolve, double lr,
+                      ycn_train_stats *stats) {
+    CHECK_CTX(ctx);
+    if (nsteps < 0 || pairsPerSolve < 1 || !(lr > 0)) return YCN_ERR_INVALID;
+    Ctx &c = *ctx;
+    int rc = need_cnn(c);
+    if (rc) return rc;
+    int used = 0;
+    if ((rc = train_preconditioner(c, step0, nsteps, pairsPerSolve, (float)lr, &used))) return rc;
+    if (stats) {
+        stats->pairs = used;
+        stats->loss_first = c.trainLossFirst;
+        stats->loss_last = c.trainLossLast;
+    }
+    return YCN_OK;
+}
+
 int ycn_sync(ycn_ctx *ctx) {
     CHECK_CTX(ctx);
     Ctx &c = *ctx;
@@ -449,7 +558,7 @@ int ycn_profile_begin(ycn_ctx *ctx, const char *kernel) {
     CHECK_CTX(ctx);
     if (!kernel) return YCN_ERR_INVALID;
     Ctx &c = *ctx;
-    static const char *names[PK_COUNT] = {"cg_normal_op", "cg_update", "dirac", "noise_cols", "bosonic"};
+    static const char *names[PK_COUNT] = {"cg_normal_op", "cg_update", "dirac", "noise_cols", "bosonic", "cnn"};
     int k = PK_NONE;
     for (int i = 0; i < PK_COUNT; ++i)
         if (std::strcmp(kernel, names[i]) == 0) k = i;

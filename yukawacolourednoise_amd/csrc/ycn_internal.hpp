@@ -7,6 +7,7 @@
 #include <vector>
 
 #include "../../include/ycn.h"
+#include "cnn_mfma.hpp"
 
 namespace ycn {
 
@@ -39,6 +40,10 @@ struct CgState {
     int maxRestarts;
     int final_;        // mixed precision: converged or gave up (no further reliable updates)
     double rr_prev;    // |r|^2 before the last update (mixed precision: beta after a reliable update)
+    double rz;         // flexible CG: <r, z> with z = CNN(r)
+    double true_prev;  // flexible CG: true relative residual at the previous reliable update
+    int precOff;       // flexible CG: 1 = preconditioner switched off (plain CG fallback)
+    int pad2;
 };
 
 // Per-replica Langevin state (device resident).
@@ -50,7 +55,7 @@ struct StepState {
     int pad;
 };
 
-enum ProfKernel { PK_NONE = -1, PK_NORMAL_OP = 0, PK_CG_UPDATE, PK_DIRAC, PK_NOISE_COLS, PK_BOSONIC, PK_COUNT };
+enum ProfKernel { PK_NONE = -1, PK_NORMAL_OP = 0, PK_CG_UPDATE, PK_DIRAC, PK_NOISE_COLS, PK_BOSONIC, PK_CNN, PK_COUNT };
 
 struct Ctx {
     ycn_params p;
@@ -91,6 +96,19 @@ struct Ctx {
     // mixed precision: [nSys][4][V] float2
     float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *Aps = nullptr, *xs = nullptr;
     float *phis = nullptr;
+
+    // learned preconditioner
+    float *cnnW = nullptr;                // master weights, Adam m, Adam v, gradient (4 x 4608)
+    bf16x8 *cnnF1 = nullptr, *cnnF2 = nullptr, *cnnF3 = nullptr, *cnnD3 = nullptr, *cnnD2 = nullptr;
+    float2 *zs = nullptr;                 // z = CNN(r) [nSys][4][V]
+    float *cnnAct = nullptr;              // training activations
+    size_t cnnActV = 0;
+    int adamStep = 0;
+    double2 *pairR = nullptr, *pairX = nullptr;   // training snapshots [pairs][4][V]
+    int pairCap = 0;
+    // statistics of the last ycn_precond_train call
+    double trainLossFirst = 0, trainLossLast = 0;
+    int trainPairs = 0;
 
     // captured CG iteration chunks: key -> executable graph
     struct GraphEntry {
@@ -149,11 +167,32 @@ int fermion_drift(Ctx &c, const double *phi_dev, double *drift_dev);
 // correlator / trace helpers
 int spinor_slices(Ctx &c, const double2 *psi, double *out_dev);
 
+// cnn.hip
+int cnn_alloc(Ctx &c);
+void cnn_free(Ctx &c);
+int cnn_make_frags(Ctx &c);
+int cnn_set_weights(Ctx &c, const float *w);
+int cnn_get_weights(Ctx &c, float *w);
+int cnn_get_grad(Ctx &c, float *g);
+// launches the fused CNN on nsys systems; returns the number of partials per system
+int cnn_apply(Ctx &c, int nsys, const CgState *st, const float2 *rs, const float2 *Ap, float2 *z, double2 *part);
+int cnn_train_step(Ctx &c, const double2 *r, const double2 *e, const float *phis, float lr, double *loss,
+                   bool update);
+// flexible CG with the CNN and the online training driver (cg.hip)
+int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t streamBase, uint64_t step,
+              SolveStats *st);
+int train_preconditioner(Ctx &c, uint64_t step0, int nsteps, int pairsPerSolve, float lr, int *pairsUsed);
+
 // update.hip
 int bosonic_update(Ctx &c, int fixedEps);
 int field_slices(Ctx &c, int rep, double *slices_dev, double *sums_dev);
 int hot_start(Ctx &c, uint64_t initSeed);
 int set_int(Ctx &c, int *ptr, int v);
+
+// precision conversion
+int spinor_to_f32(Ctx &c, const double2 *a, float2 *b, int nsys);
+int spinor_to_f64(Ctx &c, const float2 *a, double2 *b, int nsys);
+int field_to_f32(Ctx &c, const double *a, float *b);
 
 // layout conversion (abi)
 int aos_to_soa(Ctx &c, const double *aos_dev, double2 *soa, int nsys);
