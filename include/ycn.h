@@ -167,7 +167,9 @@ int ycn_sync(ycn_ctx *ctx);
 /* ---- learned preconditioner (north-star addition; no reference counterpart) ----
  * A 3-layer periodic 3x3 CNN, 8 -> 16 -> 16 -> 8 channels (channel 2c / 2c+1 = Re / Im
  * of spinor component c), no bias, tanh between layers, BF16 weights and activations
- * with FP32 accumulation on MFMA.  z = s * net(r / s), s = RMS of r's 8V real entries.
+ * with FP32 accumulation on MFMA, as a learned correction to an exact skip path:
+ * z = c0 r + s * net(r / s), c0 = 1 / (2r + M)^2, s = RMS of r's 8V real entries.
+ * The default network has a zero output layer (z = c0 r, i.e. plain CG).
  * Weights (YCN_CNN_NWEIGHTS floats): W1[9][8][16], W2[9][16][16], W3[9][16][8] as
  * [tap][in][out], tap = (dt+1)*3 + (dx+1), out(site) = sum W * in(site + (dt,dx)).
  * With precond = YCN_PRECOND_CNN the fermion-force solves run flexible CG with this

@@ -80,8 +80,8 @@ def lib():
         L.orc_sum.argtypes = [C.c_int64, dp]
         L.orc_sum.restype = C.c_double
         fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
-        L.orc_cnn_forward.argtypes = [C.c_int, C.c_int, fp, C.c_int, dp, dp]
-        L.orc_cnn_loss_grad.argtypes = [pp, dp, fp, dp, dp, dp]
+        L.orc_cnn_forward.argtypes = [C.c_int, C.c_int, fp, C.c_int, C.c_double, dp, dp]
+        L.orc_cnn_loss_grad.argtypes = [pp, dp, fp, C.c_double, dp, dp, dp]
         L.orc_cnn_loss_grad.restype = C.c_double
         L.orc_bf16_round.argtypes = [C.c_double]
         L.orc_bf16_round.restype = C.c_double
@@ -247,16 +247,22 @@ def correlator(p: OrcParams, phi):
 CNN_NW = 9 * 8 * 16 + 9 * 16 * 16 + 9 * 16 * 8
 
 
-def cnn_forward(Nt, Nx, W, r, bf16=True):
-    """z = CNN(r) for an AoS (V, 4) complex spinor; bf16 emulates the GPU's rounding."""
+def cnn_c0(r=1.0, M=0.3):
+    """Weight of the exact skip path z = c0 r + s net(r/s): c0 = 1 / (2r + M)^2."""
+    return 1.0 / (2.0 * r + M) ** 2
+
+
+def cnn_forward(Nt, Nx, W, r, bf16=True, c0=0.0):
+    """z = c0 r + s net(r/s) for an AoS (V, 4) complex spinor; bf16 emulates the GPU's
+    rounding of weights and activations."""
     W = np.ascontiguousarray(W, dtype=np.float32)
     out = np.empty(8 * Nt * Nx, np.float64)
-    lib().orc_cnn_forward(Nt, Nx, W, int(bf16), _spin_to_f64(r), out)
+    lib().orc_cnn_forward(Nt, Nx, W, int(bf16), c0, _spin_to_f64(r), out)
     return _f64_to_spin(out, Nt * Nx)
 
 
-def cnn_loss_grad(p: OrcParams, phi, W, r, e):
+def cnn_loss_grad(p: OrcParams, phi, W, r, e, c0=0.0):
     W = np.ascontiguousarray(W, dtype=np.float32)
     g = np.empty(CNN_NW, np.float64)
-    L = lib().orc_cnn_loss_grad(C.byref(p), _f64(phi).reshape(-1), W, _spin_to_f64(r), _spin_to_f64(e), g)
+    L = lib().orc_cnn_loss_grad(C.byref(p), _f64(phi).reshape(-1), W, c0, _spin_to_f64(r), _spin_to_f64(e), g)
     return L, g

@@ -666,17 +666,17 @@ static double cnn_fwd(int Nt, int Nx, const float *Wf, int bf16, const double *r
     return s;
 }
 
-void orc_cnn_forward(int Nt, int Nx, const float *W, int bf16, const double *r, double *z) {
+void orc_cnn_forward(int Nt, int Nx, const float *W, int bf16, double c0, const double *r, double *z) {
     const size_t V = (size_t)Nt * Nx;
     double *u0 = malloc(sizeof(double) * 8 * V), *y1 = malloc(sizeof(double) * 16 * V);
     double *u1 = malloc(sizeof(double) * 16 * V), *y2 = malloc(sizeof(double) * 16 * V);
     double *u2 = malloc(sizeof(double) * 16 * V), *y3 = malloc(sizeof(double) * 8 * V);
     double s = cnn_fwd(Nt, Nx, W, bf16, r, u0, y1, u1, y2, u2, y3);
-    for (size_t i = 0; i < 8 * V; ++i) z[i] = s * y3[i];
+    for (size_t i = 0; i < 8 * V; ++i) z[i] = c0 * r[i] + s * y3[i];
     free(u0); free(y1); free(u1); free(y2); free(u2); free(y3);
 }
 
-double orc_cnn_loss_grad(const orc_params *p, const double *phi, const float *W, const double *r,
+double orc_cnn_loss_grad(const orc_params *p, const double *phi, const float *W, double c0, const double *r,
                          const double *e, double *grad) {
     const int Nt = p->Nt, Nx = p->Nx;
     const size_t V = (size_t)Nt * Nx;
@@ -688,7 +688,7 @@ double orc_cnn_loss_grad(const orc_params *p, const double *phi, const float *W,
     double *g1 = malloc(sizeof(double) * 16 * V);
     double s = cnn_fwd(Nt, Nx, W, 0, r, u0, y1, u1, y2, u2, y3);
     /* d = e - z; L = <d, A d> / <e, r>; dL/dz = -2 A d / <e, r>; dL/dy3 = s dL/dz */
-    for (size_t i = 0; i < 8 * V; ++i) z[i] = e[i] - s * y3[i];
+    for (size_t i = 0; i < 8 * V; ++i) z[i] = e[i] - (c0 * r[i] + s * y3[i]);
     orc_normal_op(p, phi, z, Az, 1);
     double er = 0.0, L = 0.0;
     for (size_t i = 0; i < 8 * V; ++i) { er += e[i] * r[i]; L += z[i] * Az[i]; }

@@ -42,7 +42,7 @@ def test_cnn_forward_vs_oracle(gpu, Nt, Nx):
         assert np.array_equal(c.precond_get_weights(), W)
         z = c.precond_apply(r)
         z2 = c.precond_apply(7.5 * r)
-    ref = O.cnn_forward(Nt, Nx, W, r, bf16=True)
+    ref = O.cnn_forward(Nt, Nx, W, r, bf16=True, c0=O.cnn_c0(1.0, 0.05))
     err = np.abs(z - ref).max() / np.abs(ref).max()
     l2 = np.linalg.norm(z - ref) / np.linalg.norm(ref)
     assert err < 1e-2 and l2 < 3e-3, (err, l2)
@@ -62,7 +62,7 @@ def test_cnn_gradient_vs_oracle(gpu):
     with ctx(Y, Nt, Nx) as c:
         c.precond_set_weights(W)
         L, g = c.precond_loss_grad(r, e, phi=phi)
-    Lr, gr = O.cnn_loss_grad(p, phi, W, r, e)
+    Lr, gr = O.cnn_loss_grad(p, phi, W, r, e, c0=O.cnn_c0(1.0, 0.05))
     assert abs(L - Lr) <= 2e-2 * abs(Lr), (L, Lr)
     cos = float(np.dot(g, gr) / (np.linalg.norm(g) * np.linalg.norm(gr)))
     assert cos > 0.99, cos

@@ -53,15 +53,21 @@ def test_gradient_finite_differences(seed):
     phi = rng.uniform(-1, 1, (Nt, Nx))
     e = rng.standard_normal((V, 4)) + 1j * rng.standard_normal((V, 4))
     r = O.normal_op(p, phi, e, 1)
-    L, g = O.cnn_loss_grad(p, phi, W, r, e)
+    c0 = O.cnn_c0(1.0, 0.3)
+    L, g = O.cnn_loss_grad(p, phi, W, r, e, c0=c0)
     # one weight in every layer, including the first tap and channel edges
     for i in [0, 5, 1151, 1152, 2000, 3455, 3456, 4000, 4607] + list(rng.choice(O.CNN_NW, 6, replace=False)):
         h = np.float32(1e-3)
         Wp, Wm = W.copy(), W.copy()
         Wp[i] += h
         Wm[i] -= h
-        fd = (O.cnn_loss_grad(p, phi, Wp, r, e)[0] - O.cnn_loss_grad(p, phi, Wm, r, e)[0]) / (float(Wp[i]) - float(Wm[i]))
+        fd = (O.cnn_loss_grad(p, phi, Wp, r, e, c0=c0)[0] - O.cnn_loss_grad(p, phi, Wm, r, e, c0=c0)[0]) / (float(Wp[i]) - float(Wm[i]))
         assert abs(g[i] - fd) <= 1e-4 * max(1.0, abs(fd)), (i, g[i], fd)
-    # with z = 0 (all weights zero) the loss is exactly 1 for a consistent pair
+    # with z = 0 (all weights zero, no skip path) the loss is exactly 1 for a consistent pair
     L0, _ = O.cnn_loss_grad(p, phi, np.zeros(O.CNN_NW, np.float32), r, e)
     assert abs(L0 - 1.0) < 1e-12
+    # the skip path alone: L = <e - c0 r, A (e - c0 r)> / <e, r>
+    d = (e - c0 * r).reshape(-1)
+    Ad = O.normal_op(p, phi, d.reshape(-1, 4), 1).reshape(-1)
+    L1, _ = O.cnn_loss_grad(p, phi, np.zeros(O.CNN_NW, np.float32), r, e, c0=c0)
+    assert abs(L1 - np.vdot(d, Ad).real / np.vdot(e.reshape(-1), r.reshape(-1)).real) < 1e-12

@@ -125,6 +125,7 @@ k_fin_setup(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, 
     S.rz = 0.0;
     S.true_prev = 1.0;
     S.precOff = 0;
+    S.restartNext = 0;
     const bool run = (0 < maxIter) && (sqrt(tot.x) > S.stop);
     S.done = run ? 0 : 1;
     S.final_ = run ? 0 : 1;
@@ -192,9 +193,10 @@ k_fin_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, i
 }
 
 // mixed precision reliable update decision: converged on the FP64 true residual, or
-// resume the FP32 loop.  Mixed CG keeps the direction (beta = |r_true|^2 / |r_prev|^2);
-// flexible CG restarts it (beta = 0, z recomputed from the new residual) and switches
-// the preconditioner off when a reliable interval did not reduce the true residual.
+// resume the FP32 loop keeping the search direction.  Mixed CG: beta = |r_true|^2 /
+// |r_prev|^2.  Flexible CG: beta follows from z = CNN(r_true) (k_fin_fcg); the network
+// is switched off, and the direction restarted, when a reliable interval did not
+// reduce the true residual.
 template <bool FCG>
 __global__ void __launch_bounds__(kNT)
 k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active) {
@@ -216,9 +218,14 @@ k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
         return;
     }
     if (FCG) {
-        if (!(S.true_rel < S.true_prev)) S.precOff = 1;   // stall: plain CG from here on
+        // the direction is kept (beta follows from the new z in k_fin_fcg); a reliable
+        // interval that did not reduce the true residual switches the network off and
+        // restarts the direction (plain CG from here on)
+        if (!(S.true_rel < S.true_prev) && !S.precOff) {
+            S.precOff = 1;
+            S.restartNext = 1;
+        }
         S.true_prev = S.true_rel;
-        S.beta = 0.0;
     } else {
         S.beta = rrt / S.rr_prev;
     }
@@ -246,8 +253,10 @@ __global__ void __launch_bounds__(kNT) k_fin_alpha_fcg(const double2 *__restrict
 
 // flexible CG after z = CNN(r'): partials hold (<z, r'>, <z, Ap>).  Polak-Ribiere form
 // (Notay, SIAM J. Sci. Comput. 22 (2000)): beta = <z, r' - r> / <z_old, r> with
-// r = r' + alpha Ap, i.e. beta = -alpha <z, Ap> / rz_old.  RESTART: beta = 0.
-template <bool RESTART>
+// r = r' + alpha Ap, i.e. beta = -alpha <z, Ap> / rz_old.  MODE 0: inside the loop;
+// MODE 1: after a reliable update replaced r' by the true residual (rz_old is then the
+// value before the last iteration, rz_prev); MODE 2: restart (beta = 0).
+template <int MODE>
 __global__ void __launch_bounds__(kNT) k_fin_fcg(const double2 *__restrict__ part, int nb, CgState *__restrict__ st) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.x;
@@ -260,7 +269,10 @@ __global__ void __launch_bounds__(kNT) k_fin_fcg(const double2 *__restrict__ par
         S.beta = 0.0;
         return;
     }
-    S.beta = (RESTART || !(S.rz > 0.0)) ? 0.0 : -S.alpha.x * tot.y / S.rz;
+    const double rzOld = (MODE == 1) ? S.rz_prev : S.rz;
+    S.beta = (MODE == 2 || S.restartNext || !(rzOld > 0.0)) ? 0.0 : -S.alpha.x * tot.y / rzOld;
+    S.restartNext = 0;
+    if (MODE != 1) S.rz_prev = S.rz;
     S.rz = tot.x;
 }
 
@@ -1050,7 +1062,7 @@ static void launch_fcg_iteration(Ctx &c, int nsys, int mtype, int parity) {
     hipLaunchKernelGGL(k_fin_beta<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials,
                        site_blocks(c), c.cg_d, c.active);
     const int nbC = cnn_apply(c, nsys, c.cg_d, c.rs, c.Aps, c.zs, c.partials);
-    hipLaunchKernelGGL(k_fin_fcg<false>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC, c.cg_d);
+    hipLaunchKernelGGL(k_fin_fcg<0>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC, c.cg_d);
 }
 
 int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t streamBase, uint64_t step,
@@ -1080,7 +1092,7 @@ int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t
     // z = CNN(b), rz = <b, z>, p' = z on the first iteration
     int nbC = cnn_apply(c, nsys, c.cg_d, c.rs, c.Aps, c.zs, c.partials);
     if (nbC < 0) return nbC;
-    hipLaunchKernelGGL(k_fin_fcg<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC, c.cg_d);
+    hipLaunchKernelGGL(k_fin_fcg<2>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC, c.cg_d);
     YCN_HIP(hipGetLastError());
     const int every = std::max(1, c.p.reliableEvery);
     const int nIt = every + (every & 1);
@@ -1101,7 +1113,7 @@ int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t
                            c.cg_d, c.active);
         nbC = cnn_apply(c, nsys, c.cg_d, c.rs, c.Aps, c.zs, c.partials);
         if (nbC < 0) return nbC;
-        hipLaunchKernelGGL(k_fin_fcg<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC,
+        hipLaunchKernelGGL(k_fin_fcg<1>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC,
                            c.cg_d);
         YCN_HIP(hipGetLastError());
         rc = poll_active(c, active);

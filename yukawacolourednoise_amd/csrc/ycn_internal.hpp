@@ -41,9 +41,10 @@ struct CgState {
     int final_;        // mixed precision: converged or gave up (no further reliable updates)
     double rr_prev;    // |r|^2 before the last update (mixed precision: beta after a reliable update)
     double rz;         // flexible CG: <r, z> with z = CNN(r)
+    double rz_prev;    // flexible CG: <r, z> before the last iteration
     double true_prev;  // flexible CG: true relative residual at the previous reliable update
     int precOff;       // flexible CG: 1 = preconditioner switched off (plain CG fallback)
-    int pad2;
+    int restartNext;   // flexible CG: next beta = 0
 };
 
 // Per-replica Langevin state (device resident).
@@ -171,6 +172,7 @@ int spinor_slices(Ctx &c, const double2 *psi, double *out_dev);
 int cnn_alloc(Ctx &c);
 void cnn_free(Ctx &c);
 int cnn_make_frags(Ctx &c);
+double cnn_c0(const Ctx &c);
 int cnn_set_weights(Ctx &c, const float *w);
 int cnn_get_weights(Ctx &c, float *w);
 int cnn_get_grad(Ctx &c, float *g);
