@@ -5,9 +5,13 @@ Workload (the headline configuration the metric is quoted on): Nt = Nx = 1024,
 M = 0.05, g = 0.5, m^2 = -0.4, lambda = 1.0, r = 1, cutoff s = 0.5, eps = 0.005 fixed,
 one Gaussian noise vector per drift estimate, every CG solve run to true relative
 residual <= 1e-10 (recomputed in FP64, restarted while above).  Start: hot start
-phi ~ U[-1, 1] (Philox, seed 7), then `warmup` untimed steps; the timed region is
-`steps` full Langevin steps (coloured noise, fermion force, bosonic drift, update),
-bracketed by device synchronisation.  Inputs and state stay resident in HBM.
+phi ~ U[-1, 1] (Philox, seed 7) and `therm` untimed thermalisation steps, done once
+and shared: every CG method (plain FP64, mixed precision, learned preconditioner)
+starts from that same field at the same step counter, runs `warmup` untimed steps and
+then the timed region of `steps` full Langevin steps (coloured noise, fermion force,
+bosonic drift, update), bracketed by device synchronisation.  With the learned method
+the thermalisation steps are also its online training (untimed, reported).  Inputs
+and state stay resident in HBM.
 
 Prints one JSON line.  `roofline` is measured live: every launch of the dominant kernel
 in the timed region is bracketed by HIP events on the library's own stream.
@@ -79,17 +83,55 @@ def cpu_baseline(gpu_iters_per_solve: float, budget_s: float = 20.0):
     }
 
 
-def run_method(Y, size, method, steps, warmup, profile_kernel):
-    """Time `steps` Langevin steps of one CG method on the workload; returns a dict."""
+THERM_STEPS = 125           # shared thermalisation before any method is timed
+TRAIN_PAIRS_PER_SOLVE = 16  # (residual, error) pairs per solve for the learned preconditioner
+
+
+def make_ctx(Y, size, method):
     wl = dict(WORKLOAD)
     wl["Nt"] = wl["Nx"] = size
     p = Y.default_params(**wl)
-    p.cgPrecision = Y.YCN_CG_MIXED if method == "mixed" else Y.YCN_CG_FP64
+    p.cgPrecision = Y.YCN_CG_FP64 if method == "fp64" else Y.YCN_CG_MIXED
+    p.precond = Y.YCN_PRECOND_CNN if method == "learned" else Y.YCN_PRECOND_NONE
     p.reliableEvery = 50
-    ctx = Y.Context(p)
-    tuning = ctx.tuning_report()
+    return Y.Context(p)
+
+
+def thermalise(Y, size, therm, train):
+    """Hot start (Philox, seed 7) and `therm` Langevin steps.  With `train` the steps
+    are run by the learned method's context, which trains its preconditioner online on
+    the fermion solves of those steps (its FP64 CG leaves the (r, e) pairs); otherwise a
+    mixed-precision context runs them.  Returns (context or None, phi after the steps,
+    training stats)."""
+    ctx = make_ctx(Y, size, "learned" if train else "mixed")
     ctx.hot_start(7)
-    step = 0
+    t0 = time.perf_counter()
+    info = None
+    if train:
+        info = ctx.precond_train(0, therm, pairs_per_solve=TRAIN_PAIRS_PER_SOLVE, lr=1e-3)
+        info["steps"] = therm
+    else:
+        ctx.langevin_run(0, therm)
+    ctx.sync()
+    secs = time.perf_counter() - t0
+    if info is not None:
+        info["seconds"] = secs
+    phi = ctx.get_field()
+    if not train:
+        ctx.close()
+        ctx = None
+    return ctx, phi, info, secs
+
+
+def run_method(Y, ctx, size, method, phi0, step0, steps, warmup, profile_kernel):
+    """Time `steps` Langevin steps of one CG method.  Every method starts from the same
+    thermalised field phi0 at the same step counter step0, so all of them follow the same
+    trajectory (same noise; drifts agree to the CG tolerance) and solve the same systems."""
+    if ctx is None:
+        ctx = make_ctx(Y, size, method)
+    tuning = ctx.tuning_report()
+    ctx.set_field(phi0)
+    step = step0
     for _ in range(warmup):
         ctx.langevin_step(step)
         step += 1
@@ -101,10 +143,31 @@ def run_method(Y, size, method, steps, warmup, profile_kernel):
     ctx.sync()
     dt = time.perf_counter() - t0
     prof = ctx.profile_end() if profile_kernel else (0.0, 0, 0.0)
+    mag = float(ctx.get_field().mean())
     ctx.close()
     return {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
             "cg_iters_per_solve": st["iters"] / max(1, steps), "true_rel_resid_max": st["true_rel_resid"],
-            "reliable_updates_or_restarts": st["restarts"], "tuning": tuning, "prof": prof}
+            "reliable_updates_or_restarts": st["restarts"], "final_magnetisation": mag,
+            "tuning": tuning, "prof": prof}
+
+
+def copy_calibration(nbytes: int = 1 << 30):
+    """A plain device copy of `nbytes` (torch's copy kernel), bracketed by writes of two
+    other buffers of the same size so that neither L2 nor the last-level cache holds the
+    source: profiled with rocprofv3 --pmc FETCH_SIZE / WRITE_SIZE, the copy kernel's
+    counters against its known nbytes read + nbytes written give the scale factors that
+    turn the counters of the library's kernels into HBM bytes."""
+    import torch
+    n = nbytes // 4
+    src = torch.ones(n, dtype=torch.float32, device="cuda")
+    evict = torch.full((n,), 2.0, dtype=torch.float32, device="cuda")
+    dst = torch.empty_like(src)
+    evict.add_(1.0)
+    torch.cuda.synchronize()
+    dst.copy_(src)
+    torch.cuda.synchronize()
+    assert float(dst[-1]) == 1.0
+    print(json.dumps({"copy_calibration_bytes": nbytes, "kernel": "torch copy_ (float32)"}), flush=True)
 
 
 def main():
@@ -113,31 +176,47 @@ def main():
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--size", type=int, default=1024, help="Nt = Nx (default: the 1024^2 workload)")
-    ap.add_argument("--method", choices=["fp64", "mixed", "both"], default="both")
+    ap.add_argument("--therm", type=int, default=THERM_STEPS,
+                    help="untimed thermalisation steps shared by every method")
+    ap.add_argument("--method", choices=["fp64", "mixed", "learned", "all"], default="all")
     ap.add_argument("--profile-kernel", default="cg_normal_op")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="run only a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
     args = ap.parse_args()
     if args.gpus != 1:
         print(f"bench.py: this build runs on one GPU; --gpus {args.gpus} is not supported "
               "(the multi-GPU decomposition is design-only)", file=sys.stderr)
         sys.exit(2)
+    if args.copy_calibration:
+        copy_calibration()
+        return
 
     import yukawacolourednoise_amd as Y
 
-    methods = ["fp64", "mixed"] if args.method == "both" else [args.method]
+    methods = ["learned", "fp64", "mixed"] if args.method == "all" else [args.method]
+    train = "learned" in methods
+    lctx, phi0, train_info, therm_s = thermalise(Y, args.size, args.therm, train)
+    print(f"# thermalised {args.therm} steps in {therm_s:.1f} s"
+          + (f" (training on {train_info['pairs']} pairs, loss {train_info['loss_first']:.3f} -> "
+             f"{train_info['loss_last']:.3f})" if train else ""), file=sys.stderr, flush=True)
     res = {}
     for m in methods:
-        # the dominant kernel is profiled in the headline (last) method only
-        res[m] = run_method(Y, args.size, m, args.steps, args.warmup,
-                            args.profile_kernel if m == methods[-1] else None)
+        res[m] = run_method(Y, lctx if m == "learned" else None, args.size, m, phi0, args.therm,
+                            args.steps, args.warmup, args.profile_kernel)
+        if m == "learned":
+            res[m]["training"] = train_info
         print(f"# {m}: {res[m]['steps_per_s']:.3f} steps/s, {res[m]['cg_iters_per_solve']:.0f} CG iters/solve",
               file=sys.stderr, flush=True)
-    head = methods[-1]
+    # headline: the fastest method (every method reaches the same FP64 true residual)
+    head = max(res, key=lambda m: res[m]["steps_per_s"])
     h = res[head]
     k_ms, k_n, k_bytes = h["prof"]
     avg_ms = k_ms / max(1, k_n)
     achieved = (k_bytes / 1e9) / (avg_ms / 1e3) if k_n else 0.0
-    cg_desc = {"fp64": "plain FP64 CG", "mixed": "mixed-precision CG (FP32 inner, FP64 reliable update every 50)"}
+    cg_desc = {"fp64": "plain FP64 CG", "mixed": "mixed-precision CG (FP32 inner, FP64 reliable update every 50)",
+               "learned": "flexible CG with the online-trained BF16 CNN preconditioner (FP32 inner, FP64 reliable "
+                          "update every 50)"}
     out = {
         "metric": "Langevin steps/s @1024^2 (true resid <= 1e-10)",
         "value": h["steps_per_s"],
@@ -149,20 +228,20 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "f64" if head == "fp64" else "f32+f64",
-        "data": "synthetic (Philox hot start, seed 7)",
+        "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "bf16+f32+f64"}[head],
+        "data": f"synthetic (Philox hot start, seed 7, {args.therm} thermalisation steps)",
         "config": {
             "workload": f"{args.size}x{args.size} near-critical Langevin step, {cg_desc[head]} on D^dag D "
                         "to FP64 true relative residual 1e-10, 1 noise vector, fixed eps 0.005",
             "Nt": args.size, "Nx": args.size, "M": WORKLOAD["M"], "g": WORKLOAD["g"], "m2": WORKLOAD["m2"],
             "lambda": WORKLOAD["lam"], "s": WORKLOAD["cutFraction"], "eps": WORKLOAD["epsBar"],
-            "cg": head, "precond": "none", "parallelism": "single GPU",
+            "cg": head, "precond": "cnn" if head == "learned" else "none", "parallelism": "single GPU",
         },
         "cg_iters_per_solve": h["cg_iters_per_solve"],
         "cg_true_rel_resid_max": h["true_rel_resid_max"],
         "methods": {m: {k: v for k, v in r.items() if k != "prof"} for m, r in res.items()},
         "roofline": {
-            "kernel": args.profile_kernel + (" (fp32)" if head == "mixed" else " (fp64)"),
+            "kernel": args.profile_kernel + (" (fp64)" if head == "fp64" else " (fp32)"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
@@ -174,8 +253,12 @@ def main():
             "algorithmic_bytes_per_launch": k_bytes,
         },
     }
-    if "fp64" in res and head != "fp64":
-        out["speedup_vs_plain_fp64"] = h["steps_per_s"] / res["fp64"]["steps_per_s"]
+    if "fp64" in res:
+        out["speedup_vs_plain_fp64"] = {m: r["steps_per_s"] / res["fp64"]["steps_per_s"] for m, r in res.items()}
+    if "learned" in res:
+        base = res.get("mixed", res.get("fp64"))
+        out["cg_iters_learned_vs_plain"] = {"learned": res["learned"]["cg_iters_per_solve"],
+                                            "plain": base["cg_iters_per_solve"] if base else None}
     if not args.no_cpu_baseline:
         out["cpu_baseline"] = cpu_baseline(res.get("fp64", h)["cg_iters_per_solve"])
     print(json.dumps(out), flush=True)
