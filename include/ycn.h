@@ -180,6 +180,7 @@ typedef struct {
     int pairs;             /* (residual, error) pairs trained on                        */
     double loss_first;     /* energy-norm loss of the first pair (before its update)    */
     double loss_last;      /* loss of the last pair                                     */
+    double eps_last;       /* Langevin step size of the last training step              */
 } ycn_train_stats;
 int ycn_precond_set_weights(ycn_ctx *ctx, const float *w);
 int ycn_precond_get_weights(ycn_ctx *ctx, float *w);

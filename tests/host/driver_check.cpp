@@ -60,6 +60,7 @@ int main(int argc, char **argv) {
     const int n = ycn::run_langevin(cfg, b, io);
     std::cout << n << "\n";
     std::cout << "cfg " << cfg.m2 << " " << cfg.lambda << " " << cfg.kappa << " " << cfg.yukawa_coupling << " "
-              << cfg.Nt << " " << cfg.Nx << " " << cfg.cgTol << " " << cfg.cgTolMode << "\n";
+              << cfg.Nt << " " << cfg.Nx << " " << cfg.cgTol << " " << cfg.cgTolMode << " " << cfg.precond << " "
+              << cfg.cnnTrainPairs << " " << cfg.cnnPairsPerSolve << " " << cfg.cnnLearningRate << "\n";
     return 0;
 }

@@ -171,7 +171,8 @@ def test_parameter_mapping(driver, tmp_path):
     assert float(cfg[1]) == pytest.approx((1 - 2 * L) / k - 4, rel=1e-5)
     assert float(cfg[2]) == pytest.approx(6 * L / k ** 2, rel=1e-5)
     assert float(cfg[4]) == pytest.approx(1.0 / math.sqrt(2 * k), rel=1e-5)
-    assert cfg[5:] == ["6", "8", "1e-08", "0"]
+    assert cfg[5:9] == ["6", "8", "1e-08", "0"]
+    assert cfg[9:] == ["0", "2000", "16", "0.001"]          # no preconditioner; training defaults
     # useMass = "true": m2 = mass, lambda = g, kappa = FindKappa (main.cu:58-63,117-122)
     out, _ = run(driver, tmp_path, useMass="true")
     cfg = out[1].split()
@@ -180,6 +181,22 @@ def test_parameter_mapping(driver, tmp_path):
     assert float(cfg[1]) == pytest.approx(m2) and float(cfg[2]) == pytest.approx(lam)
     assert float(cfg[3]) == pytest.approx(kap, rel=1e-5)
     assert float(cfg[4]) == pytest.approx(1.0)
+
+
+def test_cnn_keys(driver, tmp_path):
+    """[cg] precond = "cnn" with its online-training keys."""
+    text = INPUT.format(useMass="false", tmax=1.0, adaptive="true")
+    text += '\n[cg]\nprecond = "cnn"\ncnnTrainPairs = 64\ncnnPairsPerSolve = 8\ncnnLearningRate = 0.002\n'
+    assert 'precond = "cnn"' in text
+    inp = tmp_path / "input.toml"
+    inp.write_text(text)
+    r = subprocess.run([driver, str(inp), str(tmp_path)], check=True, capture_output=True, text=True)
+    cfg = r.stdout.split("\n")[1].split()
+    assert cfg[9:] == ["1", "64", "8", "0.002"]
+    bad = tmp_path / "bad.toml"
+    bad.write_text(text.replace("cnnPairsPerSolve = 8", "cnnPairsPerSolve = 0"))
+    r = subprocess.run([driver, str(bad), str(tmp_path)], capture_output=True, text=True)
+    assert r.returncode != 0
 
 
 def test_reference_input_parses(driver, tmp_path):

@@ -71,3 +71,32 @@ def test_gradient_finite_differences(seed):
     Ad = O.normal_op(p, phi, d.reshape(-1, 4), 1).reshape(-1)
     L1, _ = O.cnn_loss_grad(p, phi, np.zeros(O.CNN_NW, np.float32), r, e, c0=c0)
     assert abs(L1 - np.vdot(d, Ad).real / np.vdot(e.reshape(-1), r.reshape(-1)).real) < 1e-12
+
+
+@pytest.mark.parametrize("t0,x0", [(0, 0), (5, 9)])
+def test_tap_orientation_and_channel_order(t0, x0):
+    """Pins the weight layout [tap][in][out], tap = (dt+1)*3 + (dx+1), out(site) =
+    sum W * in(site + (dt, dx)), periodic wrap, and the channel map 2c / 2c+1 = Re / Im
+    of component c: a delta in channel 0 (Re of component 0) at (t0, x0) through a
+    (dt, dx) = (+1, 0) tap 0 -> 5, a centre tap 5 -> 7 and a (0, +1) tap 7 -> 2 lands in
+    channel 2 (Re of component 1) at (t0 - 1, x0 - 1) only, with the value fixed by the
+    two tanh layers."""
+    Nt, Nx = 8, 12
+    V = Nt * Nx
+    a, b = 0.01, 0.7
+    W = np.zeros(O.CNN_NW, np.float64)
+    n1, n2 = 9 * 8 * 16, 9 * 16 * 16
+    W[((1 + 1) * 3 + (0 + 1)) * 8 * 16 + 0 * 16 + 5] = a           # W1 tap (+1, 0), in 0 -> out 5
+    W[n1 + (4 * 16 + 5) * 16 + 7] = 1.0                             # W2 centre, 5 -> 7
+    W[n1 + n2 + ((0 + 1) * 3 + (1 + 1)) * 16 * 8 + 7 * 8 + 2] = b   # W3 tap (0, +1), 7 -> 2
+    W = W.astype(np.float32)
+    r = np.zeros((V, 4), np.complex128)
+    v = 2.5
+    r[t0 * Nx + x0, 0] = v
+    z = O.cnn_forward(Nt, Nx, W, r, bf16=False, c0=0.0)
+    s = v / np.sqrt(8 * V)
+    want = np.zeros((V, 4), np.complex128)
+    tp, xp = (t0 - 1) % Nt, (x0 - 1) % Nx
+    af, bf = float(np.float32(a)), float(np.float32(b))                 # the weights as stored
+    want[tp * Nx + xp, 1] = s * bf * np.tanh(np.tanh(af * v / s))
+    assert np.abs(z - want).max() < 1e-12 * abs(want).max()

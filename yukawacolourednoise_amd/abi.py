@@ -75,7 +75,8 @@ class Observables(C.Structure):
 
 
 class TrainStats(C.Structure):
-    _fields_ = [("pairs", C.c_int), ("loss_first", C.c_double), ("loss_last", C.c_double)]
+    _fields_ = [("pairs", C.c_int), ("loss_first", C.c_double), ("loss_last", C.c_double),
+                ("eps_last", C.c_double)]
 
     def as_dict(self):
         return {k: getattr(self, k) for k, _ in self._fields_}

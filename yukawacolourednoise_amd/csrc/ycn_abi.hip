@@ -540,9 +540,13 @@ int ycn_precond_train(ycn_ctx *ctx, uint64_t step0, int nsteps, int pairsPerSolv
     int used = 0;
     if ((rc = train_preconditioner(c, step0, nsteps, pairsPerSolve, (float)lr, &used))) return rc;
     if (stats) {
+        StepState S{};
+        YCN_HIP(hipMemcpyAsync(&S, c.step_d, sizeof(StepState), hipMemcpyDeviceToHost, c.stream));
+        YCN_HIP(hipStreamSynchronize(c.stream));
         stats->pairs = used;
         stats->loss_first = c.trainLossFirst;
         stats->loss_last = c.trainLossLast;
+        stats->eps_last = S.eps;
     }
     return YCN_OK;
 }

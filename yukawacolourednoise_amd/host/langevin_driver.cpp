@@ -81,6 +81,11 @@ RunConfig RunConfig::from_toml(const Toml &t) {
     c.cgPrecision = parse_choice(t.opt_string("cg", "precision", "fp64"), {"fp64", "mixed"}, "cg.precision");
     c.reliableEvery = (int)t.opt_int("cg", "reliableEvery", 50);
     c.precond = parse_choice(t.opt_string("cg", "precond", "none"), {"none", "cnn"}, "cg.precond");
+    c.cnnTrainPairs = (int)t.opt_int("cg", "cnnTrainPairs", 2000);
+    c.cnnPairsPerSolve = (int)t.opt_int("cg", "cnnPairsPerSolve", 16);
+    c.cnnLearningRate = t.opt_double("cg", "cnnLearningRate", 1e-3);
+    if (c.cnnTrainPairs < 0 || c.cnnPairsPerSolve < 1 || !(c.cnnLearningRate > 0))
+        throw std::runtime_error("cg.cnnTrainPairs >= 0, cg.cnnPairsPerSolve >= 1 and cg.cnnLearningRate > 0 required");
     c.hotStart = parse_choice(t.opt_string("start", "mode", "hot"), {"cold", "hot"}, "start.mode") == 1;
     c.initSeed = t.opt_int("start", "initSeed", 0);
     c.nReplicas = (int)t.opt_int("batch", "nReplicas", 1);

@@ -40,6 +40,10 @@ struct RunConfig {
     // [cg]  (params.h:13-15 defaults)
     double cgTol = 1e-8;
     int cgTolMode = 0, cgMaxIter = 1000, cgPrecision = 0, reliableEvery = 50, precond = 0;
+    // [cg] learned preconditioner: the first Langevin steps train it online until it has
+    // seen cnnTrainPairs (residual, error) pairs, cnnPairsPerSolve per fermion solve
+    int cnnTrainPairs = 2000, cnnPairsPerSolve = 16;
+    double cnnLearningRate = 1e-3;
     // [start]
     bool hotStart = true;
     long long initSeed = 0;

@@ -80,7 +80,18 @@ void YcnBackend::start(bool hot, uint64_t initSeed) {
     }
 }
 
+// With the learned preconditioner the first steps are training steps: the same Langevin
+// step, its fermion solve by plain FP64 CG, whose iterates give the (residual, error)
+// pairs for online training, until cnnTrainPairs pairs have been used.
 double YcnBackend::step(uint64_t counter) {
+    if (cfg_.precond == YCN_PRECOND_CNN && trainedPairs_ < cfg_.cnnTrainPairs) {
+        ycn_train_stats ts{};
+        check(ctx_.get(),
+              ycn_precond_train(ctx_.get(), counter, 1, cfg_.cnnPairsPerSolve, cfg_.cnnLearningRate, &ts),
+              "ycn_precond_train");
+        trainedPairs_ += ts.pairs > 0 ? ts.pairs : 1;
+        return ts.eps_last;
+    }
     check(ctx_.get(), ycn_langevin_step(ctx_.get(), counter, eps_.data(), &last_), "ycn_langevin_step", true);
     return eps_[0];
 }

@@ -100,6 +100,7 @@ private:
     Context ctx_;
     ycn_cg_stats last_{};
     std::vector<double> eps_;
+    int trainedPairs_ = 0;   // pairs the learned preconditioner has been trained on
 };
 
 }  // namespace ycn
