@@ -34,8 +34,8 @@ HBM_PEAK_GBS = 8000.0
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by what the same counters read
 # for a 1 GiB device copy (FETCH_SIZE 0.5 KiB, WRITE_SIZE 1.0 KiB per KiB moved); the
 # summary with its commands is profiles/hbm_traffic_mixed_cg_1024x1024.md.  Keyed by
-# (precision, lattice extent); other sizes report null.
-MEASURED_TRAFFIC = {("fp32", 1024): 140720905}
+# (kernel, precision, lattice extent); others report null.
+MEASURED_TRAFFIC = {("cg_normal_op", "fp32", 1024): 140720905}
 
 WORKLOAD = dict(Nt=1024, Nx=1024, m2=-0.4, lam=1.0, g=0.5, M=0.05, r=1.0, cutFraction=0.5,
                 driftMode=0, epsBar=0.005, adaptive=0, seed=1234, cgTol=1e-10, cgTolMode=1,
@@ -253,7 +253,7 @@ def main():
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC.get(("fp64" if head == "fp64" else "fp32", args.size)),
+            "traffic": MEASURED_TRAFFIC.get((args.profile_kernel, "fp64" if head == "fp64" else "fp32", args.size)),
             "avg_launch_us": avg_ms * 1e3,
             "launches": k_n,
             "algorithmic_bytes_per_launch": k_bytes,

@@ -48,15 +48,19 @@ Nx = 8
 """
 
 
-@pytest.fixture(scope="module")
-def driver(tmp_path_factory):
-    d = tmp_path_factory.mktemp("drv")
+def build_driver(d):
+    """Compile the driver loop with the deterministic test backend into directory d."""
     exe = str(d / "driver_check")
     subprocess.run(["g++", "-O1", "-std=c++17", "-I", HOST, "-o", exe,
                     os.path.join(ROOT, "tests", "host", "driver_check.cpp"),
                     os.path.join(HOST, "langevin_driver.cpp"), os.path.join(HOST, "toml_lite.cpp")],
                    check=True)
     return exe
+
+
+@pytest.fixture(scope="module")
+def driver(tmp_path_factory):
+    return build_driver(tmp_path_factory.mktemp("drv"))
 
 
 def run(driver, tmp_path, usr2=None, **kw):
