@@ -97,8 +97,9 @@ def make_ctx(Y, size, method):
     wl = dict(WORKLOAD)
     wl["Nt"] = wl["Nx"] = size
     p = Y.default_params(**wl)
-    p.cgPrecision = Y.YCN_CG_FP64 if method == "fp64" else Y.YCN_CG_MIXED
+    p.cgPrecision = Y.YCN_CG_FP64 if method in ("fp64", "eo-fp64") else Y.YCN_CG_MIXED
     p.precond = Y.YCN_PRECOND_CNN if method == "learned" else Y.YCN_PRECOND_NONE
+    p.evenOdd = 1 if method.startswith("eo-") else 0
     p.reliableEvery = 50
     return Y.Context(p)
 
@@ -184,7 +185,7 @@ def main():
     ap.add_argument("--size", type=int, default=1024, help="Nt = Nx (default: the 1024^2 workload)")
     ap.add_argument("--therm", type=int, default=THERM_STEPS,
                     help="untimed thermalisation steps shared by every method")
-    ap.add_argument("--method", choices=["fp64", "mixed", "learned", "all"], default="all")
+    ap.add_argument("--method", choices=["fp64", "mixed", "learned", "eo-fp64", "eo-mixed", "all"], default="all")
     ap.add_argument("--profile-kernel", default="cg_normal_op")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -200,7 +201,7 @@ def main():
 
     import yukawacolourednoise_amd as Y
 
-    methods = ["learned", "fp64", "mixed"] if args.method == "all" else [args.method]
+    methods = ["learned", "fp64", "mixed", "eo-fp64", "eo-mixed"] if args.method == "all" else [args.method]
     train = "learned" in methods
     lctx, phi0, train_info, therm_s = thermalise(Y, args.size, args.therm, train)
     print(f"# thermalised {args.therm} steps in {therm_s:.1f} s"
@@ -222,7 +223,10 @@ def main():
     achieved = (k_bytes / 1e9) / (avg_ms / 1e3) if k_n else 0.0
     cg_desc = {"fp64": "plain FP64 CG", "mixed": "mixed-precision CG (FP32 inner, FP64 reliable update every 50)",
                "learned": "flexible CG with the online-trained BF16 CNN preconditioner (FP32 inner, FP64 reliable "
-                          "update every 50)"}
+                          "update every 50)",
+               "eo-fp64": "even-odd preconditioned FP64 CG (Schur complement on the even sites)",
+               "eo-mixed": "even-odd preconditioned mixed-precision CG (Schur complement on the even sites, FP32 "
+                           "inner, FP64 reliable update every 50)"}
     out = {
         "metric": "Langevin steps/s @1024^2 (true resid <= 1e-10)",
         "value": h["steps_per_s"],
@@ -234,26 +238,30 @@ def main():
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "bf16+f32+f64"}[head],
+        "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "bf16+f32+f64", "eo-fp64": "f64",
+                  "eo-mixed": "f32+f64"}[head],
         "data": f"synthetic (Philox hot start, seed 7, {args.therm} thermalisation steps)",
         "config": {
             "workload": f"{args.size}x{args.size} near-critical Langevin step, {cg_desc[head]} on D^dag D "
                         "to FP64 true relative residual 1e-10, 1 noise vector, fixed eps 0.005",
             "Nt": args.size, "Nx": args.size, "M": WORKLOAD["M"], "g": WORKLOAD["g"], "m2": WORKLOAD["m2"],
             "lambda": WORKLOAD["lam"], "s": WORKLOAD["cutFraction"], "eps": WORKLOAD["epsBar"],
-            "cg": head, "precond": "cnn" if head == "learned" else "none", "parallelism": "single GPU",
+            "cg": head, "precond": {"learned": "cnn", "eo-fp64": "even-odd", "eo-mixed": "even-odd"}.get(head, "none"),
+            "parallelism": "single GPU",
         },
         "cg_iters_per_solve": h["cg_iters_per_solve"],
         "cg_true_rel_resid_max": h["true_rel_resid_max"],
         "methods": {m: {k: v for k, v in r.items() if k != "prof"} for m, r in res.items()},
         "roofline": {
-            "kernel": args.profile_kernel + (" (fp64)" if head == "fp64" else " (fp32)"),
+            "kernel": args.profile_kernel + (" even-odd" if head.startswith("eo-") else "")
+            + (" (fp64)" if head in ("fp64", "eo-fp64") else " (fp32)"),
             "bound": "hbm",
             "achieved": achieved,
             "peak": HBM_PEAK_GBS,
             "unit": "GB/s",
             "frac": achieved / HBM_PEAK_GBS,
-            "traffic": MEASURED_TRAFFIC.get((args.profile_kernel, "fp64" if head == "fp64" else "fp32", args.size)),
+            "traffic": MEASURED_TRAFFIC.get((args.profile_kernel + ("-eo" if head.startswith("eo-") else ""),
+                                             "fp64" if head in ("fp64", "eo-fp64") else "fp32", args.size)),
             "avg_launch_us": avg_ms * 1e3,
             "launches": k_n,
             "algorithmic_bytes_per_launch": k_bytes,

@@ -29,7 +29,7 @@
 extern "C" {
 #endif
 
-#define YCN_ABI_VERSION 1
+#define YCN_ABI_VERSION 2
 
 enum {
     YCN_OK = 0,
@@ -73,6 +73,17 @@ typedef struct {
     int reliableEvery;     /* mixed precision: FP64 reliable update period (50)         */
     int precond;           /* YCN_PRECOND_*                                             */
     int device;            /* HIP device ordinal                                        */
+    int evenOdd;           /* 1: fermion force by the even-odd preconditioned solve of
+                              D^dag chi = eta (Schur complement on the even sites; the
+                              reference's even-odd blocks, Dirac.cu:256-379, are
+                              commented out).  Same stopping rule on |eta - D^dag chi|,
+                              FP64 or mixed precision per cgPrecision.  Needs a fused
+                              tile dividing the lattice: Nt even and Nx % 32 == 0, or
+                              Nt % 16 == 0 and Nx % 16 == 0.  A_o^-1 needs the diagonal
+                              2r + M + g phi away from zero (Wilson r > 0); a solve that
+                              cannot reach tol returns YCN_ERR_NOT_CONVERGED.  Exclusive
+                              with the learned preconditioner.  Default 0 (the
+                              reference's solver).                                      */
 } ycn_params;
 
 typedef struct {

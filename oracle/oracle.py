@@ -27,7 +27,7 @@ class OrcParams(C.Structure):
         ("driftMode", C.c_int), ("epsBar", C.c_double), ("adaptive", C.c_int),
         ("seed", C.c_uint64),
         ("cgTol", C.c_double), ("cgTolMode", C.c_int), ("cgMaxIter", C.c_int),
-        ("nNoise", C.c_int),
+        ("nNoise", C.c_int), ("evenOdd", C.c_int),
     ]
 
 
@@ -85,6 +85,11 @@ def lib():
         L.orc_cnn_loss_grad.restype = C.c_double
         L.orc_bf16_round.argtypes = [C.c_double]
         L.orc_bf16_round.restype = C.c_double
+        L.orc_eo_schur.argtypes = [pp, dp, dp, dp, C.c_int]
+        L.orc_eo_rhs.argtypes = [pp, dp, dp, dp]
+        L.orc_eo_reconstruct.argtypes = [pp, dp, dp, dp, dp]
+        L.orc_eo_cg.argtypes = [pp, dp, dp, dp, C.c_int, sp]
+        L.orc_eo_cg.restype = C.c_int
         L.orc_set_threads.argtypes = [C.c_int]
         L.orc_set_threads.restype = C.c_int
         _lib = L
@@ -93,9 +98,9 @@ def lib():
 
 def params(Nt=16, Nx=16, m2=-0.2, lam=0.5, g=0.3, M=0.3, r=1.0, cutFraction=0.5, driftMode=0,
            epsBar=0.01, adaptive=0, seed=1234, cgTol=1e-10, cgTolMode=1, cgMaxIter=10000,
-           nNoise=1) -> OrcParams:
+           nNoise=1, evenOdd=0) -> OrcParams:
     return OrcParams(Nt, Nx, m2, lam, g, M, r, cutFraction, driftMode, epsBar, adaptive, seed,
-                     cgTol, cgTolMode, cgMaxIter, nNoise)
+                     cgTol, cgTolMode, cgMaxIter, nNoise, evenOdd)
 
 
 def _f64(a):
@@ -242,6 +247,48 @@ def correlator(p: OrcParams, phi):
     st = OrcCgStats()
     lib().orc_correlator(C.byref(p), _f64(phi).reshape(-1), out, C.byref(st))
     return out, st.as_dict()
+
+
+def eo_index(Nt, Nx, parity):
+    """Full-lattice site indices of the compact half-lattice vector of `parity`, in
+    compact order (t-major, x >> 1)."""
+    t, x = np.meshgrid(np.arange(Nt), np.arange(Nx), indexing="ij")
+    sites = (t * Nx + x).ravel()
+    par = ((t + x) & 1).ravel()
+    sel = sites[par == parity]
+    comp = (t.ravel()[par == parity] * (Nx // 2) + (x.ravel()[par == parity] >> 1))
+    out = np.empty_like(sel)
+    out[comp] = sel
+    return out
+
+
+def eo_schur(p: OrcParams, phi, in_e, dagger=True):
+    Vh = p.Nt * p.Nx // 2
+    out = np.empty(8 * Vh, np.float64)
+    lib().orc_eo_schur(C.byref(p), _f64(phi).reshape(-1), _spin_to_f64(in_e), out, int(dagger))
+    return _f64_to_spin(out, Vh)
+
+
+def eo_rhs(p: OrcParams, phi, eta):
+    Vh = p.Nt * p.Nx // 2
+    out = np.empty(8 * Vh, np.float64)
+    lib().orc_eo_rhs(C.byref(p), _f64(phi).reshape(-1), _spin_to_f64(eta), out)
+    return _f64_to_spin(out, Vh)
+
+
+def eo_reconstruct(p: OrcParams, phi, eta, chi_e):
+    V = p.Nt * p.Nx
+    out = np.empty(8 * V, np.float64)
+    lib().orc_eo_reconstruct(C.byref(p), _f64(phi).reshape(-1), _spin_to_f64(eta), _spin_to_f64(chi_e), out)
+    return _f64_to_spin(out, V)
+
+
+def eo_cg(p: OrcParams, phi, eta, max_iter_override=0):
+    V = p.Nt * p.Nx
+    chi = np.empty(8 * V, np.float64)
+    st = OrcCgStats()
+    lib().orc_eo_cg(C.byref(p), _f64(phi).reshape(-1), _spin_to_f64(eta), chi, max_iter_override, C.byref(st))
+    return _f64_to_spin(chi, V), st.as_dict()
 
 
 CNN_NW = 9 * 8 * 16 + 9 * 16 * 16 + 9 * 16 * 8

@@ -444,11 +444,15 @@ int orc_fermion_drift_eta(const orc_params *p, const double *phi, const double *
     for (int j = 0; j < p->nNoise; ++j) {
         const cplx *eta = (const cplx *)(eta_ + (size_t)j * 2 * n);
         orc_cg_stats s;
-        fails += orc_cg(p, phi, (const double *)eta, y, 1, 0, &s);
+        if (p->evenOdd) {
+            fails += orc_eo_cg(p, phi, (const double *)eta, chi, 0, &s);   /* chi = (D^dag)^-1 eta */
+        } else {
+            fails += orc_cg(p, phi, (const double *)eta, y, 1, 0, &s);
+            orc_dirac(p, phi, y, chi, 0);                                   /* chi = D y */
+        }
         acc.iters += s.iters; acc.restarts += s.restarts; acc.converged &= s.converged;
         if (s.true_rel_resid > acc.true_rel_resid) acc.true_rel_resid = s.true_rel_resid;
         acc.rr = s.rr;
-        orc_dirac(p, phi, y, chi, 0);
         const cplx *c = (const cplx *)chi;
 #pragma omp parallel for schedule(static)
         for (int i = 0; i < V; ++i) {
@@ -544,6 +548,173 @@ int orc_langevin_step(const orc_params *p, double *phi, uint64_t step, double *e
     orc_bosonic_update(p, phi, drift, noise, eps);
     free(noise); free(drift);
     return rc;
+}
+
+/* ========================================================================== */
+/* even-odd preconditioned fermion force (no active reference counterpart: the  */
+/* reference's even-odd hopping blocks D_eo / D_oe are commented out,           */
+/* Dirac.cu:256-379; their site ordering -- t-major, x >> 1 within a parity --  */
+/* is the compact layout used here)                                             */
+/* ========================================================================== */
+/* chi = (D^dag)^-1 eta is all the drift needs (chi = D y with D^dag D y = eta).  With
+ * D^dag = A + K (A = diag(2r + M + g phi), K the hopping, connecting even and odd
+ * sites) and the Schur complement S = A_e - K_eo A_o^-1 K_oe on the even sites:
+ *   b_e = eta_e - K_eo A_o^-1 eta_o,  S chi_e = b_e,  chi_o = A_o^-1 (eta_o - K_oe chi_e).
+ * S chi_e = b_e is solved by CG on S^T S (S^T is the Schur complement of D).  Here
+ * every block is formed from full-lattice Dirac applications (orc_dirac) on vectors
+ * that vanish on one parity, plus the diagonal. Needs even Nt and Nx. */
+static inline int eo_par(int t, int x) { return (t + x) & 1; }
+
+/* compact half-lattice index of site (t, x) within its parity */
+static inline size_t eo_idx(int Nx, int t, int x) { return (size_t)t * (Nx / 2) + (size_t)(x >> 1); }
+
+static double eo_diag(const orc_params *p, const double *phi, int t, int x) {
+    return 2.0 * p->r + p->M + p->g * phi[(size_t)t * p->Nx + x];
+}
+
+/* full AoS vector from a compact vector on parity `par` (zero on the other parity) */
+static void eo_scatter(const orc_params *p, int par, const cplx *in, cplx *full) {
+    const int Nt = p->Nt, Nx = p->Nx;
+    for (int t = 0; t < Nt; ++t)
+        for (int x = 0; x < Nx; ++x)
+            for (int k = 0; k < 4; ++k) {
+                cplx z = {0.0, 0.0};
+                full[4 * ((size_t)t * Nx + x) + k] = eo_par(t, x) == par ? in[4 * eo_idx(Nx, t, x) + k] : z;
+            }
+}
+
+static void eo_gather(const orc_params *p, int par, const cplx *full, cplx *out) {
+    const int Nt = p->Nt, Nx = p->Nx;
+    for (int t = 0; t < Nt; ++t)
+        for (int x = 0; x < Nx; ++x)
+            if (eo_par(t, x) == par)
+                for (int k = 0; k < 4; ++k) out[4 * eo_idx(Nx, t, x) + k] = full[4 * ((size_t)t * Nx + x) + k];
+}
+
+/* out_e = S in_e (dagger = 1: S of D^dag) or S^T in_e (dagger = 0: S of D) */
+void orc_eo_schur(const orc_params *p, const double *phi, const double *in_e, double *out_e, int dagger) {
+    const int Nt = p->Nt, Nx = p->Nx;
+    const size_t V = (size_t)Nt * Nx, Vh = V / 2;
+    cplx *v = (cplx *)malloc(sizeof(cplx) * 4 * V), *w = (cplx *)malloc(sizeof(cplx) * 4 * V);
+    cplx *u = (cplx *)malloc(sizeof(cplx) * 4 * Vh), *we = (cplx *)malloc(sizeof(cplx) * 4 * Vh);
+    cplx *ye = (cplx *)malloc(sizeof(cplx) * 4 * Vh);
+    eo_scatter(p, 0, (const cplx *)in_e, v);
+    orc_dirac(p, phi, (const double *)v, (double *)w, dagger);      /* w_e = A_e in, w_o = K_oe in */
+    eo_gather(p, 0, w, we);
+    eo_gather(p, 1, w, u);
+    for (int t = 0; t < Nt; ++t)
+        for (int x = 0; x < Nx; ++x)
+            if (eo_par(t, x) == 1) {
+                const double d = eo_diag(p, phi, t, x);
+                for (int k = 0; k < 4; ++k) u[4 * eo_idx(Nx, t, x) + k] = c_scl(1.0 / d, u[4 * eo_idx(Nx, t, x) + k]);
+            }
+    eo_scatter(p, 1, u, v);
+    orc_dirac(p, phi, (const double *)v, (double *)w, dagger);      /* w_e = K_eo A_o^-1 K_oe in */
+    eo_gather(p, 0, w, ye);
+    cplx *out = (cplx *)out_e;
+    for (size_t i = 0; i < 4 * Vh; ++i) out[i] = c_sub(we[i], ye[i]);
+    free(v); free(w); free(u); free(we); free(ye);
+}
+
+/* b_e = eta_e - K_eo A_o^-1 eta_o */
+void orc_eo_rhs(const orc_params *p, const double *phi, const double *eta_, double *b_e) {
+    const int Nt = p->Nt, Nx = p->Nx;
+    const size_t V = (size_t)Nt * Nx, Vh = V / 2;
+    const cplx *eta = (const cplx *)eta_;
+    cplx *u = (cplx *)malloc(sizeof(cplx) * 4 * Vh), *v = (cplx *)malloc(sizeof(cplx) * 4 * V);
+    cplx *w = (cplx *)malloc(sizeof(cplx) * 4 * V), *ye = (cplx *)malloc(sizeof(cplx) * 4 * Vh);
+    eo_gather(p, 1, eta, u);
+    for (int t = 0; t < Nt; ++t)
+        for (int x = 0; x < Nx; ++x)
+            if (eo_par(t, x) == 1) {
+                const double d = eo_diag(p, phi, t, x);
+                for (int k = 0; k < 4; ++k) u[4 * eo_idx(Nx, t, x) + k] = c_scl(1.0 / d, u[4 * eo_idx(Nx, t, x) + k]);
+            }
+    eo_scatter(p, 1, u, v);
+    orc_dirac(p, phi, (const double *)v, (double *)w, 1);
+    eo_gather(p, 0, w, ye);
+    eo_gather(p, 0, eta, u);
+    cplx *b = (cplx *)b_e;
+    for (size_t i = 0; i < 4 * Vh; ++i) b[i] = c_sub(u[i], ye[i]);
+    free(u); free(v); free(w); free(ye);
+}
+
+/* full chi from chi_e: chi_o = A_o^-1 (eta_o - K_oe chi_e) */
+void orc_eo_reconstruct(const orc_params *p, const double *phi, const double *eta_, const double *chi_e,
+                        double *chi_) {
+    const int Nt = p->Nt, Nx = p->Nx;
+    const size_t V = (size_t)Nt * Nx;
+    const cplx *eta = (const cplx *)eta_;
+    cplx *v = (cplx *)malloc(sizeof(cplx) * 4 * V), *w = (cplx *)malloc(sizeof(cplx) * 4 * V);
+    cplx *chi = (cplx *)chi_;
+    eo_scatter(p, 0, (const cplx *)chi_e, v);
+    orc_dirac(p, phi, (const double *)v, (double *)w, 1);            /* w_o = K_oe chi_e */
+    for (int t = 0; t < Nt; ++t)
+        for (int x = 0; x < Nx; ++x) {
+            const size_t s = (size_t)t * Nx + x;
+            for (int k = 0; k < 4; ++k)
+                chi[4 * s + k] = eo_par(t, x) == 0 ? v[4 * s + k]
+                                                   : c_scl(1.0 / eo_diag(p, phi, t, x), c_sub(eta[4 * s + k], w[4 * s + k]));
+        }
+    free(v); free(w);
+}
+
+/* CG on S^T S chi_e = S^T b_e, x0 = 0 (CGNR): p' = r + beta p, v = S p', Ap = S^T v,
+ * alpha = |r|^2 / |v|^2, x += alpha p', r -= alpha Ap, beta = |r'|^2 / |r|^2.  Stop on
+ * |r| <= tol |S^T b| (cgTolMode 1) or |r| <= tol (0); then the true residual of the
+ * unpreconditioned system, |eta - D^dag chi| / |eta| = |b_e - S chi_e| / |eta|; in mode 1,
+ * while it exceeds tol, the iteration continues (same Krylov state) with the threshold
+ * lowered to 0.5 |r| tol / true residual, at most 4 times. */
+int orc_eo_cg(const orc_params *p, const double *phi, const double *eta_, double *chi_, int maxIterOverride,
+              orc_cg_stats *st) {
+    const size_t V = (size_t)p->Nt * p->Nx, n = 4 * (V / 2);
+    cplx *b = (cplx *)malloc(sizeof(cplx) * n), *x = (cplx *)malloc(sizeof(cplx) * n);
+    cplx *r = (cplx *)malloc(sizeof(cplx) * n), *pp = (cplx *)malloc(sizeof(cplx) * n);
+    cplx *v = (cplx *)malloc(sizeof(cplx) * n), *Ap = (cplx *)malloc(sizeof(cplx) * n);
+    const int maxIter = maxIterOverride > 0 ? maxIterOverride : p->cgMaxIter;
+    orc_eo_rhs(p, phi, eta_, (double *)b);
+    orc_eo_schur(p, phi, (const double *)b, (double *)r, 0);         /* r = S^T b */
+    const double etanorm = sqrt(cabs_(cdot(4 * V, (const cplx *)eta_, (const cplx *)eta_)));
+    const double cnorm = sqrt(cabs_(cdot(n, r, r)));
+    double stopAbs = p->cgTolMode == 0 ? p->cgTol : p->cgTol * cnorm;
+    int k = 0, restarts = 0, converged = 0;
+    double rr = cabs_(cdot(n, r, r)), trueRel = 0.0, beta = 0.0;
+    for (size_t i = 0; i < n; ++i) { x[i].re = 0.0; x[i].im = 0.0; pp[i].re = 0.0; pp[i].im = 0.0; }
+    for (;;) {
+        while (k < maxIter && sqrt(rr) > stopAbs) {
+            for (size_t i = 0; i < n; ++i) pp[i] = c_add(r[i], c_scl(beta, pp[i]));
+            orc_eo_schur(p, phi, (const double *)pp, (double *)v, 1);
+            orc_eo_schur(p, phi, (const double *)v, (double *)Ap, 0);
+            const double vv = cabs_(cdot(n, v, v));
+            const double alpha = rr / vv;
+            for (size_t i = 0; i < n; ++i) {
+                x[i] = c_add(x[i], c_scl(alpha, pp[i]));
+                r[i] = c_sub(r[i], c_scl(alpha, Ap[i]));
+            }
+            const double rrn = cabs_(cdot(n, r, r));
+            beta = rrn / rr;
+            rr = rrn;
+            ++k;
+        }
+        converged = sqrt(rr) <= stopAbs;
+        orc_eo_schur(p, phi, (const double *)x, (double *)v, 1);        /* res = b - S x */
+        for (size_t i = 0; i < n; ++i) v[i] = c_sub(b[i], v[i]);
+        const double tr = sqrt(cabs_(cdot(n, v, v)));
+        trueRel = etanorm > 0 ? tr / etanorm : tr;
+        if (p->cgTolMode == 1 && trueRel > p->cgTol && k < maxIter && restarts < 4) {
+            /* the normal-equation residual met its threshold but the system residual did
+             * not: keep the Krylov state and tighten the threshold by the missing factor */
+            stopAbs = sqrt(rr) * 0.5 * p->cgTol / trueRel;
+            ++restarts;
+            continue;
+        }
+        if (p->cgTolMode == 1) converged = trueRel <= p->cgTol;
+        break;
+    }
+    orc_eo_reconstruct(p, phi, eta_, (const double *)x, chi_);
+    if (st) { st->iters = k; st->restarts = restarts; st->converged = converged; st->rr = rr; st->true_rel_resid = trueRel; }
+    free(b); free(x); free(r); free(pp); free(v); free(Ap);
+    return converged ? 0 : 1;
 }
 
 /* ========================================================================== */

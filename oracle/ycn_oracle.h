@@ -44,6 +44,8 @@ typedef struct {
                                   restarted from x while it exceeds tol               */
     int cgMaxIter;          /* params.h:15 (1000)                                     */
     int nNoise;             /* noise vectors per drift estimate (reference: 1)        */
+    int evenOdd;            /* 1: fermion force through the even-odd preconditioned
+                               solve (orc_eo_cg); 0: CG on D^dag D as the reference */
 } orc_params;
 
 typedef struct {
@@ -105,6 +107,17 @@ double orc_sum(int64_t n, const double *v);
 void orc_time_slices(int Nt, int Nx, const double *phi, double *slices);
 /* point-source propagator: CG(DD^dag) then D^dag; corr[t] = sum_x sum_k Re psi */
 int orc_correlator(const orc_params *p, const double *phi, double *corr, orc_cg_stats *st);
+
+/* ---- even-odd preconditioning (Schur complement on the even sites; the reference's
+ *      even-odd blocks are commented out, Dirac.cu:256-379).  Compact half-lattice
+ *      vectors: AoS, site index t * (Nx/2) + (x >> 1) within a parity.  Even Nt, Nx. */
+void orc_eo_schur(const orc_params *p, const double *phi, const double *in_e, double *out_e, int dagger);
+void orc_eo_rhs(const orc_params *p, const double *phi, const double *eta, double *b_e);
+void orc_eo_reconstruct(const orc_params *p, const double *phi, const double *eta, const double *chi_e,
+                        double *chi);
+/* chi = (D^dag)^-1 eta (full AoS) by CG on S^T S; stats as orc_cg */
+int orc_eo_cg(const orc_params *p, const double *phi, const double *eta, double *chi, int maxIterOverride,
+              orc_cg_stats *st);
 
 /* ---- learned preconditioner (no reference counterpart) ------------------------
  * 3 layers of 3x3 periodic convolutions, 8 -> 16 -> 16 -> 8 channels, no bias, tanh

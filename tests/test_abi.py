@@ -30,7 +30,7 @@ def test_library_exports_every_declared_symbol():
 
 
 def test_abi_version_and_defaults():
-    assert Y.lib.ycn_abi_version() == 1
+    assert Y.lib.ycn_abi_version() == 2
     p = Y.default_params()
     # reference defaults: params.h:13-15, input.toml
     assert (p.Nt, p.Nx, p.cgTol, p.cgMaxIter, p.cgTolMode) == (16, 16, 1e-8, 1000, 0)

@@ -1,0 +1,139 @@
+"""Even-odd preconditioned fermion force on the GPU against the oracle: fixed-iteration
+iterates of the Schur-complement CG (same recurrence on both sides) to 1e-12, converged
+drifts against the unpreconditioned oracle drift to the solver accuracy, Langevin steps,
+mixed precision, batching, every fused tile shape and the lattice-size sweep shapes."""
+import numpy as np
+import pytest
+
+import oracle as O
+from test_gpu_parity import RTOL, mk, rel
+
+pytestmark = pytest.mark.gpu
+
+EO_TILES64 = [(8, 32), (16, 16), (4, 64), (4, 32), (2, 32)]
+EO_TILES32 = [(8, 32), (16, 32), (8, 64), (16, 16), (4, 64), (2, 32)]
+
+
+@pytest.mark.parametrize("Nt,Nx", [(16, 16), (32, 32), (6, 32), (8, 64)])
+def test_eo_force_fixed_iterations(gpu, Nt, Nx):
+    Y = gpu
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, cgTol=0.0, cgTolMode=0, cgMaxIter=30)
+    p.evenOdd = 1
+    orc.evenOdd = 1
+    phi = O.hot_start(11, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        d, st = c.fermion_force(5)
+    dr, sr = O.fermion_drift(orc, phi, 5)
+    assert st["iters"] == sr["iters"] == 30
+    assert rel(d, dr) < RTOL
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+@pytest.mark.parametrize("Nt,Nx", [(32, 32), (64, 64)])
+def test_eo_force_converged(gpu, precision, Nt, Nx):
+    """Converged to |eta - D^dag chi| / |eta| <= 1e-10: equals the oracle's
+    unpreconditioned drift to the solver accuracy."""
+    Y = gpu
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.05, g=0.5, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    p.evenOdd = 1
+    if precision == "mixed":
+        p.cgPrecision = Y.YCN_CG_MIXED
+    phi = O.hot_start(3, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        d, st = c.fermion_force(2)
+    dr, sr = O.fermion_drift(orc, phi, 2)
+    assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+    assert st["iters"] < sr["iters"]
+    assert rel(d, dr) < 1e-8
+
+
+@pytest.mark.parametrize("adaptive", [0, 1])
+def test_eo_langevin_steps_fixed_iterations(gpu, adaptive):
+    Y = gpu
+    Nt, Nx = 16, 32
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, adaptive=adaptive, cgTol=0.0, cgTolMode=0, cgMaxIter=25)
+    p.evenOdd = 1
+    orc.evenOdd = 1
+    phi = O.hot_start(4, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        for step in range(3):
+            r = c.langevin_step(step)
+            phi, eps, _ = O.langevin_step(orc, phi, step)
+            assert rel(c.get_field(), phi) < RTOL, step
+            assert abs(r.eps[0] - eps) <= 1e-12 * eps
+
+
+@pytest.mark.parametrize("tile", range(len(EO_TILES64)))
+def test_eo_every_fp64_tile(gpu, tile, monkeypatch):
+    monkeypatch.setenv("YCN_TILE_EO", str(tile))
+    Y = gpu
+    Nt, Nx = 16, 64
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, cgTol=0.0, cgTolMode=0, cgMaxIter=20)
+    p.evenOdd = 1
+    orc.evenOdd = 1
+    phi = O.hot_start(8, Nt, Nx)
+    with Y.Context(p) as c:
+        assert "forced by YCN_TILE_EO" in c.tuning_report()
+        c.set_field(phi)
+        d, _ = c.fermion_force(1)
+    dr, _ = O.fermion_drift(orc, phi, 1)
+    assert rel(d, dr) < RTOL, EO_TILES64[tile]
+
+
+@pytest.mark.parametrize("tile", range(len(EO_TILES32)))
+def test_eo_every_fp32_tile(gpu, tile, monkeypatch):
+    monkeypatch.setenv("YCN_TILE_EO32", str(tile))
+    Y = gpu
+    Nt, Nx = 32, 64
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.1, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    p.evenOdd = 1
+    p.cgPrecision = Y.YCN_CG_MIXED
+    phi = O.hot_start(9, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        d, st = c.fermion_force(1)
+    dr, _ = O.fermion_drift(orc, phi, 1)
+    assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10
+    assert rel(d, dr) < 1e-8, EO_TILES32[tile]
+
+
+def test_eo_batched_replicas_and_noise(gpu):
+    Y = gpu
+    Nt, Nx, R, J = 16, 32, 3, 2
+    p, _ = mk(Y, Nt=Nt, Nx=Nx, nReplicas=R, nNoise=J, cgTol=1e-11, cgTolMode=1, cgMaxIter=20000)
+    p.evenOdd = 1
+    m2 = np.linspace(-0.5, 0.2, R)
+    seeds = np.arange(20, 20 + R, dtype=np.uint64)
+    phi = np.stack([O.hot_start(30 + k, Nt, Nx) for k in range(R)])
+    with Y.Context(p) as c:
+        c.set_replicas(m2, seeds)
+        c.set_field(phi)
+        d, st = c.fermion_force(6)
+    assert st["converged"] == 1
+    for k in range(R):
+        o = O.params(Nt=Nt, Nx=Nx, m2=m2[k], lam=0.5, g=0.3, M=0.3, cutFraction=0.5, seed=int(seeds[k]),
+                     cgTol=1e-11, cgTolMode=1, cgMaxIter=20000, nNoise=J)
+        dr, _ = O.fermion_drift(o, phi[k], 6)
+        assert rel(d[k], dr) < 1e-8, k
+
+
+@pytest.mark.parametrize("Nt", [2, 10, 22])
+def test_eo_small_time_extents(gpu, Nt):
+    """Time extents of the lattice-size sweep of runs.py:28-36 (Nt = 2 .. 22 at Nx = 32,
+    rescaled drift; Wilson r = 1, where the even-odd diagonal stays away from zero):
+    fixed-iteration even-odd steps against the oracle."""
+    Y = gpu
+    p, orc = mk(Y, Nt=Nt, Nx=32, m2=-0.125, lam=0.3955, g=0.07, M=0.01, r=1.0, driftMode=1, epsBar=0.005,
+                adaptive=1, cgTol=0.0, cgTolMode=0, cgMaxIter=20)
+    p.evenOdd = 1
+    orc.evenOdd = 1
+    phi = O.hot_start(12, Nt, 32)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        for step in range(2):
+            c.langevin_step(step)
+            phi, _, _ = O.langevin_step(orc, phi, step)
+            assert rel(c.get_field(), phi) < RTOL, step

@@ -165,6 +165,25 @@ def test_langevin_step(gpu, adaptive, Nt, Nx):
             assert abs(r.eps[0] - eps) <= 1e-13 * eps, (step, r.eps[0], eps)
 
 
+@pytest.mark.parametrize("Nt", [2, 10, 22])
+def test_sweep_configuration(gpu, Nt):
+    """The lattice-size sweep of runs.py:28-36: Nt = 2 .. 22 at Nx = 32, Wilson parameter
+    0, fermion mass 0.01, Yukawa 0.07, rescaled drift, s = 0.5, m^2 = -0.5 s^2, lambda =
+    1.582 s^2 (useMass = true), adaptive step: three fixed-iteration steps vs the oracle."""
+    Y = gpu
+    p, orc = mk(Y, Nt=Nt, Nx=32, m2=-0.125, lam=0.3955, g=0.07, M=0.01, r=0.0, cutFraction=0.5, driftMode=1,
+                epsBar=0.005, adaptive=1, cgTol=0.0, cgTolMode=0, cgMaxIter=30)
+    phi0 = O.hot_start(5, Nt, 32)
+    with Y.Context(p) as c:
+        c.set_field(phi0)
+        phi = phi0.copy()
+        for step in range(3):
+            r = c.langevin_step(step)
+            phi, eps, _ = O.langevin_step(orc, phi, step)
+            assert rel(c.get_field(), phi) < RTOL, step
+            assert abs(r.eps[0] - eps) <= 1e-12 * eps, step
+
+
 @pytest.mark.parametrize("adaptive", [0, 1])
 def test_langevin_step_converged(gpu, adaptive):
     """Solves to true residual 1e-10: the drift is fixed to the solver accuracy, so the

@@ -53,7 +53,7 @@ class Params(C.Structure):
         ("epsBar", C.c_double), ("adaptive", C.c_int), ("seed", C.c_uint64),
         ("cgTol", C.c_double), ("cgTolMode", C.c_int), ("cgMaxIter", C.c_int),
         ("cgPrecision", C.c_int), ("reliableEvery", C.c_int), ("precond", C.c_int),
-        ("device", C.c_int),
+        ("device", C.c_int), ("evenOdd", C.c_int),
     ]
 
 

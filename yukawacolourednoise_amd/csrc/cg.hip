@@ -28,13 +28,13 @@
 #include <vector>
 
 #include "cg_fused.hpp"
+#include "cg_host.hpp"
 #include "ycn_device.hpp"
 #include "ycn_internal.hpp"
 
 namespace ycn {
 
 constexpr int kNT = 256;        // threads per block for the stencil and BLAS-1 kernels
-constexpr int kGraphIters = 8;  // CG iterations per captured graph (even: keeps the p parity)
 
 __device__ __forceinline__ WilsonCoef make_coef(double r, double M, double g) {
     WilsonCoef w;
@@ -94,9 +94,22 @@ __global__ void k_set_int(int *p, int v) {
 // ---------------------------------------------------------------------------
 // finalize kernels: one block per system, partials summed in block order
 // ---------------------------------------------------------------------------
+// Sum of nb block partials by one block, in a fixed order (thread t adds entries t,
+// t + kNT, ... in sequence, then the block tree).  Loads are issued in batches of kFinU
+// before they are added, so the partials' memory latency is paid nb / (kNT kFinU) times
+// rather than nb / kNT times; the order of the additions is unchanged.
+constexpr int kFinU = 8;
 __device__ __forceinline__ double2 fin_sum(const double2 *part, int nb, double2 *lds) {
     double2 acc = make_double2(0.0, 0.0);
-    for (int i = threadIdx.x; i < nb; i += kNT) acc = cadd(acc, part[i]);
+    int i = threadIdx.x;
+    for (; i + (kFinU - 1) * kNT < nb; i += kFinU * kNT) {
+        double2 v[kFinU];
+#pragma unroll
+        for (int u = 0; u < kFinU; ++u) v[u] = part[i + u * kNT];
+#pragma unroll
+        for (int u = 0; u < kFinU; ++u) acc = cadd(acc, v[u]);
+    }
+    for (; i < nb; i += kNT) acc = cadd(acc, part[i]);
     return block_sum2<kNT>(acc, lds);
 }
 
@@ -137,9 +150,9 @@ k_fin_setup(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, 
 __global__ void __launch_bounds__(kNT) k_fin_alpha(const double2 *__restrict__ part, int nb, CgState *__restrict__ st) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.x;
-    if (st[sys].done) return;
+    const int skip = st[sys].done;   // loaded together with the partials
     const double2 dot = fin_sum(part + (size_t)sys * nb, nb, lds);
-    if (threadIdx.x) return;
+    if (skip || threadIdx.x) return;
     const double rr = st[sys].rr;
     const double den = dot.x * dot.x + dot.y * dot.y;
     st[sys].alpha = make_double2(rr * dot.x / den, -rr * dot.y / den);
@@ -152,9 +165,9 @@ __global__ void __launch_bounds__(kNT)
 k_fin_beta(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.x;
-    if (st[sys].done) return;
+    const int skip = st[sys].done;   // loaded together with the partials
     const double2 tot = fin_sum(part + (size_t)sys * nb, nb, lds);
-    if (threadIdx.x) return;
+    if (skip || threadIdx.x) return;
     CgState &S = st[sys];
     const double rrn = tot.x;
     S.rr_prev = S.rr;
@@ -202,9 +215,9 @@ __global__ void __launch_bounds__(kNT)
 k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.x;
-    if (st[sys].final_) return;
+    const int skip = st[sys].final_;   // loaded together with the partials
     const double2 tot = fin_sum(part + (size_t)sys * nb, nb, lds);
-    if (threadIdx.x) return;
+    if (skip || threadIdx.x) return;
     CgState &S = st[sys];
     const double rrt = tot.x;
     const double bn = sqrt(S.bnorm2);
@@ -237,9 +250,9 @@ k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
 __global__ void __launch_bounds__(kNT) k_fin_alpha_fcg(const double2 *__restrict__ part, int nb, CgState *__restrict__ st) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.x;
-    if (st[sys].done) return;
+    const int skip = st[sys].done;   // loaded together with the partials
     const double2 dot = fin_sum(part + (size_t)sys * nb, nb, lds);
-    if (threadIdx.x) return;
+    if (skip || threadIdx.x) return;
     CgState &S = st[sys];
     if (!(dot.x > 0.0) || !isfinite(S.rz) || !(S.rz > 0.0)) {
         // the preconditioned direction is not a descent direction: no step, and plain
@@ -260,9 +273,9 @@ template <int MODE>
 __global__ void __launch_bounds__(kNT) k_fin_fcg(const double2 *__restrict__ part, int nb, CgState *__restrict__ st) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.x;
-    if (st[sys].done) return;
+    const int skip = st[sys].done;   // loaded together with the partials
     const double2 tot = fin_sum(part + (size_t)sys * nb, nb, lds);
-    if (threadIdx.x) return;
+    if (skip || threadIdx.x) return;
     CgState &S = st[sys];
     if (!isfinite(tot.x) || !isfinite(tot.y)) {
         S.precOff = 1;
@@ -279,19 +292,6 @@ __global__ void __launch_bounds__(kNT) k_fin_fcg(const double2 *__restrict__ par
 // ---------------------------------------------------------------------------
 // FP64 CG kernels
 // ---------------------------------------------------------------------------
-__device__ __forceinline__ void eta_site(const uint64_t *seeds, uint32_t streamBase, uint64_t step, int sys,
-                                         int nNoise, int s, double2 b[4]) {
-    // fillNormalRND (FermionicDrift.cu:141-149): component c of site s is normal
-    // number 4s+c of the stream; imaginary part 0
-    const int rep = sys / nNoise, j = sys - rep * nNoise;
-    const double2 g0 = normal_pair(seeds[rep], streamBase + (uint32_t)j, step, (uint64_t)2 * s);
-    const double2 g1 = normal_pair(seeds[rep], streamBase + (uint32_t)j, step, (uint64_t)2 * s + 1);
-    b[0] = make_double2(g0.x, 0.0);
-    b[1] = make_double2(g0.y, 0.0);
-    b[2] = make_double2(g1.x, 0.0);
-    b[3] = make_double2(g1.y, 0.0);
-}
-
 // setup: eta (generated or given) -> b; x = 0; r = b; p0 = p1 = b; partial |b|^2
 template <bool GEN>
 __global__ void __launch_bounds__(kNT)
@@ -633,7 +633,9 @@ static int site_blocks(const Ctx &c) { return (c.V + kNT - 1) / kNT; }
 
 int cg_alloc(Ctx &c) {
     const size_t n = (size_t)c.nSys * 4 * c.V;
-    c.maxBlocks = (c.V + 63) / 64;   // partial slots for the smallest (64-site) blocks
+    // partial slots: the smallest (64-site) blocks, or one full-lattice plus two
+    // half-lattice site grids of 256-thread blocks (even-odd setup), whichever is larger
+    c.maxBlocks = (c.V + 63) / 64 + 4;
     double2 **bufs[] = {&c.eta, &c.x, &c.r, &c.pv[0], &c.pv[1], &c.t, &c.Ap};
     for (double2 **b : bufs) {
         YCN_HIP(hipMalloc(b, sizeof(double2) * n));
@@ -725,43 +727,51 @@ static void launch_iteration(Ctx &c, const LatticeArgs &L, const double *phi, in
                        site_blocks(c), c.cg_d, c.active);
 }
 
-// Launch n CG iterations (n even keeps the p ping-pong parity).  When graphs are on and
-// no kernel is being profiled, the n iterations are captured once into a hipGraph per
-// (field, systems, operator, n, precision, tile) and replayed; every kernel reads its
-// scalars from device state and exits early for converged systems, so replays need no
-// host-side parameters.  No memset nodes are captured: all state changes are kernels.
-template <typename Launch>
-static int run_chunk(Ctx &c, const void *phi, int nsys, int mtype, int n, int precision, int tile, Launch launch) {
-    if (!c.useGraphs || c.profTarget != PK_NONE) {
-        for (int i = 0; i < n; ++i) launch(i & 1);
-        YCN_HIP(hipGetLastError());
-        return YCN_OK;
-    }
-    for (const Ctx::GraphEntry &e : c.graphs)
-        if (e.phi == phi && e.nsys == nsys && e.mtype == mtype && e.n == n && e.precision == precision &&
-            e.tile == tile) {
-            YCN_HIP(hipGraphLaunch(e.exec, c.stream));
-            return YCN_OK;
-        }
-    hipGraph_t g;
-    YCN_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < n; ++i) launch(i & 1);
-    const hipError_t le = hipGetLastError();
-    YCN_HIP(hipStreamEndCapture(c.stream, &g));
-    if (le != hipSuccess) {
-        (void)hipGraphDestroy(g);
-        c.err = std::string("launch during capture: ") + hipGetErrorString(le);
-        return YCN_ERR_HIP;
-    }
-    hipGraphExec_t exec;
-    YCN_HIP(hipGraphInstantiate(&exec, g, nullptr, nullptr, 0));
-    (void)hipGraphDestroy(g);
-    c.graphs.push_back({phi, nsys, mtype, n, precision, tile, exec});
-    YCN_HIP(hipGraphLaunch(exec, c.stream));
+void launch_fin_alpha(Ctx &c, int nsys, int nb) {
+    hipLaunchKernelGGL(k_fin_alpha, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb, c.cg_d);
+}
+
+void launch_fin_beta(Ctx &c, int nsys, int nb, bool mixed) {
+    if (mixed)
+        hipLaunchKernelGGL(k_fin_beta<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
+                           c.cg_d, c.active);
+    else
+        hipLaunchKernelGGL(k_fin_beta<false>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
+                           c.cg_d, c.active);
+}
+
+int launch_cg_update(Ctx &c, int nsys, int n, const double2 *p) {
+    const dim3 g((n + kNT - 1) / kNT, nsys);
+    prof_before(c, PK_CG_UPDATE, (double)nsys * n * 6 * 64);
+    hipLaunchKernelGGL(k_cg_c, g, dim3(kNT), 0, c.stream, n, (const CgState *)c.cg_d, c.x, c.r, p,
+                       (const double2 *)c.Ap, c.partials);
+    prof_after(c, PK_CG_UPDATE);
+    return (int)g.x;
+}
+
+int launch_mp_update(Ctx &c, int nsys, int n, const float2 *p) {
+    const dim3 g((n + kNT - 1) / kNT, nsys);
+    prof_before(c, PK_CG_UPDATE, (double)nsys * n * 6 * 32);
+    hipLaunchKernelGGL(k_mp_update, g, dim3(kNT), 0, c.stream, n, (const CgState *)c.cg_d, c.xs, c.rs, p,
+                       (const float2 *)c.Aps, c.partials);
+    prof_after(c, PK_CG_UPDATE);
+    return (int)g.x;
+}
+
+int launch_mp_accum(Ctx &c, int nsys, int n) {
+    hipLaunchKernelGGL(k_mp_accum, dim3((n + kNT - 1) / kNT, nsys), dim3(kNT), 0, c.stream, n,
+                       (const CgState *)c.cg_d, c.x, c.xs);
+    YCN_HIP(hipGetLastError());
     return YCN_OK;
 }
 
-static int poll_active(Ctx &c, int &active) {
+int to_f32_n(Ctx &c, const double *a, float *b, int64_t n) {
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, a, b, n);
+    YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+
+int poll_active(Ctx &c, int &active) {
     YCN_HIP(hipMemcpyAsync(c.active_h, c.active, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     YCN_HIP(hipStreamSynchronize(c.stream));
     active = *c.active_h;
@@ -840,7 +850,7 @@ int cg_autotune(Ctx &c) {
     return YCN_OK;
 }
 
-static int read_stats(Ctx &c, int nsys, SolveStats *st) {
+int read_stats(Ctx &c, int nsys, SolveStats *st) {
     std::vector<CgState> S(nsys);
     YCN_HIP(hipMemcpyAsync(S.data(), c.cg_d, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream));
     YCN_HIP(hipStreamSynchronize(c.stream));
@@ -865,7 +875,9 @@ int cg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
              SolveStats *st) {
     if (c.p.precond == YCN_PRECOND_CNN && mtype == YCN_DAGGER)
         return fcg_solve(c, phi, nsys, mtype, gen, streamBase, step, st);
-    if (c.p.cgPrecision == YCN_CG_MIXED || c.p.precond == YCN_PRECOND_CNN)
+    // mixed precision where the FP32 tiles fit the lattice (with even-odd preconditioning
+    // the fermion force may run mixed on shapes the plain FP32 tiles do not cover)
+    if ((c.p.cgPrecision == YCN_CG_MIXED && c.tile32 >= 0) || c.p.precond == YCN_PRECOND_CNN)
         return mp_solve(c, phi, nsys, mtype, gen, streamBase, step, st);
     const LatticeArgs L = largs(c);
     const dim3 g = site_grid(c, nsys);

@@ -46,6 +46,23 @@ __device__ __forceinline__ void wilson_flavour_r(R hp, R hm, R hr, R h, R d, V2 
     o1.y = d * in1.y - (b_hm * Tb1.y + b_hp * Tf1.y + hr * (Xf1.y + Xb1.y) + h * (Xb0.y - Xf0.y));
 }
 
+// The hopping part alone (wilson_flavour_r with d = 0): o = H psi (DAG = false) or
+// K psi = H^T psi (DAG = true), the even-odd blocks of D and D^dag.
+template <bool DAG, typename R, typename V2>
+__device__ __forceinline__ void hop_flavour_r(R hp, R hm, R hr, R h, V2 tf0, V2 tf1, R sf, V2 tb0, V2 tb1, R sb,
+                                              V2 xf0, V2 xf1, V2 xb0, V2 xb1, V2 &o0, V2 &o1) {
+    const V2 Tb0 = DAG ? tf0 : tb0, Tb1 = DAG ? tf1 : tb1;
+    const V2 Tf0 = DAG ? tb0 : tf0, Tf1 = DAG ? tb1 : tf1;
+    const R sTb = DAG ? sf : sb, sTf = DAG ? sb : sf;
+    const V2 Xf0 = DAG ? xb0 : xf0, Xf1 = DAG ? xb1 : xf1;
+    const V2 Xb0 = DAG ? xf0 : xb0, Xb1 = DAG ? xf1 : xb1;
+    const R a_hp = hp * sTb, a_hm = hm * sTf, b_hm = hm * sTb, b_hp = hp * sTf;
+    o0.x = -(a_hp * Tb0.x + a_hm * Tf0.x + hr * (Xf0.x + Xb0.x) + h * (Xb1.x - Xf1.x));
+    o0.y = -(a_hp * Tb0.y + a_hm * Tf0.y + hr * (Xf0.y + Xb0.y) + h * (Xb1.y - Xf1.y));
+    o1.x = -(b_hm * Tb1.x + b_hp * Tf1.x + hr * (Xf1.x + Xb1.x) + h * (Xb0.x - Xf0.x));
+    o1.y = -(b_hm * Tb1.y + b_hp * Tf1.y + hr * (Xf1.y + Xb1.y) + h * (Xb0.y - Xf0.y));
+}
+
 struct FusedArgs {
     int Nt, Nx, V, nNoise;
     double r, M, g;

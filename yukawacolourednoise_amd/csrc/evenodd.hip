@@ -1,0 +1,772 @@
+// evenodd.hip -- even-odd preconditioned fermion force on gfx950.
+//
+// The drift only needs chi = (D^dag)^-1 eta (chi = D y with D^dag D y = eta,
+// FermionicDrift.cu:96-139).  With D^dag = A + K, A = diag(2r + M + g phi) and K the
+// hopping, which connects even and odd sites, the Schur complement on the even sites
+//   S = A_e - K_eo A_o^-1 K_oe,   b_e = eta_e - K_eo A_o^-1 eta_o,
+//   S chi_e = b_e,   chi_o = A_o^-1 (eta_o - K_oe chi_e)
+// is solved by CG on S^T S chi_e = S^T b_e; S^T = A_e - H_eo A_o^-1 H_oe is the Schur
+// complement of D (H = K^T, the hopping of D).  |eta - D^dag chi| = |b_e - S chi_e|
+// exactly (the odd rows vanish by construction), so the solve stops on the residual of
+// the unpreconditioned system, the quantity the plain solver checks.
+//
+// Half-lattice vectors use the compact checkerboard order of the reference's even-odd
+// blocks (commented out in Dirac.cu:256-379): site (t, x) of a parity at t * Nx/2 +
+// (x >> 1), SoA planes [sys][c][Vh].  Needs even Nt and Nx.
+//
+// One CG iteration = k_eo_normal_fused (p' = r + beta p, v = S p', Ap = S^T v, |v|^2:
+// four half-lattice hops through LDS) + the BLAS-1 update on Vh sites: 5S + P bytes per
+// site of the full lattice, against 10S + P for the unpreconditioned iteration.
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstdlib>
+#include <type_traits>
+
+#include "cg_fused.hpp"
+#include "cg_host.hpp"
+#include "ycn_device.hpp"
+#include "ycn_internal.hpp"
+
+namespace ycn {
+
+namespace {
+constexpr int kNT = 256;
+}
+
+struct EoArgs {
+    int Nt, Nx, Nh, V, Vh, nNoise;   // Nh = Nx / 2, Vh = V / 2
+    double r, M, g;
+    int tilesX, tilesPerSys, xcdRemap;
+};
+
+// site (t, x) of compact index j on parity par
+__device__ __forceinline__ void eo_site(const EoArgs &A, int par, int j, int &t, int &x) {
+    t = j / A.Nh;
+    x = 2 * (j - t * A.Nh) + ((t + par) & 1);
+}
+__device__ __forceinline__ int eo_j(const EoArgs &A, int t, int x) { return t * A.Nh + (x >> 1); }
+
+// a mod n in [0, n) for any a: the 4-site halo of a tile spans several periods when an
+// extent is smaller than 4 (Nt = 2)
+__device__ __forceinline__ int wrapm(int a, int n) {
+    const int m = a % n;
+    return m < 0 ? m + n : m;
+}
+
+// Hopping (K for DAG, H otherwise) at site (t, x); ld(t', x', c) returns component c of
+// the input at neighbour (t', x') of the other parity.
+template <bool DAG, typename R, typename V2, typename Ld>
+__device__ __forceinline__ void eo_hop(const EoArgs &A, int t, int x, Ld ld, V2 o[4]) {
+    const int tp = (t == A.Nt - 1) ? 0 : t + 1, tm = (t == 0) ? A.Nt - 1 : t - 1;
+    const int xp = (x == A.Nx - 1) ? 0 : x + 1, xm = (x == 0) ? A.Nx - 1 : x - 1;
+    const R sf = (t == A.Nt - 1) ? (R)-1 : (R)1, sb = (t == 0) ? (R)-1 : (R)1;
+    const R hp = (R)(0.5 * (A.r + 1.0)), hm = (R)(0.5 * (A.r - 1.0)), hr = (R)(0.5 * A.r), h = (R)0.5;
+#pragma unroll
+    for (int f = 0; f < 4; f += 2)
+        hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, ld(tp, x, f), ld(tp, x, f + 1), sf, ld(tm, x, f), ld(tm, x, f + 1),
+                                  sb, ld(t, xp, f), ld(t, xp, f + 1), ld(t, xm, f), ld(t, xm, f + 1), o[f], o[f + 1]);
+}
+
+// ---------------------------------------------------------------------------
+// setup, Schur blocks for setup and reliable updates, drift (FP64, untiled)
+// ---------------------------------------------------------------------------
+
+// eta generated (GEN) or given; partial |eta|^2 over the full lattice
+template <bool GEN>
+__global__ void __launch_bounds__(kNT)
+k_eo_eta(EoArgs A, const uint64_t *__restrict__ seeds, uint32_t streamBase, uint64_t step, double2 *__restrict__ eta,
+         double2 *__restrict__ part) {
+    __shared__ double2 lds[kNT / kWave];
+    const int sys = blockIdx.y, s = blockIdx.x * kNT + threadIdx.x;
+    const size_t V = A.V, off = (size_t)sys * 4 * V;
+    double2 acc = make_double2(0.0, 0.0);
+    if (s < A.V) {
+        double2 b[4];
+        if (GEN) {
+            eta_site(seeds, streamBase, step, sys, A.nNoise, s, b);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) eta[off + c * V + s] = b[c];
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) b[c] = eta[off + c * V + s];
+        }
+#pragma unroll
+        for (int c = 0; c < 4; ++c) acc.x += b[c].x * b[c].x + b[c].y * b[c].y;
+    }
+    acc = block_sum2<kNT>(acc, lds);
+    if (threadIdx.x == 0) part[(size_t)sys * gridDim.x + blockIdx.x] = acc;
+}
+
+// b_e = eta_e - K_eo A_o^-1 eta_o
+__global__ void __launch_bounds__(kNT)
+k_eo_rhs(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__ eta, double2 *__restrict__ b) {
+    const int sys = blockIdx.y, j = blockIdx.x * kNT + threadIdx.x;
+    if (j >= A.Vh) return;
+    int t, x;
+    eo_site(A, 0, j, t, x);
+    const size_t V = A.V, Vh = A.Vh;
+    const double2 *e = eta + (size_t)sys * 4 * V;
+    const double *ph = phi + (size_t)(sys / A.nNoise) * V;
+    const double d0 = 2.0 * A.r + A.M;
+    double2 k[4];
+    eo_hop<true, double, double2>(A, t, x, [&](int tn, int xn, int c) {
+        const size_t fs = (size_t)tn * A.Nx + xn;
+        const double inv = 1.0 / (d0 + A.g * ph[fs]);
+        const double2 v = e[c * V + fs];
+        return make_double2(v.x * inv, v.y * inv);
+    }, k);
+    const size_t fs = (size_t)t * A.Nx + x;
+    double2 *bo = b + (size_t)sys * 4 * Vh;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) bo[c * Vh + j] = csub(e[c * V + fs], k[c]);
+}
+
+// U_o = A_o^-1 K_oe in_e (DAG) or A_o^-1 H_oe in_e
+template <bool DAG>
+__global__ void __launch_bounds__(kNT)
+k_eo_hop_oe(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__ in, double2 *__restrict__ U) {
+    const int sys = blockIdx.y, j = blockIdx.x * kNT + threadIdx.x;
+    if (j >= A.Vh) return;
+    int t, x;
+    eo_site(A, 1, j, t, x);
+    const size_t Vh = A.Vh, off = (size_t)sys * 4 * Vh;
+    const double *ph = phi + (size_t)(sys / A.nNoise) * A.V;
+    double2 k[4];
+    eo_hop<DAG, double, double2>(A, t, x, [&](int tn, int xn, int c) { return in[off + c * Vh + eo_j(A, tn, xn)]; },
+                                 k);
+    const double inv = 1.0 / (2.0 * A.r + A.M + A.g * ph[(size_t)t * A.Nx + x]);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) U[off + c * Vh + j] = make_double2(k[c].x * inv, k[c].y * inv);
+}
+
+// v_e = A_e in_e - K_eo U_o (DAG: S in) or A_e in_e - H_eo U_o (S^T in).
+// MODE 0: out = v; MODE 1: out = b - v and partial |out|^2; MODE 2: out = v and partial
+// |v|^2.  out32 (optional) receives an FP32 copy of out.
+template <bool DAG, int MODE>
+__global__ void __launch_bounds__(kNT)
+k_eo_hop_eo(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__ in, const double2 *__restrict__ U,
+            const double2 *__restrict__ b, double2 *__restrict__ out, float2 *__restrict__ out32,
+            double2 *__restrict__ part) {
+    __shared__ double2 lds[kNT / kWave];
+    const int sys = blockIdx.y, j = blockIdx.x * kNT + threadIdx.x;
+    const size_t Vh = A.Vh, off = (size_t)sys * 4 * Vh;
+    double2 acc = make_double2(0.0, 0.0);
+    if (j < A.Vh) {
+        int t, x;
+        eo_site(A, 0, j, t, x);
+        const double *ph = phi + (size_t)(sys / A.nNoise) * A.V;
+        double2 k[4];
+        eo_hop<DAG, double, double2>(A, t, x, [&](int tn, int xn, int c) { return U[off + c * Vh + eo_j(A, tn, xn)]; },
+                                     k);
+        const double d = 2.0 * A.r + A.M + A.g * ph[(size_t)t * A.Nx + x];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double2 iv = in[off + c * Vh + j];
+            double2 v = make_double2(d * iv.x - k[c].x, d * iv.y - k[c].y);
+            if (MODE == 1) v = csub(b[off + c * Vh + j], v);
+            out[off + c * Vh + j] = v;
+            if (out32) out32[off + c * Vh + j] = make_float2((float)v.x, (float)v.y);
+            acc.x += v.x * v.x + v.y * v.y;
+        }
+    }
+    if (MODE != 0) {
+        acc = block_sum2<kNT>(acc, lds);
+        if (threadIdx.x == 0) part[(size_t)sys * gridDim.x + blockIdx.x] = acc;
+    }
+}
+
+__global__ void __launch_bounds__(kNT) k_eo_zero(int64_t n, double2 *__restrict__ a, double2 *__restrict__ b,
+                                                 double2 *__restrict__ c) {
+    const int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x;
+    if (i >= n) return;
+    const double2 z = make_double2(0.0, 0.0);
+    a[i] = z;
+    b[i] = z;
+    c[i] = z;
+}
+
+__global__ void __launch_bounds__(kNT) k_eo_zero32(int64_t n, float2 *__restrict__ a, float2 *__restrict__ b,
+                                                   float2 *__restrict__ c) {
+    const int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x;
+    if (i >= n) return;
+    const float2 z = make_float2(0.f, 0.f);
+    a[i] = z;
+    b[i] = z;
+    c[i] = z;
+}
+
+// drift_x = fac Re sum_k conj(chi_xk) eta_xk (FermionicDrift.cu:122-139) with chi_e from
+// the solve and chi_o = A_o^-1 (eta_o - K_oe chi_e) formed on the fly; mean over noise
+// vectors.
+__global__ void __launch_bounds__(kNT)
+k_eo_drift(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__ chi_e,
+           const double2 *__restrict__ eta, double *__restrict__ drift, double fac) {
+    const int s = blockIdx.x * kNT + threadIdx.x;
+    if (s >= A.V) return;
+    const int rep = blockIdx.y;
+    const size_t V = A.V, Vh = A.Vh;
+    const int t = s / A.Nx, x = s - t * A.Nx;
+    const double *ph = phi + (size_t)rep * V;
+    const bool odd = ((t + x) & 1) != 0;
+    const double d = 2.0 * A.r + A.M + A.g * ph[s];
+    double acc = 0.0;
+    for (int jn = 0; jn < A.nNoise; ++jn) {
+        const size_t sys = (size_t)rep * A.nNoise + jn;
+        const double2 *ce = chi_e + sys * 4 * Vh;
+        const double2 *e = eta + sys * 4 * V;
+        double2 ch[4];
+        if (odd) {
+            double2 k[4];
+            eo_hop<true, double, double2>(A, t, x, [&](int tn, int xn, int c) { return ce[c * Vh + eo_j(A, tn, xn)]; },
+                                          k);
+            const double inv = 1.0 / d;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const double2 ev = e[c * V + s];
+                ch[c] = make_double2((ev.x - k[c].x) * inv, (ev.y - k[c].y) * inv);
+            }
+        } else {
+#pragma unroll
+            for (int c = 0; c < 4; ++c) ch[c] = ce[c * Vh + eo_j(A, t, x)];
+        }
+        double2 sm = cdotc(ch[0], e[s]);
+        sm = cadd(sm, cdotc(ch[1], e[V + s]));
+        sm = cadd(sm, cdotc(ch[2], e[2 * V + s]));
+        sm = cadd(sm, cdotc(ch[3], e[3 * V + s]));
+        acc += sm.x;
+    }
+    drift[(size_t)rep * V + s] = (A.nNoise == 1) ? fac * acc : fac * acc / (double)A.nNoise;
+}
+
+// ---------------------------------------------------------------------------
+// finalize kernels (one block per system, fixed-order sums)
+// ---------------------------------------------------------------------------
+__device__ __forceinline__ double2 eo_fin_sum(const double2 *part, int nb, double2 *lds) {
+    double2 acc = make_double2(0.0, 0.0);
+    int i = threadIdx.x;
+    for (; i + 7 * kNT < nb; i += 8 * kNT) {
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) v[u] = part[i + u * kNT];
+#pragma unroll
+        for (int u = 0; u < 8; ++u) acc = cadd(acc, v[u]);
+    }
+    for (; i < nb; i += kNT) acc = cadd(acc, part[i]);
+    return block_sum2<kNT>(acc, lds);
+}
+
+// |eta|^2 and |c|^2 = |S^T b|^2: rr, thresholds and run flags of a new solve
+__global__ void __launch_bounds__(kNT)
+k_fin_eo_setup(const double2 *__restrict__ partE, int nbE, const double2 *__restrict__ partC, int nbC,
+               CgState *__restrict__ st, int *__restrict__ active, double tol, int mode, int maxIter, int maxRestarts) {
+    __shared__ double2 lds0[kNT / kWave], lds1[kNT / kWave];
+    const int sys = blockIdx.x;
+    const double2 e = eo_fin_sum(partE + (size_t)sys * nbE, nbE, lds0);
+    const double2 cc = eo_fin_sum(partC + (size_t)sys * nbC, nbC, lds1);
+    if (threadIdx.x) return;
+    CgState &S = st[sys];
+    S.enorm2 = e.x;
+    S.rr = cc.x;
+    S.rr_prev = cc.x;
+    S.bnorm2 = cc.x;
+    S.tol = tol;
+    S.mode = mode;
+    S.maxIter = maxIter;
+    S.maxRestarts = maxRestarts;
+    S.stop = (mode == YCN_TOL_ABS_RECURSIVE) ? tol : tol * sqrt(cc.x);
+    S.beta = 0.0;
+    S.alpha = make_double2(0.0, 0.0);
+    S.iter = 0;
+    S.restarts = 0;
+    S.true_rel = 0.0;
+    S.rz = 0.0;
+    S.true_prev = 1.0;
+    S.precOff = 0;
+    S.restartNext = 0;
+    const bool run = (0 < maxIter) && (sqrt(cc.x) > S.stop);
+    S.done = run ? 0 : 1;
+    S.final_ = run ? 0 : 1;
+    S.converged = run ? 0 : 1;
+    if (!run) atomicSub(active, 1);
+}
+
+// FP64 solve: true residual |b_e - S x_e| / |eta|.  While it exceeds tol (relative mode)
+// the iteration continues from its current state with the normal-equation threshold
+// lowered by the missing factor (times 1/2).
+__global__ void __launch_bounds__(kNT)
+k_fin_eo_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active) {
+    __shared__ double2 lds[kNT / kWave];
+    const int sys = blockIdx.x;
+    const double2 tot = eo_fin_sum(part + (size_t)sys * nb, nb, lds);
+    if (threadIdx.x) return;
+    CgState &S = st[sys];
+    S.true_rel = S.enorm2 > 0.0 ? sqrt(tot.x / S.enorm2) : sqrt(tot.x);
+    if (S.mode != YCN_TOL_REL_TRUE) return;
+    if (S.true_rel > S.tol && S.iter < S.maxIter && S.restarts < S.maxRestarts) {
+        S.restarts += 1;
+        S.stop = sqrt(S.rr) * 0.5 * S.tol / S.true_rel;
+        S.converged = 0;
+        if (S.done) {
+            S.done = 0;
+            atomicAdd(active, 1);
+        }
+    } else {
+        S.converged = S.true_rel <= S.tol ? 1 : 0;
+    }
+}
+
+// mixed precision reliable update: partRes = |b_e - S x_e|^2, partN = |S^T (b_e - S x_e)|^2
+// (the new FP64 normal residual).  Converged on the system residual; otherwise the FP32
+// loop resumes with the direction kept (beta = |r_true|^2 / |r_prev|^2) and, if its
+// normal-residual threshold was already met, a threshold lowered by the missing factor.
+__global__ void __launch_bounds__(kNT)
+k_fin_eo_mp_true(const double2 *__restrict__ partRes, const double2 *__restrict__ partN, int nb,
+                 CgState *__restrict__ st, int *__restrict__ active) {
+    __shared__ double2 lds0[kNT / kWave], lds1[kNT / kWave];
+    const int sys = blockIdx.x;
+    const int skip = st[sys].final_;
+    const double2 res = eo_fin_sum(partRes + (size_t)sys * nb, nb, lds0);
+    const double2 rn = eo_fin_sum(partN + (size_t)sys * nb, nb, lds1);
+    if (skip || threadIdx.x) return;
+    CgState &S = st[sys];
+    S.true_rel = S.enorm2 > 0.0 ? sqrt(res.x / S.enorm2) : sqrt(res.x);
+    S.restarts += 1;
+    const bool conv = (S.mode == YCN_TOL_REL_TRUE) ? (S.true_rel <= S.tol) : (sqrt(rn.x) <= S.stop);
+    if (conv || S.iter >= S.maxIter) {
+        S.converged = conv ? 1 : 0;
+        S.final_ = 1;
+        if (!S.done) {
+            S.done = 1;
+            atomicSub(active, 1);
+        }
+        return;
+    }
+    S.beta = rn.x / S.rr_prev;
+    S.rr = rn.x;
+    if (S.mode == YCN_TOL_REL_TRUE && sqrt(rn.x) <= S.stop) S.stop = sqrt(rn.x) * 0.5 * S.tol / S.true_rel;
+    if (S.done) {
+        S.done = 0;
+        atomicAdd(active, 1);
+    }
+}
+
+// ---------------------------------------------------------------------------
+// fused Schur normal operator: p' = r + beta p_old; v = S p'; Ap = S^T v; |v|^2
+// ---------------------------------------------------------------------------
+// One block owns a TT x TX tile of the full lattice (TT, TX even) and produces Ap on the
+// tile's TT*TX/2 even sites.  LDS holds one (TT+8) x (TX+8) array of four SoA planes in
+// full-lattice coordinates; each stage fills the parity it produces, on a halo one site
+// smaller than the stage before:
+//   0: p'  on even sites, 4-site halo (from HBM/L2; p' of the tile also written out)
+//   1: U = K p' / d on odd sites, 3-site halo          (= A_o^-1 K_oe p')
+//   2: v = d p' - K U on even sites, 2-site halo        (= S p', overwrites p')
+//   3: W = H v / d on odd sites, 1-site halo            (overwrites U)
+//   4: Ap = d v - H W on the tile's even sites          (= S^T S p')
+// The diagonal d = 2r + M + g phi is staged for the 3-site halo.  |v|^2 over the tile
+// (= <p', Ap>) is the block's partial.  HBM traffic per launch: read r, p_old (Vh sites
+// each) and phi, write p', Ap: 4 S/2 + P per site of the full lattice.
+constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave : tt * tx / 2; }
+
+template <typename R, int TT, int TX>
+__global__ void __launch_bounds__(eo_threads(TT, TX))
+k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict__ st,
+                  const typename Vec2<R>::T *__restrict__ r, const typename Vec2<R>::T *__restrict__ pold,
+                  typename Vec2<R>::T *__restrict__ pnew, typename Vec2<R>::T *__restrict__ Ap,
+                  double2 *__restrict__ part) {
+    using V2 = typename Vec2<R>::T;
+    constexpr int H = TT + 8, W = TX + 8, HF = TT + 6, WF = TX + 6, NT = eo_threads(TT, TX);
+    __shared__ V2 P[4][H * W];
+    __shared__ R F[HF * WF];
+    __shared__ double2 red[NT / kWave > 0 ? NT / kWave : 1];
+
+    const int sys = blockIdx.y;
+    if (st[sys].done) return;
+    const int nb = gridDim.x;
+    int tile = blockIdx.x;
+    if (A.xcdRemap) tile = (tile % 8) * (nb / 8) + tile / 8;
+    const int ty = tile / A.tilesX, tx = tile - ty * A.tilesX;
+    const int t0 = ty * TT, x0 = tx * TX;
+    const int Nt = A.Nt, Nx = A.Nx, Nh = A.Nh;
+    const size_t Vh = A.Vh, off = (size_t)sys * 4 * Vh;
+    const R beta = (R)st[sys].beta;
+    const R *ph = phi + (size_t)(sys / A.nNoise) * A.V;
+    const R hp = (R)(0.5 * (A.r + 1.0)), hm = (R)(0.5 * (A.r - 1.0)), hr = (R)(0.5 * A.r), h = (R)0.5;
+    const R d0 = (R)(2.0 * A.r + A.M), g = (R)A.g;
+
+    // stage 0: p' on the even sites of the whole array
+    for (int w = threadIdx.x; w < H * (W / 2); w += NT) {
+        const int lr = w / (W / 2), lc = 2 * (w - lr * (W / 2)) + (lr & 1);
+        const int gt = wrapm(t0 - 4 + lr, Nt), gx = wrapm(x0 - 4 + lc, Nx);
+        const size_t j = (size_t)gt * Nh + (gx >> 1);
+        const bool inner = lr >= 4 && lr < 4 + TT && lc >= 4 && lc < 4 + TX;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const V2 a = r[off + c * Vh + j], b = pold[off + c * Vh + j];
+            const V2 pv = mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
+            P[c][lr * W + lc] = pv;
+            if (inner) pnew[off + c * Vh + j] = pv;
+        }
+    }
+    for (int w = threadIdx.x; w < HF * WF; w += NT) {
+        const int lr = w / WF, lc = w - lr * WF;
+        const int gt = wrapm(t0 - 3 + lr, Nt), gx = wrapm(x0 - 3 + lc, Nx);
+        F[w] = d0 + g * ph[(size_t)gt * Nx + gx];
+    }
+    __syncthreads();
+
+    // the sites of parity `par` in rows/cols [4 - hh, 4 + T + hh) of the array
+    auto site_of = [](int w, int hh, int par, int &lr, int &lc) {
+        const int c0 = 4 - hh, half = (TX + 2 * hh) / 2;
+        lr = c0 + w / half;
+        lc = c0 + 2 * (w - (lr - c0) * half) + (((par + lr) ^ c0) & 1);
+    };
+    auto hop_at = [&](auto dag, int lr, int lc, V2 o[4]) {
+        constexpr bool DAG = decltype(dag)::value;
+        const int gt = wrapm(t0 - 4 + lr, Nt);
+        const R sf = (gt == Nt - 1) ? (R)-1 : (R)1, sb = (gt == 0) ? (R)-1 : (R)1;
+        const int i = lr * W + lc;
+#pragma unroll
+        for (int f = 0; f < 4; f += 2)
+            hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, P[f][i + W], P[f + 1][i + W], sf, P[f][i - W], P[f + 1][i - W],
+                                      sb, P[f][i + 1], P[f + 1][i + 1], P[f][i - 1], P[f + 1][i - 1], o[f], o[f + 1]);
+    };
+    auto dg = [&](int lr, int lc) { return F[(lr - 1) * WF + (lc - 1)]; };
+
+    // stage 1: U = K p' / d on odd sites, 3-site halo
+    for (int w = threadIdx.x; w < (TT + 6) * ((TX + 6) / 2); w += NT) {
+        int lr, lc;
+        site_of(w, 3, 1, lr, lc);
+        V2 o[4];
+        hop_at(std::integral_constant<bool, true>(), lr, lc, o);
+        const R inv = (R)1 / dg(lr, lc);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) P[c][lr * W + lc] = mk2<R>(o[c].x * inv, o[c].y * inv);
+    }
+    __syncthreads();
+
+    // stage 2: v = d p' - K U on even sites, 2-site halo; |v|^2 over the tile
+    double acc = 0.0;
+    for (int w = threadIdx.x; w < (TT + 4) * ((TX + 4) / 2); w += NT) {
+        int lr, lc;
+        site_of(w, 2, 0, lr, lc);
+        V2 o[4];
+        hop_at(std::integral_constant<bool, true>(), lr, lc, o);
+        const R d = dg(lr, lc);
+        const bool inner = lr >= 4 && lr < 4 + TT && lc >= 4 && lc < 4 + TX;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const V2 pv = P[c][lr * W + lc];
+            const V2 v = mk2<R>(d * pv.x - o[c].x, d * pv.y - o[c].y);
+            P[c][lr * W + lc] = v;
+            if (inner) acc += (double)(v.x * v.x + v.y * v.y);
+        }
+    }
+    __syncthreads();
+
+    // stage 3: W = H v / d on odd sites, 1-site halo
+    for (int w = threadIdx.x; w < (TT + 2) * ((TX + 2) / 2); w += NT) {
+        int lr, lc;
+        site_of(w, 1, 1, lr, lc);
+        V2 o[4];
+        hop_at(std::integral_constant<bool, false>(), lr, lc, o);
+        const R inv = (R)1 / dg(lr, lc);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) P[c][lr * W + lc] = mk2<R>(o[c].x * inv, o[c].y * inv);
+    }
+    __syncthreads();
+
+    // stage 4: Ap = d v - H W on the tile's even sites
+    for (int w = threadIdx.x; w < TT * (TX / 2); w += NT) {
+        int lr, lc;
+        site_of(w, 0, 0, lr, lc);
+        V2 o[4];
+        hop_at(std::integral_constant<bool, false>(), lr, lc, o);
+        const R d = dg(lr, lc);
+        const int gt = wrapm(t0 - 4 + lr, Nt), gx = wrapm(x0 - 4 + lc, Nx);
+        const size_t j = (size_t)gt * Nh + (gx >> 1);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const V2 v = P[c][lr * W + lc];
+            Ap[off + c * Vh + j] = mk2<R>(d * v.x - o[c].x, d * v.y - o[c].y);
+        }
+    }
+    double2 tot = block_sum2<NT>(make_double2(acc, 0.0), red);
+    if (threadIdx.x == 0) part[(size_t)sys * nb + blockIdx.x] = tot;
+}
+
+// ---------------------------------------------------------------------------
+// host side
+// ---------------------------------------------------------------------------
+// at least 64 sites per tile: the partials buffer holds V/64 entries per system
+constexpr TileShape kEoTiles64[] = {{8, 32}, {16, 16}, {4, 64}, {4, 32}, {2, 32}};
+constexpr TileShape kEoTiles32[] = {{8, 32}, {16, 32}, {8, 64}, {16, 16}, {4, 64}, {2, 32}};
+constexpr int kNumEo64 = sizeof(kEoTiles64) / sizeof(kEoTiles64[0]);
+constexpr int kNumEo32 = sizeof(kEoTiles32) / sizeof(kEoTiles32[0]);
+
+static EoArgs eo_args(const Ctx &c, int TT, int TX) {
+    EoArgs A;
+    A.Nt = c.p.Nt;
+    A.Nx = c.p.Nx;
+    A.Nh = c.p.Nx / 2;
+    A.V = c.V;
+    A.Vh = c.V / 2;
+    A.nNoise = c.nNoise;
+    A.r = c.p.r;
+    A.M = c.p.M;
+    A.g = c.p.g;
+    A.tilesX = TX > 0 ? c.p.Nx / TX : 0;
+    A.tilesPerSys = (TT > 0 && TX > 0) ? (c.p.Nt / TT) * (c.p.Nx / TX) : 0;
+    A.xcdRemap = (A.tilesPerSys % 8 == 0) ? 1 : 0;
+    return A;
+}
+
+static bool eo_fits(const Ctx &c, TileShape t) { return c.p.Nt % t.TT == 0 && c.p.Nx % t.TX == 0; }
+
+bool eo_available(const Ctx &c) { return c.p.Nt % 2 == 0 && c.p.Nx % 2 == 0 && c.tileEo64 >= 0; }
+
+template <typename R, int TT, int TX>
+static void launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, const CgState *st,
+                         const typename Vec2<R>::T *r, const typename Vec2<R>::T *pold, typename Vec2<R>::T *pnew,
+                         typename Vec2<R>::T *Ap, double2 *part) {
+    hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX>), dim3(A.tilesPerSys, nsys), dim3(eo_threads(TT, TX)), 0,
+                       c.stream, A, phi, st, r, pold, pnew, Ap, part);
+}
+
+// launch the fused Schur normal operator with tile `ti`; returns the partial count
+static int launch_eo_fused(Ctx &c, int ti, int nsys, const double *phi, const double2 *r, const double2 *pold,
+                           double2 *pnew, double2 *Ap) {
+    const TileShape t = kEoTiles64[ti];
+    const EoArgs A = eo_args(c, t.TT, t.TX);
+    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (2 * 64 + 8));
+    switch (ti) {
+    case 0: launch_eo_tt<double, 8, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    case 1: launch_eo_tt<double, 16, 16>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    case 2: launch_eo_tt<double, 4, 64>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    case 3: launch_eo_tt<double, 4, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    default: launch_eo_tt<double, 2, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    }
+    prof_after(c, PK_NORMAL_OP);
+    return A.tilesPerSys;
+}
+
+static int launch_eo_fused32(Ctx &c, int ti, int nsys, const float *phi, const float2 *r, const float2 *pold,
+                             float2 *pnew, float2 *Ap) {
+    const TileShape t = kEoTiles32[ti];
+    const EoArgs A = eo_args(c, t.TT, t.TX);
+    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (2 * 32 + 4));
+    switch (ti) {
+    case 0: launch_eo_tt<float, 8, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    case 1: launch_eo_tt<float, 16, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    case 2: launch_eo_tt<float, 8, 64>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    case 3: launch_eo_tt<float, 16, 16>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    case 4: launch_eo_tt<float, 4, 64>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    default: launch_eo_tt<float, 2, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    }
+    prof_after(c, PK_NORMAL_OP);
+    return A.tilesPerSys;
+}
+
+static dim3 half_grid(const Ctx &c, int nsys) { return dim3((c.V / 2 + kNT - 1) / kNT, nsys); }
+
+// Times every even-odd tile that fits (FP64, and FP32 when the mixed buffers exist) on
+// the context's buffers and keeps the fastest.  The trial launches only touch CG scratch.
+int eo_autotune(Ctx &c) {
+    c.tileEo64 = c.tileEo32 = -1;
+    if (c.p.Nt % 2 || c.p.Nx % 2) return YCN_OK;
+    hipEvent_t e0, e1;
+    YCN_HIP(hipEventCreate(&e0));
+    YCN_HIP(hipEventCreate(&e1));
+    YCN_HIP(hipMemsetAsync(c.cg_d, 0, sizeof(CgState) * c.nSys, c.stream));
+    const int reps = c.V >= (1 << 16) ? 10 : 3;
+    auto time_it = [&](auto launch) -> double {
+        launch(0);
+        launch(1);
+        (void)hipEventRecord(e0, c.stream);
+        for (int i = 0; i < reps; ++i) launch(i & 1);
+        (void)hipEventRecord(e1, c.stream);
+        (void)hipEventSynchronize(e1);
+        float ms = 0;
+        (void)hipEventElapsedTime(&ms, e0, e1);
+        return 1e3 * ms / reps;
+    };
+    char line[96];
+    c.tuneReport += " | even-odd fp64 tiles (us/launch):";
+    double best = 1e30;
+    for (int i = 0; i < kNumEo64; ++i) {
+        if (!eo_fits(c, kEoTiles64[i])) continue;
+        const double us = time_it([&](int par) {
+            (void)launch_eo_fused(c, i, c.nSys, c.phi[c.cur], c.r, c.pv[par], c.pv[par ^ 1], c.Ap);
+        });
+        std::snprintf(line, sizeof line, " %dx%d %.1f;", kEoTiles64[i].TT, kEoTiles64[i].TX, us);
+        c.tuneReport += line;
+        if (us < best) { best = us; c.tileEo64 = i; }
+    }
+    if (c.rs && c.phis) {
+        c.tuneReport += " | even-odd fp32 tiles (us/launch):";
+        best = 1e30;
+        for (int i = 0; i < kNumEo32; ++i) {
+            if (!eo_fits(c, kEoTiles32[i])) continue;
+            const double us = time_it([&](int par) {
+                (void)launch_eo_fused32(c, i, c.nSys, c.phis, c.rs, c.ps[par], c.ps[par ^ 1], c.Aps);
+            });
+            std::snprintf(line, sizeof line, " %dx%d %.1f;", kEoTiles32[i].TT, kEoTiles32[i].TX, us);
+            c.tuneReport += line;
+            if (us < best) { best = us; c.tileEo32 = i; }
+        }
+    }
+    // YCN_TILE_EO / YCN_TILE_EO32 force a tile of the tables (tests cover every shape)
+    if (const char *f = std::getenv("YCN_TILE_EO")) {
+        const int i = std::atoi(f);
+        if (i >= 0 && i < kNumEo64 && eo_fits(c, kEoTiles64[i])) c.tileEo64 = i;
+        c.tuneReport += " (even-odd fp64 tile forced by YCN_TILE_EO)";
+    }
+    if (const char *f = std::getenv("YCN_TILE_EO32")) {
+        const int i = std::atoi(f);
+        if (c.rs && c.phis && i >= 0 && i < kNumEo32 && eo_fits(c, kEoTiles32[i])) c.tileEo32 = i;
+        c.tuneReport += " (even-odd fp32 tile forced by YCN_TILE_EO32)";
+    }
+    (void)hipEventDestroy(e0);
+    (void)hipEventDestroy(e1);
+    YCN_HIP(hipGetLastError());
+    YCN_HIP(hipMemsetAsync(c.cg_d, 0, sizeof(CgState) * c.nSys, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    return YCN_OK;
+}
+
+// scratch of a solve: b_e and U_o in c.t (first and second half), the residual in c.Ap
+struct EoBufs {
+    double2 *b, *U, *res;
+};
+static EoBufs eo_bufs(Ctx &c, int nsys) {
+    const size_t h = (size_t)nsys * 4 * (c.V / 2);
+    return {c.t, c.t + h, c.Ap};
+}
+
+static void launch_eo_iteration(Ctx &c, int nsys, const double *phi, int parity) {
+    const int nb = launch_eo_fused(c, c.tileEo64, nsys, phi, c.r, c.pv[parity], c.pv[parity ^ 1], c.Ap);
+    launch_fin_alpha(c, nsys, nb);
+    const int nbu = launch_cg_update(c, nsys, c.V / 2, c.pv[parity ^ 1]);
+    launch_fin_beta(c, nsys, nbu, false);
+}
+
+static void launch_eo_mp_iteration(Ctx &c, int nsys, int parity) {
+    const int nb = launch_eo_fused32(c, c.tileEo32, nsys, c.phis, c.rs, c.ps[parity], c.ps[parity ^ 1], c.Aps);
+    launch_fin_alpha(c, nsys, nb);
+    const int nbu = launch_mp_update(c, nsys, c.V / 2, c.ps[parity ^ 1]);
+    launch_fin_beta(c, nsys, nbu, true);
+}
+
+// Solve S chi_e = b_e for every system (eta from the Philox streams when gen); chi_e is
+// left in c.x, eta in c.eta.  FP64 CG, or FP32 inner iterations with FP64 reliable
+// updates when cgPrecision is mixed.
+int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase, uint64_t step, SolveStats *st) {
+    if (!eo_available(c)) {
+        c.err = "even-odd preconditioning needs even Nt and Nx and a fused tile that divides the lattice";
+        return YCN_ERR_UNSUPPORTED;
+    }
+    const bool mixed = c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis;
+    const EoArgs A = eo_args(c, 0, 0);
+    const int Vh = c.V / 2;
+    const dim3 gF((c.V + kNT - 1) / kNT, nsys), gH = half_grid(c, nsys);
+    const int nbF = gF.x, nbH = gH.x;
+    double2 *partE = c.partials, *partH = c.partials + (size_t)nsys * nbF, *partH2 = partH + (size_t)nsys * nbH;
+    const EoBufs B = eo_bufs(c, nsys);
+    int rc = set_int(c, c.active, nsys);
+    if (rc) return rc;
+    const int maxIter = c.p.cgMaxIter, maxRestarts = mixed ? 1 << 20 : 4;
+    if (gen)
+        hipLaunchKernelGGL(k_eo_eta<true>, gF, dim3(kNT), 0, c.stream, A, (const uint64_t *)c.seed_d, streamBase, step,
+                           c.eta, partE);
+    else
+        hipLaunchKernelGGL(k_eo_eta<false>, gF, dim3(kNT), 0, c.stream, A, (const uint64_t *)c.seed_d, streamBase,
+                           step, c.eta, partE);
+    hipLaunchKernelGGL(k_eo_rhs, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.eta, B.b);
+    // r = S^T b (and its FP32 copy), x = 0, p = 0
+    hipLaunchKernelGGL(k_eo_hop_oe<false>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.b, B.U);
+    hipLaunchKernelGGL((k_eo_hop_eo<false, 2>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.b,
+                       (const double2 *)B.U, (const double2 *)nullptr, c.r, mixed ? c.rs : (float2 *)nullptr, partH);
+    const int64_t nh = (int64_t)nsys * 4 * Vh;
+    hipLaunchKernelGGL(k_eo_zero, dim3((unsigned)((nh + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, nh, c.x, c.pv[0],
+                       c.pv[1]);
+    if (mixed) {
+        hipLaunchKernelGGL(k_eo_zero32, dim3((unsigned)((nh + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, nh, c.xs,
+                           c.ps[0], c.ps[1]);
+        const int nrepUsed = (nsys + c.nNoise - 1) / c.nNoise;
+        rc = to_f32_n(c, phi, c.phis, (int64_t)std::min(nrepUsed, c.nRep) * c.V);
+        if (rc) return rc;
+    }
+    hipLaunchKernelGGL(k_fin_eo_setup, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partE, nbF,
+                       (const double2 *)partH, nbH, c.cg_d, c.active, c.p.cgTol, c.p.cgTolMode, maxIter, maxRestarts);
+    YCN_HIP(hipGetLastError());
+
+    int launched = 0, active = nsys;
+    if (mixed) {
+        const int every = std::max(1, c.p.reliableEvery);
+        const int nIt = every + (every & 1);
+        const int hardCap = maxIter + nIt + 8;
+        rc = poll_active(c, active);
+        if (rc) return rc;
+        while (active > 0 && launched < hardCap) {
+            rc = run_chunk(c, phi, nsys, YCN_DAGGER, nIt, 4, c.tileEo32,
+                           [&](int par) { launch_eo_mp_iteration(c, nsys, par); });
+            if (rc) return rc;
+            launched += nIt;
+            // reliable update: x += x_s; res = b - S x; r = S^T res (FP64, FP32 copy)
+            rc = launch_mp_accum(c, nsys, Vh);
+            if (rc) return rc;
+            hipLaunchKernelGGL(k_eo_hop_oe<true>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x, B.U);
+            hipLaunchKernelGGL((k_eo_hop_eo<true, 1>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x,
+                               (const double2 *)B.U, (const double2 *)B.b, B.res, (float2 *)nullptr, partH);
+            hipLaunchKernelGGL(k_eo_hop_oe<false>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.res, B.U);
+            hipLaunchKernelGGL((k_eo_hop_eo<false, 2>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.res,
+                               (const double2 *)B.U, (const double2 *)nullptr, c.r, c.rs, partH2);
+            hipLaunchKernelGGL(k_fin_eo_mp_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partH,
+                               (const double2 *)partH2, nbH, c.cg_d, c.active);
+            YCN_HIP(hipGetLastError());
+            rc = poll_active(c, active);
+            if (rc) return rc;
+        }
+        return read_stats(c, nsys, st);
+    }
+
+    int chunk = std::max(kGraphIters, std::min(c.lastIters, maxIter));
+    const int hardCap = maxIter * (maxRestarts + 1) + 2 * kGraphIters;
+    for (int guard = 0; guard < maxRestarts + 2; ++guard) {
+        while (active > 0 && launched < hardCap) {
+            const int n = std::min(chunk, hardCap - launched);
+            for (int k = 0; k < n; k += kGraphIters) {
+                rc = run_chunk(c, phi, nsys, YCN_DAGGER, kGraphIters, 3, c.tileEo64,
+                               [&](int par) { launch_eo_iteration(c, nsys, phi, par); });
+                if (rc) return rc;
+            }
+            launched += ((n + kGraphIters - 1) / kGraphIters) * kGraphIters;
+            rc = poll_active(c, active);
+            if (rc) return rc;
+            chunk = std::max(kGraphIters, std::min(64, chunk / 2));
+        }
+        // true residual of the unpreconditioned system, |b_e - S x_e| / |eta|
+        hipLaunchKernelGGL(k_eo_hop_oe<true>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x, B.U);
+        hipLaunchKernelGGL((k_eo_hop_eo<true, 1>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x,
+                           (const double2 *)B.U, (const double2 *)B.b, B.res, (float2 *)nullptr, partH);
+        hipLaunchKernelGGL(k_fin_eo_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partH, nbH, c.cg_d,
+                           c.active);
+        YCN_HIP(hipGetLastError());
+        rc = poll_active(c, active);
+        if (rc) return rc;
+        if (active == 0 || launched >= hardCap) break;
+        chunk = kGraphIters;
+    }
+    return read_stats(c, nsys, st);
+}
+
+int eo_drift(Ctx &c, const double *phi, double *drift) {
+    const EoArgs A = eo_args(c, 0, 0);
+    const double fac = -c.p.g * (c.p.driftMode == 1 ? c.p.cutFraction * c.p.cutFraction : 1.0);
+    hipLaunchKernelGGL(k_eo_drift, dim3((c.V + kNT - 1) / kNT, c.nRep), dim3(kNT), 0, c.stream, A, phi,
+                       (const double2 *)c.x, (const double2 *)c.eta, drift, fac);
+    YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+
+}  // namespace ycn

@@ -113,7 +113,7 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         (p->cgTolMode != YCN_TOL_ABS_RECURSIVE && p->cgTolMode != YCN_TOL_REL_TRUE) ||
         (p->driftMode != 0 && p->driftMode != 1) ||
         (p->cgPrecision != YCN_CG_FP64 && p->cgPrecision != YCN_CG_MIXED) || p->reliableEvery < 1 ||
-        (p->precond != YCN_PRECOND_NONE && p->precond != YCN_PRECOND_CNN))
+        (p->precond != YCN_PRECOND_NONE && p->precond != YCN_PRECOND_CNN) || (p->evenOdd != 0 && p->evenOdd != 1))
         return YCN_ERR_INVALID;
     if ((int64_t)p->Nt * p->Nx > (int64_t)1 << 28) return YCN_ERR_INVALID;
     int ndev = 0;
@@ -158,14 +158,26 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
     // FP32 and CNN buffers whenever the lattice admits the FP32 tiles / CNN tiles
     const bool f32ok = (p->Nt % 4 == 0) && (p->Nx % 16 == 0);
     const bool cnnok = (p->Nt % 16 == 0) && (p->Nx % 16 == 0);
-    if (f32ok && (rc = mp_alloc(c))) return bail(rc);
+    const bool eoDims = (p->Nt % 2 == 0) && (p->Nx % 2 == 0);
+    if ((f32ok || (p->evenOdd && eoDims)) && (rc = mp_alloc(c))) return bail(rc);
     if (cnnok && (rc = cnn_alloc(c))) return bail(rc);
     if (c.p.precond == YCN_PRECOND_CNN && !cnnok) {
         c.err = "the learned preconditioner needs Nt % 16 == 0 and Nx % 16 == 0";
         return bail(YCN_ERR_UNSUPPORTED);
     }
     if ((rc = cg_autotune(c))) return bail(rc);
-    if (c.p.cgPrecision == YCN_CG_MIXED && c.tile32 < 0) {
+    if (c.p.evenOdd) {
+        if (c.p.precond == YCN_PRECOND_CNN) {
+            c.err = "even-odd preconditioning and the learned preconditioner are exclusive";
+            return bail(YCN_ERR_UNSUPPORTED);
+        }
+        if ((rc = eo_autotune(c))) return bail(rc);
+        if (!eo_available(c)) {
+            c.err = "even-odd preconditioning needs Nt even and Nx % 32 == 0, or Nt % 16 == 0 and Nx % 16 == 0";
+            return bail(YCN_ERR_UNSUPPORTED);
+        }
+    }
+    if (c.p.cgPrecision == YCN_CG_MIXED && c.tile32 < 0 && !(c.p.evenOdd && c.tileEo32 >= 0)) {
         c.err = "mixed-precision CG needs Nt % 4 == 0 and Nx % 16 == 0";
         return bail(YCN_ERR_UNSUPPORTED);
     }
@@ -322,6 +334,11 @@ int ycn_cg_solve(ycn_ctx *ctx, const double *phi, const double *b, double *x, in
 }
 
 static int force_impl(Ctx &c, uint64_t step, uint32_t streamBase, SolveStats *st) {
+    if (c.p.evenOdd) {   // chi = (D^dag)^-1 eta through the Schur complement on the even sites
+        int rc = eo_solve(c, c.phi[c.cur], c.nSys, true, streamBase, step, st);
+        if (rc) return rc;
+        return eo_drift(c, c.phi[c.cur], c.drift);
+    }
     int rc = cg_solve(c, c.phi[c.cur], c.nSys, YCN_DAGGER, true, streamBase, step, st);
     if (rc) return rc;
     return fermion_drift(c, c.phi[c.cur], c.drift);

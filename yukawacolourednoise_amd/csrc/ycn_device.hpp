@@ -174,4 +174,18 @@ __device__ __forceinline__ void wilson_flavour(const WilsonCoef &w, double d, V2
     o1.y = d * in1.y - s1.y;
 }
 
+// Gaussian noise vector of system sys at site s: component c is normal number 4s + c
+// of Philox stream (seed of the replica, streamBase + noise index), imaginary part 0
+// (replaces fillNormalRND, FermionicDrift.cu:141-149).
+__device__ __forceinline__ void eta_site(const uint64_t *seeds, uint32_t streamBase, uint64_t step, int sys,
+                                         int nNoise, int s, double2 b[4]) {
+    const int rep = sys / nNoise, j = sys - rep * nNoise;
+    const double2 g0 = normal_pair(seeds[rep], streamBase + (uint32_t)j, step, (uint64_t)2 * s);
+    const double2 g1 = normal_pair(seeds[rep], streamBase + (uint32_t)j, step, (uint64_t)2 * s + 1);
+    b[0] = make_double2(g0.x, 0.0);
+    b[1] = make_double2(g0.y, 0.0);
+    b[2] = make_double2(g1.x, 0.0);
+    b[3] = make_double2(g1.y, 0.0);
+}
+
 }  // namespace ycn

@@ -45,6 +45,7 @@ struct CgState {
     double true_prev;  // flexible CG: true relative residual at the previous reliable update
     int precOff;       // flexible CG: 1 = preconditioner switched off (plain CG fallback)
     int restartNext;   // flexible CG: next beta = 0
+    double enorm2;     // even-odd solve: |eta|^2 of the unpreconditioned system
 };
 
 // Per-replica Langevin state (device resident).
@@ -92,6 +93,8 @@ struct Ctx {
 
     // fused normal-operator tiles chosen by the autotuner (-1: three-kernel path)
     int tile64 = -1, tile32 = -1;
+    // even-odd fused Schur normal operator tiles (-1: even-odd not available)
+    int tileEo64 = -1, tileEo32 = -1;
     std::string tuneReport;
 
     // mixed precision: [nSys][4][V] float2
@@ -190,6 +193,24 @@ int bosonic_update(Ctx &c, int fixedEps);
 int field_slices(Ctx &c, int rep, double *slices_dev, double *sums_dev);
 int hot_start(Ctx &c, uint64_t initSeed);
 int set_int(Ctx &c, int *ptr, int v);
+int poll_active(Ctx &c, int &active);
+int read_stats(Ctx &c, int nsys, SolveStats *st);
+// CG building blocks shared with the even-odd solver (cg.hip): finalize alpha / beta from
+// the partials in c.partials, the BLAS-1 updates x += alpha p, r -= alpha Ap on
+// [nsys][4][n] vectors (c.x, c.r, c.Ap; mixed: c.xs, c.rs, c.Aps), returning their block
+// count, the reliable-update accumulation x += x_s, and an FP64 -> FP32 copy.
+void launch_fin_alpha(Ctx &c, int nsys, int nb);
+void launch_fin_beta(Ctx &c, int nsys, int nb, bool mixed);
+int launch_cg_update(Ctx &c, int nsys, int n, const double2 *p);
+int launch_mp_update(Ctx &c, int nsys, int n, const float2 *p);
+int launch_mp_accum(Ctx &c, int nsys, int n);
+int to_f32_n(Ctx &c, const double *a, float *b, int64_t n);
+
+// even-odd preconditioned fermion force (evenodd.hip)
+bool eo_available(const Ctx &c);
+int eo_autotune(Ctx &c);
+int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase, uint64_t step, SolveStats *st);
+int eo_drift(Ctx &c, const double *phi, double *drift);
 
 // precision conversion
 int spinor_to_f32(Ctx &c, const double2 *a, float2 *b, int nsys);
