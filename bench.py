@@ -13,8 +13,10 @@ bosonic drift, update), bracketed by device synchronisation.  With the learned m
 the thermalisation steps are also its online training (untimed, reported).  Inputs
 and state stay resident in HBM.
 
-Prints one JSON line.  `roofline` is measured live: every launch of the dominant kernel
-in the timed region is bracketed by HIP events on the library's own stream.
+Prints one JSON line.  The timed region runs uninstrumented, CG iteration chunks replayed
+as captured hipGraphs.  `roofline` is measured live right after it: in a profiled pass
+of `prof_steps` steps every launch of the dominant kernel is bracketed by HIP events on
+the library's own stream (launches that exit at once for converged systems excluded).
 `cpu_baseline` times the FP64 CPU oracle on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -130,10 +132,16 @@ def thermalise(Y, size, therm, train):
     return ctx, phi, info, secs
 
 
-def run_method(Y, ctx, size, method, phi0, step0, steps, warmup, profile_kernel):
+def run_method(Y, ctx, size, method, phi0, step0, steps, warmup, profile_kernel, prof_steps):
     """Time `steps` Langevin steps of one CG method.  Every method starts from the same
     thermalised field phi0 at the same step counter step0, so all of them follow the same
-    trajectory (same noise; drifts agree to the CG tolerance) and solve the same systems."""
+    trajectory (same noise; drifts agree to the CG tolerance) and solve the same systems.
+
+    The timed region runs uninstrumented (CG chunks replayed as captured graphs).  A
+    profiled pass of `prof_steps` more steps follows it: every launch of
+    `profile_kernel` there is bracketed by HIP events on the library's stream (which
+    launches kernels one by one instead of replaying graphs); its average duration gives
+    the roofline figure."""
     if ctx is None:
         ctx = make_ctx(Y, size, method)
     tuning = ctx.tuning_report()
@@ -143,13 +151,16 @@ def run_method(Y, ctx, size, method, phi0, step0, steps, warmup, profile_kernel)
         ctx.langevin_step(step)
         step += 1
     ctx.sync()
-    if profile_kernel:
-        ctx.profile_begin(profile_kernel)
     t0 = time.perf_counter()
     st = ctx.langevin_run(step, steps)
     ctx.sync()
     dt = time.perf_counter() - t0
-    prof = ctx.profile_end() if profile_kernel else (0.0, 0, 0.0)
+    step += steps
+    prof = (0.0, 0, 0.0)
+    if profile_kernel and prof_steps > 0:
+        ctx.profile_begin(profile_kernel)
+        ctx.langevin_run(step, prof_steps)
+        prof = ctx.profile_end()
     mag = float(ctx.get_field().mean())
     ctx.close()
     return {"steps_per_s": steps / dt, "ms_per_step": 1e3 * dt / steps,
@@ -187,6 +198,8 @@ def main():
                     help="untimed thermalisation steps shared by every method")
     ap.add_argument("--method", choices=["fp64", "mixed", "learned", "eo-fp64", "eo-mixed", "all"], default="all")
     ap.add_argument("--profile-kernel", default="cg_normal_op")
+    ap.add_argument("--prof-steps", type=int, default=3,
+                    help="steps of the profiled pass after the timed region (roofline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="run only a 1 GiB device copy (HBM counter calibration under rocprofv3 --pmc)")
@@ -210,7 +223,7 @@ def main():
     res = {}
     for m in methods:
         res[m] = run_method(Y, lctx if m == "learned" else None, args.size, m, phi0, args.therm,
-                            args.steps, args.warmup, args.profile_kernel)
+                            args.steps, args.warmup, args.profile_kernel, args.prof_steps)
         if m == "learned":
             res[m]["training"] = train_info
         print(f"# {m}: {res[m]['steps_per_s']:.3f} steps/s, {res[m]['cg_iters_per_solve']:.0f} CG iters/solve",

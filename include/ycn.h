@@ -208,10 +208,12 @@ int ycn_precond_train(ycn_ctx *ctx, uint64_t step0, int nsteps, int pairsPerSolv
                       ycn_train_stats *stats);
 
 /* Kernel timing with HIP events on the context stream (for roofline figures):
- * between begin and end every launch of the named kernel is bracketed by events;
- * end returns the summed duration, the launch count and the algorithmic bytes
- * moved per launch.  Kernel names: "cg_normal_op", "cg_update", "dirac",
- * "noise_cols", "bosonic", "cnn". */
+ * between begin and end every launch of the named kernel is bracketed by events
+ * (iteration chunks are then launched one kernel at a time, not as graph replays);
+ * end returns the summed duration and count of the launches that did work (launches
+ * shorter than a quarter of the median exited at once for converged systems and are
+ * left out) and the algorithmic bytes moved per launch.  Kernel names:
+ * "cg_normal_op", "cg_update", "dirac", "noise_cols", "bosonic", "cnn". */
 int ycn_profile_begin(ycn_ctx *ctx, const char *kernel);
 int ycn_profile_end(ycn_ctx *ctx, double *total_ms, int64_t *launches, double *bytes_per_launch);
 

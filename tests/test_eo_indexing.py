@@ -1,9 +1,10 @@
 """CPU check of the index arithmetic of the fused even-odd kernel (k_eo_normal_fused in
 yukawacolourednoise_amd/csrc/evenodd.hip), transcribed here: for every tile of its
 tables and every lattice the tables accept, including time extents smaller than the
-4-site halo (Nt = 2), every global and LDS access stays in bounds, every stage covers
-its region's sites of the right parity exactly once, and the stage-4 outputs cover the
-even sites of the lattice exactly once."""
+4-site halo (Nt = 2), every global and LDS access stays in bounds, the parity-split LDS
+slots used for each neighbour hold exactly that neighbour, every stage covers its
+region's sites of the right parity once, and stage 4 writes every even site of the
+lattice once."""
 import itertools
 
 import pytest
@@ -13,57 +14,60 @@ EO32 = [(8, 32), (16, 32), (8, 64), (16, 16), (4, 64), (2, 32)]
 
 
 def wrapm(a, n):
-    m = a % n          # Python % is already non-negative; the kernel adds n when negative
+    m = a % n
     return m
 
 
-def site_of(w, hh, par, TX):
+def slot_col(row, k, par):
+    """array column held by slot (row, k) of the parity-`par` array"""
+    return 2 * k + ((row + par) & 1)
+
+
+def site_of(w, hh, par, TX, K):
     c0 = 4 - hh
     half = (TX + 2 * hh) // 2
-    lr = c0 + w // half
-    lc = c0 + 2 * (w - (lr - c0) * half) + (((par + lr) ^ c0) & 1)
-    return lr, lc
+    row = c0 + w // half
+    col = c0 + 2 * (w - (row - c0) * half) + (((par + row) ^ c0) & 1)
+    k = col >> 1
+    i = row * K + k
+    return row, col, i, i + K, i - K, row * K + ((col + 1) >> 1), row * K + ((col - 1) >> 1)
 
 
 def check(Nt, Nx, TT, TX):
-    H, W, HF, WF = TT + 8, TX + 8, TT + 6, TX + 6
+    H, K = TT + 8, (TX + 8) // 2
     Nh, Vh = Nx // 2, Nt * Nx // 2
     out = set()
     for ty, tx in itertools.product(range(Nt // TT), range(Nx // TX)):
         t0, x0 = ty * TT, tx * TX
-        # stage 0: even sites of the whole array
-        seen = set()
-        for w in range(H * (W // 2)):
-            lr = w // (W // 2)
-            lc = 2 * (w - lr * (W // 2)) + (lr & 1)
-            assert 0 <= lr < H and 0 <= lc < W and (lr + lc) % 2 == 0
-            gt, gx = wrapm(t0 - 4 + lr, Nt), wrapm(x0 - 4 + lc, Nx)
-            assert (gt + gx) % 2 == 0
-            j = gt * Nh + (gx >> 1)
-            assert 0 <= j < Vh
-            seen.add((lr, lc))
-        assert len(seen) == H * W // 2
-        for w in range(HF * WF):
-            lr, lc = w // WF, w % WF
-            gt, gx = wrapm(t0 - 3 + lr, Nt), wrapm(x0 - 3 + lc, Nx)
-            assert 0 <= gt * Nx + gx < Nt * Nx
-        # stages 1-4
+        for w in range(H * K):
+            row, k = w // K, w % K
+            gt = wrapm(t0 - 4 + row, Nt)
+            colE, colO = 2 * k + (row & 1), 2 * k + 1 - (row & 1)
+            assert (row + colE) % 2 == 0 and (row + colO) % 2 == 1
+            gxE, gxO = wrapm(x0 - 4 + colE, Nx), wrapm(x0 - 4 + colO, Nx)
+            assert (gt + gxE) % 2 == 0
+            assert 0 <= gt * Nh + (gxE >> 1) < Vh
+            assert 0 <= gt * Nx + gxE < Nt * Nx and 0 <= gt * Nx + gxO < Nt * Nx
         for hh, par in [(3, 1), (2, 0), (1, 1), (0, 0)]:
             n = (TT + 2 * hh) * ((TX + 2 * hh) // 2)
             cells = set()
             for w in range(n):
-                lr, lc = site_of(w, hh, par, TX)
-                assert (lr + lc) % 2 == par
-                assert 4 - hh <= lr < 4 + TT + hh and 4 - hh <= lc < 4 + TX + hh
-                for dr, dc in [(1, 0), (-1, 0), (0, 1), (0, -1)]:
-                    assert 0 <= lr + dr < H and 0 <= lc + dc < W
-                assert 0 <= (lr - 1) * WF + (lc - 1) < HF * WF
-                cells.add((lr, lc))
+                row, col, i, itf, itb, ixf, ixb = site_of(w, hh, par, TX, K)
+                assert (row + col) % 2 == par
+                assert slot_col(row, col >> 1, par) == col and i == row * K + (col >> 1)
+                for idx in (i, itf, itb, ixf, ixb):
+                    assert 0 <= idx < H * K
+                # neighbour slots of the other parity hold (row +- 1, col), (row, col +- 1)
+                q = 1 - par
+                assert slot_col(row + 1, itf % K, q) == col and itf // K == row + 1
+                assert slot_col(row - 1, itb % K, q) == col and itb // K == row - 1
+                assert slot_col(row, ixf % K, q) == col + 1 and ixf // K == row
+                assert slot_col(row, ixb % K, q) == col - 1 and ixb // K == row
+                cells.add((row, col))
                 if hh == 0:
-                    gt, gx = wrapm(t0 - 4 + lr, Nt), wrapm(x0 - 4 + lc, Nx)
+                    gt, gx = wrapm(t0 - 4 + row, Nt), wrapm(x0 - 4 + col, Nx)
                     j = gt * Nh + (gx >> 1)
-                    assert 0 <= j < Vh
-                    assert j not in out
+                    assert 0 <= j < Vh and j not in out
                     out.add(j)
             assert len(cells) == n
     assert out == set(range(Vh))

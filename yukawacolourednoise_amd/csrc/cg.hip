@@ -160,7 +160,11 @@ __global__ void __launch_bounds__(kNT) k_fin_alpha(const double2 *__restrict__ p
 
 // beta = |r'|^2 / |r|^2 and the reference stopping rule (CGsolver.cu:160,208-211).
 // MIXED: the FP32 loop pauses at the stop or the cap and waits for a reliable update.
-template <bool MIXED>
+// EVEN: a system pauses only after an even number of iterations (or at the cap): chunks
+// have even length and p ping-pongs between two buffers, so the last direction is then
+// in p[0], where the iteration that resumes it after a reliable update or a tightened
+// threshold reads it.
+template <bool MIXED, bool EVEN = MIXED>
 __global__ void __launch_bounds__(kNT)
 k_fin_beta(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active) {
     __shared__ double2 lds[kNT / kWave];
@@ -174,7 +178,7 @@ k_fin_beta(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, i
     S.beta = rrn / S.rr;
     S.rr = rrn;
     S.iter += 1;
-    const bool run = (S.iter < S.maxIter) && (sqrt(rrn) > S.stop);
+    const bool run = (S.iter < S.maxIter) && (sqrt(rrn) > S.stop || (EVEN && (S.iter & 1)));
     if (!run) {
         S.done = 1;
         if (!MIXED) S.converged = sqrt(rrn) <= S.stop ? 1 : 0;
@@ -633,9 +637,9 @@ static int site_blocks(const Ctx &c) { return (c.V + kNT - 1) / kNT; }
 
 int cg_alloc(Ctx &c) {
     const size_t n = (size_t)c.nSys * 4 * c.V;
-    // partial slots: the smallest (64-site) blocks, or one full-lattice plus two
-    // half-lattice site grids of 256-thread blocks (even-odd setup), whichever is larger
-    c.maxBlocks = (c.V + 63) / 64 + 4;
+    // partial slots: two per block of the smallest (64-site) tiles, which also covers one
+    // full-lattice plus two half-lattice site grids of 256-thread blocks (even-odd setup)
+    c.maxBlocks = (c.V + 31) / 32 + 4;
     double2 **bufs[] = {&c.eta, &c.x, &c.r, &c.pv[0], &c.pv[1], &c.t, &c.Ap};
     for (double2 **b : bufs) {
         YCN_HIP(hipMalloc(b, sizeof(double2) * n));
@@ -731,10 +735,13 @@ void launch_fin_alpha(Ctx &c, int nsys, int nb) {
     hipLaunchKernelGGL(k_fin_alpha, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb, c.cg_d);
 }
 
-void launch_fin_beta(Ctx &c, int nsys, int nb, bool mixed) {
+void launch_fin_beta(Ctx &c, int nsys, int nb, bool mixed, bool even) {
     if (mixed)
         hipLaunchKernelGGL(k_fin_beta<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
                            c.cg_d, c.active);
+    else if (even)
+        hipLaunchKernelGGL((k_fin_beta<false, true>), dim3(nsys), dim3(kNT), 0, c.stream,
+                           (const double2 *)c.partials, nb, c.cg_d, c.active);
     else
         hipLaunchKernelGGL(k_fin_beta<false>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
                            c.cg_d, c.active);
@@ -952,11 +959,19 @@ int mp_alloc(Ctx &c) {
     }
     YCN_HIP(hipMalloc(&c.phis, sizeof(float) * (size_t)c.nRep * c.V));
     YCN_HIP(hipMemset(c.phis, 0, sizeof(float) * (size_t)c.nRep * c.V));
+    if (c.p.evenOdd) {   // second r and Ap buffers of the pipelined even-odd iteration (half lattice)
+        const size_t h = (size_t)c.nSys * 4 * (c.V / 2);
+        float2 **pf[] = {&c.pfR1, &c.pfA1};
+        for (float2 **b : pf) {
+            YCN_HIP(hipMalloc(b, sizeof(float2) * h));
+            YCN_HIP(hipMemset(*b, 0, sizeof(float2) * h));
+        }
+    }
     return YCN_OK;
 }
 
 void mp_free(Ctx &c) {
-    float2 **bufs[] = {&c.rs, &c.ps[0], &c.ps[1], &c.Aps, &c.xs};
+    float2 **bufs[] = {&c.rs, &c.ps[0], &c.ps[1], &c.Aps, &c.xs, &c.pfR1, &c.pfA1};
     for (float2 **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (c.phis) { (void)hipFree(c.phis); c.phis = nullptr; }

@@ -316,12 +316,268 @@ k_fin_eo_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
     }
 }
 
-// mixed precision reliable update: partRes = |b_e - S x_e|^2, partN = |S^T (b_e - S x_e)|^2
-// (the new FP64 normal residual).  Converged on the system residual; otherwise the FP32
-// loop resumes with the direction kept (beta = |r_true|^2 / |r_prev|^2) and, if its
-// normal-residual threshold was already met, a threshold lowered by the missing factor.
+// ---------------------------------------------------------------------------
+// fused Schur normal operator: p' = r + beta p_old; v = S p'; Ap = S^T v; |v|^2
+// ---------------------------------------------------------------------------
+// One block owns a TT x TX tile of the full lattice (TT, TX even) and produces Ap on the
+// tile's TT*TX/2 even sites.  LDS holds the tile plus a 4-site halo, (TT+8) x (TX+8)
+// sites in full-lattice coordinates, split by lattice parity: E[c][row][k] and
+// O[c][row][k] hold the site of that parity in array row `row`, column 2k or 2k+1
+// (k = column >> 1), four SoA planes each.  Lanes of a wave take consecutive k, so every
+// neighbour read (rows row +- 1 at the same k, the same row at k or k +- 1) is a
+// contiguous run: no bank conflicts.  Each stage fills the parity it produces on a halo
+// one site smaller than the stage before:
+//   0: p'  on even sites, 4-site halo (from HBM/L2; p' of the tile also written out)
+//   1: U = K p' / d on odd sites, 3-site halo          (= A_o^-1 K_oe p')
+//   2: v = d p' - K U on even sites, 2-site halo        (= S p', overwrites p')
+//   3: W = H v / d on odd sites, 1-site halo            (overwrites U)
+//   4: Ap = d v - H W on the tile's even sites          (= S^T S p')
+// The diagonal d = 2r + M + g phi is staged per parity for the whole array.  |v|^2 over
+// the tile (= <p', Ap>) is the block's partial.  HBM traffic per launch: read r, p_old
+// (Vh sites each) and phi, write p', Ap: 4 S/2 + P per site of the full lattice.
+constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave : tt * tx / 2; }
+
+// PF = false: the operator alone, p' = r + beta p_old, partial <p', Ap> = |v|^2.
+// PF = true, the pipelined FP32 iteration with one reduction.  CG with the residual-norm
+// recurrence of D'Azevedo, Eijkhout and Romine (LAPACK Working Note 56, 1993): with
+// rho = |r|^2, mu = <p, Ap>, nu = |Ap|^2, alpha_k = rho_k / mu_k and <r_k, Ap_k> = mu_k,
+// rho_{k+1} = alpha_k^2 nu_k - rho_k, so beta_{k+1} is known before r_{k+1} exists and
+// the x, r updates move into the next operator application.  Stage 0 then forms, on its
+// even sites, r' = r - alpha Ap, p' = r' + beta p and (tile only) x += alpha p; the block
+// partials are (mu, nu) and (rho', 0), rho' the exact |r'|^2 that re-anchors the
+// recurrence every iteration.  r, Ap and p ping-pong between two buffers: blocks read the
+// old ones on their halo while other blocks write the new ones on their tiles.
+template <typename R, int TT, int TX, bool PF>
+__global__ void __launch_bounds__(eo_threads(TT, TX))
+k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict__ st,
+                  const typename Vec2<R>::T *__restrict__ r, typename Vec2<R>::T *__restrict__ r_out,
+                  const typename Vec2<R>::T *__restrict__ Ap_in, const typename Vec2<R>::T *__restrict__ pold,
+                  typename Vec2<R>::T *__restrict__ pnew, typename Vec2<R>::T *__restrict__ Ap,
+                  typename Vec2<R>::T *__restrict__ xs, double2 *__restrict__ part) {
+    using V2 = typename Vec2<R>::T;
+    constexpr int H = TT + 8, K = (TX + 8) / 2, NT = eo_threads(TT, TX);
+    __shared__ V2 E[4][H * K];
+    __shared__ V2 O[4][H * K];
+    __shared__ R FE[H * K], FO[H * K];
+    __shared__ double2 red[NT / kWave > 0 ? NT / kWave : 1];
+    __shared__ double2 red2[NT / kWave > 0 ? NT / kWave : 1];
+
+    const int sys = blockIdx.y;
+    if (st[sys].done) return;
+    const int nb = gridDim.x;
+    int tile = blockIdx.x;
+    if (A.xcdRemap) tile = (tile % 8) * (nb / 8) + tile / 8;
+    const int ty = tile / A.tilesX, tx = tile - ty * A.tilesX;
+    const int t0 = ty * TT, x0 = tx * TX;
+    const int Nt = A.Nt, Nx = A.Nx, Nh = A.Nh;
+    const size_t Vh = A.Vh, off = (size_t)sys * 4 * Vh;
+    const R beta = (R)st[sys].beta;
+    const R alpha = PF ? (R)st[sys].alpha.x : (R)0;
+    const R *ph = phi + (size_t)(sys / A.nNoise) * A.V;
+    const R hp = (R)(0.5 * (A.r + 1.0)), hm = (R)(0.5 * (A.r - 1.0)), hr = (R)(0.5 * A.r), h = (R)0.5;
+    const R d0 = (R)(2.0 * A.r + A.M), g = (R)A.g;
+
+    // stage 0: p' on the even sites of the whole array, and d on both parities.
+    // Array (row, col) is lattice site (t0 - 4 + row, x0 - 4 + col); its parity is
+    // (row + col) & 1 because t0 + x0 is even.
+    double rho = 0.0;
+    for (int w = threadIdx.x; w < H * K; w += NT) {
+        const int row = w / K, k = w - row * K;
+        const int gt = wrapm(t0 - 4 + row, Nt);
+        const int colE = 2 * k + (row & 1), colO = 2 * k + 1 - (row & 1);
+        const int gxE = wrapm(x0 - 4 + colE, Nx), gxO = wrapm(x0 - 4 + colO, Nx);
+        const size_t j = (size_t)gt * Nh + (gxE >> 1);
+        const bool inner = row >= 4 && row < 4 + TT && colE >= 4 && colE < 4 + TX;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const size_t i = off + c * Vh + j;
+            V2 a = r[i];
+            const V2 b = pold[i];
+            if (PF) {
+                const V2 ap = Ap_in[i];
+                a = mk2<R>(a.x - alpha * ap.x, a.y - alpha * ap.y);
+            }
+            const V2 pv = mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
+            E[c][w] = pv;
+            if (inner) {
+                pnew[i] = pv;
+                if (PF) {
+                    r_out[i] = a;
+                    const V2 xv = xs[i];
+                    xs[i] = mk2<R>(xv.x + alpha * b.x, xv.y + alpha * b.y);
+                    rho += (double)(a.x * a.x + a.y * a.y);
+                }
+            }
+        }
+        FE[w] = d0 + g * ph[(size_t)gt * Nx + gxE];
+        FO[w] = d0 + g * ph[(size_t)gt * Nx + gxO];
+    }
+    __syncthreads();
+
+    // the sites of parity `par` in rows/cols [4 - hh, 4 + T + hh): w -> (row, k) and the
+    // array indices of the site and its four neighbours (other parity)
+    struct Site {
+        int row, col, i, itf, itb, ixf, ixb;
+    };
+    auto site_of = [](int w, int hh, int par) {
+        constexpr int Kw = K;
+        const int c0 = 4 - hh, half = (TX + 2 * hh) / 2;
+        Site q;
+        q.row = c0 + w / half;
+        q.col = c0 + 2 * (w - (q.row - c0) * half) + (((par + q.row) ^ c0) & 1);
+        const int k = q.col >> 1;
+        q.i = q.row * Kw + k;
+        q.itf = q.i + Kw;
+        q.itb = q.i - Kw;
+        q.ixf = q.row * Kw + ((q.col + 1) >> 1);
+        q.ixb = q.row * Kw + ((q.col - 1) >> 1);
+        return q;
+    };
+    // hopping at site q from the other-parity array N
+    auto hop_at = [&](auto dag, const Site &q, const V2 (*N)[H * K], V2 o[4]) {
+        constexpr bool DAG = decltype(dag)::value;
+        const int gt = wrapm(t0 - 4 + q.row, Nt);
+        const R sf = (gt == Nt - 1) ? (R)-1 : (R)1, sb = (gt == 0) ? (R)-1 : (R)1;
+#pragma unroll
+        for (int f = 0; f < 4; f += 2)
+            hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, N[f][q.itf], N[f + 1][q.itf], sf, N[f][q.itb], N[f + 1][q.itb],
+                                      sb, N[f][q.ixf], N[f + 1][q.ixf], N[f][q.ixb], N[f + 1][q.ixb], o[f], o[f + 1]);
+    };
+
+    // stage 1: U = K p' / d on odd sites, 3-site halo
+    for (int w = threadIdx.x; w < (TT + 6) * ((TX + 6) / 2); w += NT) {
+        const Site q = site_of(w, 3, 1);
+        V2 o[4];
+        hop_at(std::integral_constant<bool, true>(), q, E, o);
+        const R inv = (R)1 / FO[q.i];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) O[c][q.i] = mk2<R>(o[c].x * inv, o[c].y * inv);
+    }
+    __syncthreads();
+
+    // stage 2: v = d p' - K U on even sites, 2-site halo; |v|^2 over the tile
+    double acc = 0.0;
+    for (int w = threadIdx.x; w < (TT + 4) * ((TX + 4) / 2); w += NT) {
+        const Site q = site_of(w, 2, 0);
+        V2 o[4];
+        hop_at(std::integral_constant<bool, true>(), q, O, o);
+        const R d = FE[q.i];
+        const bool inner = q.row >= 4 && q.row < 4 + TT && q.col >= 4 && q.col < 4 + TX;
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const V2 pv = E[c][q.i];
+            const V2 v = mk2<R>(d * pv.x - o[c].x, d * pv.y - o[c].y);
+            E[c][q.i] = v;
+            if (inner) acc += (double)(v.x * v.x + v.y * v.y);
+        }
+    }
+    __syncthreads();
+
+    // stage 3: W = H v / d on odd sites, 1-site halo
+    for (int w = threadIdx.x; w < (TT + 2) * ((TX + 2) / 2); w += NT) {
+        const Site q = site_of(w, 1, 1);
+        V2 o[4];
+        hop_at(std::integral_constant<bool, false>(), q, E, o);
+        const R inv = (R)1 / FO[q.i];
+#pragma unroll
+        for (int c = 0; c < 4; ++c) O[c][q.i] = mk2<R>(o[c].x * inv, o[c].y * inv);
+    }
+    __syncthreads();
+
+    // stage 4: Ap = d v - H W on the tile's even sites (PF: and |Ap|^2)
+    double nu = 0.0;
+    for (int w = threadIdx.x; w < TT * (TX / 2); w += NT) {
+        const Site q = site_of(w, 0, 0);
+        V2 o[4];
+        hop_at(std::integral_constant<bool, false>(), q, O, o);
+        const R d = FE[q.i];
+        const int gt = wrapm(t0 - 4 + q.row, Nt), gx = wrapm(x0 - 4 + q.col, Nx);
+        const size_t j = (size_t)gt * Nh + (gx >> 1);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const V2 v = E[c][q.i];
+            const V2 a = mk2<R>(d * v.x - o[c].x, d * v.y - o[c].y);
+            Ap[off + c * Vh + j] = a;
+            if (PF) nu += (double)(a.x * a.x + a.y * a.y);
+        }
+    }
+    if (PF) {
+        const double2 t1 = block_sum2<NT>(make_double2(acc, nu), red);
+        const double2 t2 = block_sum2<NT>(make_double2(rho, 0.0), red2);
+        if (threadIdx.x == 0) {
+            part[2 * ((size_t)sys * nb + blockIdx.x)] = t1;
+            part[2 * ((size_t)sys * nb + blockIdx.x) + 1] = t2;
+        }
+    } else {
+        const double2 tot = block_sum2<NT>(make_double2(acc, 0.0), red);
+        if (threadIdx.x == 0) part[(size_t)sys * nb + blockIdx.x] = tot;
+    }
+}
+
+// pipelined iteration: from (mu, nu) and the exact rho of the iteration just done,
+// alpha = rho / mu and beta = (alpha^2 nu - rho) / rho for the next one (a negative
+// prediction, from cancellation, restarts the direction).  Pauses at the stop after an
+// even number of iterations (the last direction applied is then in p[1]), at the cap, or
+// on a non-positive <p, Ap>; a reliable update follows.
 __global__ void __launch_bounds__(kNT)
-k_fin_eo_mp_true(const double2 *__restrict__ partRes, const double2 *__restrict__ partN, int nb,
+k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active) {
+    __shared__ double2 lds0[kNT / kWave], lds1[kNT / kWave];
+    const int sys = blockIdx.x;
+    const int skip = st[sys].done;
+    const double2 *pp = part + 2 * (size_t)sys * nb;
+    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+    for (int i = threadIdx.x; i < nb; i += kNT) {
+        a0 = cadd(a0, pp[2 * i]);
+        a1 = cadd(a1, pp[2 * i + 1]);
+    }
+    const double2 mn = block_sum2<kNT>(a0, lds0);
+    const double2 rh = block_sum2<kNT>(a1, lds1);
+    if (skip || threadIdx.x) return;
+    CgState &S = st[sys];
+    const double mu = mn.x, nu = mn.y, rho = rh.x;
+    S.iter += 1;
+    S.rr_prev = S.rr;
+    S.rr = rho;
+    const bool stop = (S.iter >= S.maxIter) || (sqrt(rho) <= S.stop && !(S.iter & 1)) ||
+                      (!(mu > 0.0) && !(S.iter & 1)) || !isfinite(mu) || !isfinite(rho);
+    if (stop) {
+        S.done = 1;
+        atomicSub(active, 1);
+        return;
+    }
+    const double al = (mu > 0.0) ? rho / mu : 0.0;
+    double rhoNext = al * al * nu - rho;
+    if (!(rhoNext > 0.0)) rhoNext = 0.0;
+    S.alpha = make_double2(al, 0.0);
+    S.beta = (rho > 0.0) ? rhoNext / rho : 0.0;
+}
+
+// reliable update of the pipelined iteration: x += x_s (FP64), x_s = 0, and the last
+// direction applied (in p[1]) copied to p[0], where the next iteration reads it
+__global__ void __launch_bounds__(kNT)
+k_pf_reliable(int n, const CgState *__restrict__ st, double2 *__restrict__ x, float2 *__restrict__ xs,
+              const float2 *__restrict__ p1, float2 *__restrict__ p0) {
+    const int sys = blockIdx.y;
+    if (st[sys].final_) return;
+    const int s = blockIdx.x * kNT + threadIdx.x;
+    if (s >= n) return;
+    const size_t off = (size_t)sys * 4 * n;
+#pragma unroll
+    for (int c = 0; c < 4; ++c) {
+        const size_t i = off + (size_t)c * n + s;
+        const float2 d = xs[i];
+        const double2 xv = x[i];
+        x[i] = make_double2(xv.x + (double)d.x, xv.y + (double)d.y);
+        xs[i] = make_float2(0.f, 0.f);
+        p0[i] = p1[i];
+    }
+}
+
+// after the reliable update of the pipelined iteration: partRes = |b_e - S x_e|^2 and
+// partN = |S^T (b_e - S x_e)|^2 (the new normal residual).  Converged on the system residual, or resume with alpha = 0 (r already is the true
+// residual) and beta = |r_true|^2 / |r_prev|^2 (the direction is kept)
+__global__ void __launch_bounds__(kNT)
+k_fin_eo_pf_true(const double2 *__restrict__ partRes, const double2 *__restrict__ partN, int nb,
                  CgState *__restrict__ st, int *__restrict__ active) {
     __shared__ double2 lds0[kNT / kWave], lds1[kNT / kWave];
     const int sys = blockIdx.x;
@@ -342,157 +598,14 @@ k_fin_eo_mp_true(const double2 *__restrict__ partRes, const double2 *__restrict_
         }
         return;
     }
-    S.beta = rn.x / S.rr_prev;
+    S.alpha = make_double2(0.0, 0.0);
+    S.beta = (S.rr_prev > 0.0) ? rn.x / S.rr_prev : 0.0;
     S.rr = rn.x;
     if (S.mode == YCN_TOL_REL_TRUE && sqrt(rn.x) <= S.stop) S.stop = sqrt(rn.x) * 0.5 * S.tol / S.true_rel;
     if (S.done) {
         S.done = 0;
         atomicAdd(active, 1);
     }
-}
-
-// ---------------------------------------------------------------------------
-// fused Schur normal operator: p' = r + beta p_old; v = S p'; Ap = S^T v; |v|^2
-// ---------------------------------------------------------------------------
-// One block owns a TT x TX tile of the full lattice (TT, TX even) and produces Ap on the
-// tile's TT*TX/2 even sites.  LDS holds one (TT+8) x (TX+8) array of four SoA planes in
-// full-lattice coordinates; each stage fills the parity it produces, on a halo one site
-// smaller than the stage before:
-//   0: p'  on even sites, 4-site halo (from HBM/L2; p' of the tile also written out)
-//   1: U = K p' / d on odd sites, 3-site halo          (= A_o^-1 K_oe p')
-//   2: v = d p' - K U on even sites, 2-site halo        (= S p', overwrites p')
-//   3: W = H v / d on odd sites, 1-site halo            (overwrites U)
-//   4: Ap = d v - H W on the tile's even sites          (= S^T S p')
-// The diagonal d = 2r + M + g phi is staged for the 3-site halo.  |v|^2 over the tile
-// (= <p', Ap>) is the block's partial.  HBM traffic per launch: read r, p_old (Vh sites
-// each) and phi, write p', Ap: 4 S/2 + P per site of the full lattice.
-constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave : tt * tx / 2; }
-
-template <typename R, int TT, int TX>
-__global__ void __launch_bounds__(eo_threads(TT, TX))
-k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict__ st,
-                  const typename Vec2<R>::T *__restrict__ r, const typename Vec2<R>::T *__restrict__ pold,
-                  typename Vec2<R>::T *__restrict__ pnew, typename Vec2<R>::T *__restrict__ Ap,
-                  double2 *__restrict__ part) {
-    using V2 = typename Vec2<R>::T;
-    constexpr int H = TT + 8, W = TX + 8, HF = TT + 6, WF = TX + 6, NT = eo_threads(TT, TX);
-    __shared__ V2 P[4][H * W];
-    __shared__ R F[HF * WF];
-    __shared__ double2 red[NT / kWave > 0 ? NT / kWave : 1];
-
-    const int sys = blockIdx.y;
-    if (st[sys].done) return;
-    const int nb = gridDim.x;
-    int tile = blockIdx.x;
-    if (A.xcdRemap) tile = (tile % 8) * (nb / 8) + tile / 8;
-    const int ty = tile / A.tilesX, tx = tile - ty * A.tilesX;
-    const int t0 = ty * TT, x0 = tx * TX;
-    const int Nt = A.Nt, Nx = A.Nx, Nh = A.Nh;
-    const size_t Vh = A.Vh, off = (size_t)sys * 4 * Vh;
-    const R beta = (R)st[sys].beta;
-    const R *ph = phi + (size_t)(sys / A.nNoise) * A.V;
-    const R hp = (R)(0.5 * (A.r + 1.0)), hm = (R)(0.5 * (A.r - 1.0)), hr = (R)(0.5 * A.r), h = (R)0.5;
-    const R d0 = (R)(2.0 * A.r + A.M), g = (R)A.g;
-
-    // stage 0: p' on the even sites of the whole array
-    for (int w = threadIdx.x; w < H * (W / 2); w += NT) {
-        const int lr = w / (W / 2), lc = 2 * (w - lr * (W / 2)) + (lr & 1);
-        const int gt = wrapm(t0 - 4 + lr, Nt), gx = wrapm(x0 - 4 + lc, Nx);
-        const size_t j = (size_t)gt * Nh + (gx >> 1);
-        const bool inner = lr >= 4 && lr < 4 + TT && lc >= 4 && lc < 4 + TX;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const V2 a = r[off + c * Vh + j], b = pold[off + c * Vh + j];
-            const V2 pv = mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
-            P[c][lr * W + lc] = pv;
-            if (inner) pnew[off + c * Vh + j] = pv;
-        }
-    }
-    for (int w = threadIdx.x; w < HF * WF; w += NT) {
-        const int lr = w / WF, lc = w - lr * WF;
-        const int gt = wrapm(t0 - 3 + lr, Nt), gx = wrapm(x0 - 3 + lc, Nx);
-        F[w] = d0 + g * ph[(size_t)gt * Nx + gx];
-    }
-    __syncthreads();
-
-    // the sites of parity `par` in rows/cols [4 - hh, 4 + T + hh) of the array
-    auto site_of = [](int w, int hh, int par, int &lr, int &lc) {
-        const int c0 = 4 - hh, half = (TX + 2 * hh) / 2;
-        lr = c0 + w / half;
-        lc = c0 + 2 * (w - (lr - c0) * half) + (((par + lr) ^ c0) & 1);
-    };
-    auto hop_at = [&](auto dag, int lr, int lc, V2 o[4]) {
-        constexpr bool DAG = decltype(dag)::value;
-        const int gt = wrapm(t0 - 4 + lr, Nt);
-        const R sf = (gt == Nt - 1) ? (R)-1 : (R)1, sb = (gt == 0) ? (R)-1 : (R)1;
-        const int i = lr * W + lc;
-#pragma unroll
-        for (int f = 0; f < 4; f += 2)
-            hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, P[f][i + W], P[f + 1][i + W], sf, P[f][i - W], P[f + 1][i - W],
-                                      sb, P[f][i + 1], P[f + 1][i + 1], P[f][i - 1], P[f + 1][i - 1], o[f], o[f + 1]);
-    };
-    auto dg = [&](int lr, int lc) { return F[(lr - 1) * WF + (lc - 1)]; };
-
-    // stage 1: U = K p' / d on odd sites, 3-site halo
-    for (int w = threadIdx.x; w < (TT + 6) * ((TX + 6) / 2); w += NT) {
-        int lr, lc;
-        site_of(w, 3, 1, lr, lc);
-        V2 o[4];
-        hop_at(std::integral_constant<bool, true>(), lr, lc, o);
-        const R inv = (R)1 / dg(lr, lc);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) P[c][lr * W + lc] = mk2<R>(o[c].x * inv, o[c].y * inv);
-    }
-    __syncthreads();
-
-    // stage 2: v = d p' - K U on even sites, 2-site halo; |v|^2 over the tile
-    double acc = 0.0;
-    for (int w = threadIdx.x; w < (TT + 4) * ((TX + 4) / 2); w += NT) {
-        int lr, lc;
-        site_of(w, 2, 0, lr, lc);
-        V2 o[4];
-        hop_at(std::integral_constant<bool, true>(), lr, lc, o);
-        const R d = dg(lr, lc);
-        const bool inner = lr >= 4 && lr < 4 + TT && lc >= 4 && lc < 4 + TX;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const V2 pv = P[c][lr * W + lc];
-            const V2 v = mk2<R>(d * pv.x - o[c].x, d * pv.y - o[c].y);
-            P[c][lr * W + lc] = v;
-            if (inner) acc += (double)(v.x * v.x + v.y * v.y);
-        }
-    }
-    __syncthreads();
-
-    // stage 3: W = H v / d on odd sites, 1-site halo
-    for (int w = threadIdx.x; w < (TT + 2) * ((TX + 2) / 2); w += NT) {
-        int lr, lc;
-        site_of(w, 1, 1, lr, lc);
-        V2 o[4];
-        hop_at(std::integral_constant<bool, false>(), lr, lc, o);
-        const R inv = (R)1 / dg(lr, lc);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) P[c][lr * W + lc] = mk2<R>(o[c].x * inv, o[c].y * inv);
-    }
-    __syncthreads();
-
-    // stage 4: Ap = d v - H W on the tile's even sites
-    for (int w = threadIdx.x; w < TT * (TX / 2); w += NT) {
-        int lr, lc;
-        site_of(w, 0, 0, lr, lc);
-        V2 o[4];
-        hop_at(std::integral_constant<bool, false>(), lr, lc, o);
-        const R d = dg(lr, lc);
-        const int gt = wrapm(t0 - 4 + lr, Nt), gx = wrapm(x0 - 4 + lc, Nx);
-        const size_t j = (size_t)gt * Nh + (gx >> 1);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const V2 v = P[c][lr * W + lc];
-            Ap[off + c * Vh + j] = mk2<R>(d * v.x - o[c].x, d * v.y - o[c].y);
-        }
-    }
-    double2 tot = block_sum2<NT>(make_double2(acc, 0.0), red);
-    if (threadIdx.x == 0) part[(size_t)sys * nb + blockIdx.x] = tot;
 }
 
 // ---------------------------------------------------------------------------
@@ -525,43 +638,74 @@ static bool eo_fits(const Ctx &c, TileShape t) { return c.p.Nt % t.TT == 0 && c.
 
 bool eo_available(const Ctx &c) { return c.p.Nt % 2 == 0 && c.p.Nx % 2 == 0 && c.tileEo64 >= 0; }
 
+// buffers of one fused launch (PF: the pipelined iteration's ping-pong pairs and x_s)
+template <typename R>
+struct EoPtrs {
+    using V2 = typename Vec2<R>::T;
+    const V2 *r = nullptr;
+    V2 *r_out = nullptr;
+    const V2 *Ap_in = nullptr;
+    const V2 *pold = nullptr;
+    V2 *pnew = nullptr, *Ap = nullptr, *xs = nullptr;
+};
+
 template <typename R, int TT, int TX>
-static void launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, const CgState *st,
-                         const typename Vec2<R>::T *r, const typename Vec2<R>::T *pold, typename Vec2<R>::T *pnew,
-                         typename Vec2<R>::T *Ap, double2 *part) {
-    hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX>), dim3(A.tilesPerSys, nsys), dim3(eo_threads(TT, TX)), 0,
-                       c.stream, A, phi, st, r, pold, pnew, Ap, part);
+static void launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, bool pf, const EoPtrs<R> &b) {
+    const dim3 grid(A.tilesPerSys, nsys), block(eo_threads(TT, TX));
+    if constexpr (std::is_same<R, float>::value) {
+        if (pf) {
+            hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, true>), grid, block, 0, c.stream, A, phi, c.cg_d, b.r,
+                               b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials);
+            return;
+        }
+    }
+    hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), grid, block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out,
+                       b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials);
 }
 
-// launch the fused Schur normal operator with tile `ti`; returns the partial count
+// launch the fused Schur normal operator (FP64) with tile `ti`; returns the partial count
 static int launch_eo_fused(Ctx &c, int ti, int nsys, const double *phi, const double2 *r, const double2 *pold,
                            double2 *pnew, double2 *Ap) {
     const TileShape t = kEoTiles64[ti];
     const EoArgs A = eo_args(c, t.TT, t.TX);
+    EoPtrs<double> b;
+    b.r = r;
+    b.pold = pold;
+    b.pnew = pnew;
+    b.Ap = Ap;
     prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (2 * 64 + 8));
     switch (ti) {
-    case 0: launch_eo_tt<double, 8, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
-    case 1: launch_eo_tt<double, 16, 16>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
-    case 2: launch_eo_tt<double, 4, 64>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
-    case 3: launch_eo_tt<double, 4, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
-    default: launch_eo_tt<double, 2, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    case 0: launch_eo_tt<double, 8, 32>(c, nsys, A, phi, false, b); break;
+    case 1: launch_eo_tt<double, 16, 16>(c, nsys, A, phi, false, b); break;
+    case 2: launch_eo_tt<double, 4, 64>(c, nsys, A, phi, false, b); break;
+    case 3: launch_eo_tt<double, 4, 32>(c, nsys, A, phi, false, b); break;
+    default: launch_eo_tt<double, 2, 32>(c, nsys, A, phi, false, b); break;
     }
     prof_after(c, PK_NORMAL_OP);
     return A.tilesPerSys;
 }
 
-static int launch_eo_fused32(Ctx &c, int ti, int nsys, const float *phi, const float2 *r, const float2 *pold,
-                             float2 *pnew, float2 *Ap) {
+// the pipelined FP32 iteration's fused kernel with tile `ti` and ping-pong parity `par`
+static int launch_eo_pf(Ctx &c, int ti, int nsys, int par) {
     const TileShape t = kEoTiles32[ti];
     const EoArgs A = eo_args(c, t.TT, t.TX);
-    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (2 * 32 + 4));
+    EoPtrs<float> b;
+    b.r = par ? c.pfR1 : c.rs;
+    b.r_out = par ? c.rs : c.pfR1;
+    b.Ap_in = par ? c.pfA1 : c.Aps;
+    b.Ap = par ? c.Aps : c.pfA1;
+    b.pold = c.ps[par];
+    b.pnew = c.ps[par ^ 1];
+    b.xs = c.xs;
+    // read r, Ap, p, x_s and write r, p, Ap, x_s (half-lattice FP32 vectors) and read phi
+    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (8 * 16 + 4));
     switch (ti) {
-    case 0: launch_eo_tt<float, 8, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
-    case 1: launch_eo_tt<float, 16, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
-    case 2: launch_eo_tt<float, 8, 64>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
-    case 3: launch_eo_tt<float, 16, 16>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
-    case 4: launch_eo_tt<float, 4, 64>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
-    default: launch_eo_tt<float, 2, 32>(c, nsys, A, phi, c.cg_d, r, pold, pnew, Ap, c.partials); break;
+    case 0: launch_eo_tt<float, 8, 32>(c, nsys, A, c.phis, true, b); break;
+    case 1: launch_eo_tt<float, 16, 32>(c, nsys, A, c.phis, true, b); break;
+    case 2: launch_eo_tt<float, 8, 64>(c, nsys, A, c.phis, true, b); break;
+    case 3: launch_eo_tt<float, 16, 16>(c, nsys, A, c.phis, true, b); break;
+    case 4: launch_eo_tt<float, 4, 64>(c, nsys, A, c.phis, true, b); break;
+    default: launch_eo_tt<float, 2, 32>(c, nsys, A, c.phis, true, b); break;
     }
     prof_after(c, PK_NORMAL_OP);
     return A.tilesPerSys;
@@ -602,14 +746,12 @@ int eo_autotune(Ctx &c) {
         c.tuneReport += line;
         if (us < best) { best = us; c.tileEo64 = i; }
     }
-    if (c.rs && c.phis) {
-        c.tuneReport += " | even-odd fp32 tiles (us/launch):";
+    if (c.rs && c.phis && c.pfR1 && c.pfA1) {
+        c.tuneReport += " | even-odd fp32 pipelined tiles (us/launch):";
         best = 1e30;
         for (int i = 0; i < kNumEo32; ++i) {
             if (!eo_fits(c, kEoTiles32[i])) continue;
-            const double us = time_it([&](int par) {
-                (void)launch_eo_fused32(c, i, c.nSys, c.phis, c.rs, c.ps[par], c.ps[par ^ 1], c.Aps);
-            });
+            const double us = time_it([&](int par) { (void)launch_eo_pf(c, i, c.nSys, par); });
             std::snprintf(line, sizeof line, " %dx%d %.1f;", kEoTiles32[i].TT, kEoTiles32[i].TX, us);
             c.tuneReport += line;
             if (us < best) { best = us; c.tileEo32 = i; }
@@ -623,7 +765,7 @@ int eo_autotune(Ctx &c) {
     }
     if (const char *f = std::getenv("YCN_TILE_EO32")) {
         const int i = std::atoi(f);
-        if (c.rs && c.phis && i >= 0 && i < kNumEo32 && eo_fits(c, kEoTiles32[i])) c.tileEo32 = i;
+        if (c.rs && c.phis && c.pfR1 && i >= 0 && i < kNumEo32 && eo_fits(c, kEoTiles32[i])) c.tileEo32 = i;
         c.tuneReport += " (even-odd fp32 tile forced by YCN_TILE_EO32)";
     }
     (void)hipEventDestroy(e0);
@@ -647,14 +789,14 @@ static void launch_eo_iteration(Ctx &c, int nsys, const double *phi, int parity)
     const int nb = launch_eo_fused(c, c.tileEo64, nsys, phi, c.r, c.pv[parity], c.pv[parity ^ 1], c.Ap);
     launch_fin_alpha(c, nsys, nb);
     const int nbu = launch_cg_update(c, nsys, c.V / 2, c.pv[parity ^ 1]);
-    launch_fin_beta(c, nsys, nbu, false);
+    launch_fin_beta(c, nsys, nbu, false, true);
 }
 
-static void launch_eo_mp_iteration(Ctx &c, int nsys, int parity) {
-    const int nb = launch_eo_fused32(c, c.tileEo32, nsys, c.phis, c.rs, c.ps[parity], c.ps[parity ^ 1], c.Aps);
-    launch_fin_alpha(c, nsys, nb);
-    const int nbu = launch_mp_update(c, nsys, c.V / 2, c.ps[parity ^ 1]);
-    launch_fin_beta(c, nsys, nbu, true);
+// one pipelined FP32 iteration: fused kernel + one finalize
+static void launch_eo_pf_iteration(Ctx &c, int nsys, int parity) {
+    const int nb = launch_eo_pf(c, c.tileEo32, nsys, parity);
+    hipLaunchKernelGGL(k_fin_pf, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb, c.cg_d,
+                       c.active);
 }
 
 // Solve S chi_e = b_e for every system (eta from the Philox streams when gen); chi_e is
@@ -665,7 +807,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         c.err = "even-odd preconditioning needs even Nt and Nx and a fused tile that divides the lattice";
         return YCN_ERR_UNSUPPORTED;
     }
-    const bool mixed = c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis;
+    const bool mixed = c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfR1 && c.pfA1;
     const EoArgs A = eo_args(c, 0, 0);
     const int Vh = c.V / 2;
     const dim3 gF((c.V + kNT - 1) / kNT, nsys), gH = half_grid(c, nsys);
@@ -691,7 +833,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
                        c.pv[1]);
     if (mixed) {
         hipLaunchKernelGGL(k_eo_zero32, dim3((unsigned)((nh + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, nh, c.xs,
-                           c.ps[0], c.ps[1]);
+                           c.ps[0], c.Aps);
         const int nrepUsed = (nsys + c.nNoise - 1) / c.nNoise;
         rc = to_f32_n(c, phi, c.phis, (int64_t)std::min(nrepUsed, c.nRep) * c.V);
         if (rc) return rc;
@@ -709,19 +851,20 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         if (rc) return rc;
         while (active > 0 && launched < hardCap) {
             rc = run_chunk(c, phi, nsys, YCN_DAGGER, nIt, 4, c.tileEo32,
-                           [&](int par) { launch_eo_mp_iteration(c, nsys, par); });
+                           [&](int par) { launch_eo_pf_iteration(c, nsys, par); });
             if (rc) return rc;
             launched += nIt;
-            // reliable update: x += x_s; res = b - S x; r = S^T res (FP64, FP32 copy)
-            rc = launch_mp_accum(c, nsys, Vh);
-            if (rc) return rc;
+            // reliable update: x += x_s, p[0] = p[1]; res = b - S x; r = S^T res (FP64, FP32
+            // copy into r[0])
+            hipLaunchKernelGGL(k_pf_reliable, gH, dim3(kNT), 0, c.stream, Vh, (const CgState *)c.cg_d, c.x, c.xs,
+                               (const float2 *)c.ps[1], c.ps[0]);
             hipLaunchKernelGGL(k_eo_hop_oe<true>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x, B.U);
             hipLaunchKernelGGL((k_eo_hop_eo<true, 1>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x,
                                (const double2 *)B.U, (const double2 *)B.b, B.res, (float2 *)nullptr, partH);
             hipLaunchKernelGGL(k_eo_hop_oe<false>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.res, B.U);
             hipLaunchKernelGGL((k_eo_hop_eo<false, 2>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.res,
                                (const double2 *)B.U, (const double2 *)nullptr, c.r, c.rs, partH2);
-            hipLaunchKernelGGL(k_fin_eo_mp_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partH,
+            hipLaunchKernelGGL(k_fin_eo_pf_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partH,
                                (const double2 *)partH2, nbH, c.cg_d, c.active);
             YCN_HIP(hipGetLastError());
             rc = poll_active(c, active);

@@ -595,14 +595,22 @@ int ycn_profile_end(ycn_ctx *ctx, double *total_ms, int64_t *launches, double *b
     CHECK_CTX(ctx);
     Ctx &c = *ctx;
     YCN_HIP(hipStreamSynchronize(c.stream));
+    // Launches replayed after every system converged exit at once (a few us against tens
+    // for a working launch); those shorter than a quarter of the median are not counted.
+    std::vector<float> d(c.evCount);
+    for (int i = 0; i < c.evCount; ++i) YCN_HIP(hipEventElapsedTime(&d[i], c.evStart[i], c.evStop[i]));
+    std::vector<float> sorted(d);
+    std::sort(sorted.begin(), sorted.end());
+    const float cut = sorted.empty() ? 0.f : 0.25f * sorted[sorted.size() / 2];
     double tot = 0;
-    for (int i = 0; i < c.evCount; ++i) {
-        float ms = 0;
-        YCN_HIP(hipEventElapsedTime(&ms, c.evStart[i], c.evStop[i]));
-        tot += ms;
-    }
+    int64_t n = 0;
+    for (float ms : d)
+        if (ms >= cut) {
+            tot += ms;
+            ++n;
+        }
     if (total_ms) *total_ms = tot;
-    if (launches) *launches = c.evCount;
+    if (launches) *launches = n;
     if (bytes_per_launch) *bytes_per_launch = c.profBytes;
     c.profTarget = PK_NONE;
     return YCN_OK;

@@ -99,6 +99,8 @@ struct Ctx {
 
     // mixed precision: [nSys][4][V] float2
     float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *Aps = nullptr, *xs = nullptr;
+    // pipelined even-odd FP32 iteration: the second buffers of r and Ap ([nSys][4][V/2])
+    float2 *pfR1 = nullptr, *pfA1 = nullptr;
     float *phis = nullptr;
 
     // learned preconditioner
@@ -200,7 +202,7 @@ int read_stats(Ctx &c, int nsys, SolveStats *st);
 // [nsys][4][n] vectors (c.x, c.r, c.Ap; mixed: c.xs, c.rs, c.Aps), returning their block
 // count, the reliable-update accumulation x += x_s, and an FP64 -> FP32 copy.
 void launch_fin_alpha(Ctx &c, int nsys, int nb);
-void launch_fin_beta(Ctx &c, int nsys, int nb, bool mixed);
+void launch_fin_beta(Ctx &c, int nsys, int nb, bool mixed, bool even);
 int launch_cg_update(Ctx &c, int nsys, int n, const double2 *p);
 int launch_mp_update(Ctx &c, int nsys, int n, const float2 *p);
 int launch_mp_accum(Ctx &c, int nsys, int n);
