@@ -32,12 +32,14 @@ ROOT = os.path.dirname(os.path.abspath(__file__))
 sys.path.insert(0, ROOT)
 
 HBM_PEAK_GBS = 8000.0
-# HBM bytes per launch of the fused normal operator at 1024x1024 (read + write), from
+# HBM bytes per launch of the dominant CG kernel at 1024x1024 (read + write), from
 # rocprofv3 --pmc FETCH_SIZE and WRITE_SIZE runs scaled by what the same counters read
 # for a 1 GiB device copy (FETCH_SIZE 0.5 KiB, WRITE_SIZE 1.0 KiB per KiB moved); the
-# summary with its commands is profiles/hbm_traffic_mixed_cg_1024x1024.md.  Keyed by
-# (kernel, precision, lattice extent); others report null.
-MEASURED_TRAFFIC = {("cg_normal_op", "fp32", 1024): 140720905}
+# summaries with their commands are profiles/hbm_traffic_mixed_cg_1024x1024.md (fused
+# normal operator) and profiles/hbm_traffic_eo_mixed_cg_1024x1024.md (pipelined
+# even-odd iteration).  Keyed by (kernel, precision, lattice extent); others report null.
+MEASURED_TRAFFIC = {("cg_normal_op", "fp32", 1024): 140720905,
+                    ("cg_normal_op-eo", "fp32", 1024): 143248032}
 
 WORKLOAD = dict(Nt=1024, Nx=1024, m2=-0.4, lam=1.0, g=0.5, M=0.05, r=1.0, cutFraction=0.5,
                 driftMode=0, epsBar=0.005, adaptive=0, seed=1234, cgTol=1e-10, cgTolMode=1,
@@ -276,6 +278,8 @@ def main():
             "traffic": MEASURED_TRAFFIC.get((args.profile_kernel + ("-eo" if head.startswith("eo-") else ""),
                                              "fp64" if head in ("fp64", "eo-fp64") else "fp32", args.size)),
             "avg_launch_us": avg_ms * 1e3,
+            "measured": f"HIP events around every working launch in a profiled pass of {args.prof_steps} steps "
+                        "after the timed region (interval includes the launch's dispatch latency)",
             "launches": k_n,
             "algorithmic_bytes_per_launch": k_bytes,
         },

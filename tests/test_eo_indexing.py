@@ -10,7 +10,7 @@ import itertools
 import pytest
 
 EO64 = [(8, 32), (16, 16), (4, 64), (4, 32), (2, 32)]
-EO32 = [(8, 32), (16, 32), (8, 64), (16, 16), (4, 64), (2, 32)]
+EO32 = [(8, 32), (16, 32), (8, 64), (16, 16), (4, 64), (2, 32), (32, 32), (16, 64)]
 
 
 def wrapm(a, n):

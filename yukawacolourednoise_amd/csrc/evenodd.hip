@@ -379,38 +379,64 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict
 
     // stage 0: p' on the even sites of the whole array, and d on both parities.
     // Array (row, col) is lattice site (t0 - 4 + row, x0 - 4 + col); its parity is
-    // (row + col) & 1 because t0 + x0 is even.
+    // (row + col) & 1 because t0 + x0 is even.  A thread handles M0 sites: all their
+    // loads are issued before any is used.
+    constexpr int M0 = (H * K + NT - 1) / NT;
     double rho = 0.0;
-    for (int w = threadIdx.x; w < H * K; w += NT) {
-        const int row = w / K, k = w - row * K;
-        const int gt = wrapm(t0 - 4 + row, Nt);
-        const int colE = 2 * k + (row & 1), colO = 2 * k + 1 - (row & 1);
-        const int gxE = wrapm(x0 - 4 + colE, Nx), gxO = wrapm(x0 - 4 + colO, Nx);
-        const size_t j = (size_t)gt * Nh + (gxE >> 1);
-        const bool inner = row >= 4 && row < 4 + TT && colE >= 4 && colE < 4 + TX;
+    {
+        V2 ra[M0][4], pa[M0][4], aa[M0][4], xa[M0][4];
+        R fe[M0], fo[M0];
+        size_t jj[M0];
 #pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const size_t i = off + c * Vh + j;
-            V2 a = r[i];
-            const V2 b = pold[i];
-            if (PF) {
-                const V2 ap = Ap_in[i];
-                a = mk2<R>(a.x - alpha * ap.x, a.y - alpha * ap.y);
-            }
-            const V2 pv = mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
-            E[c][w] = pv;
-            if (inner) {
-                pnew[i] = pv;
-                if (PF) {
-                    r_out[i] = a;
-                    const V2 xv = xs[i];
-                    xs[i] = mk2<R>(xv.x + alpha * b.x, xv.y + alpha * b.y);
-                    rho += (double)(a.x * a.x + a.y * a.y);
+        for (int u = 0; u < M0; ++u) {
+            const int w = threadIdx.x + u * NT;
+            if (w < H * K) {
+                const int row = w / K, k = w - row * K;
+                const int gt = wrapm(t0 - 4 + row, Nt);
+                const int colE = 2 * k + (row & 1), colO = 2 * k + 1 - (row & 1);
+                const int gxE = wrapm(x0 - 4 + colE, Nx), gxO = wrapm(x0 - 4 + colO, Nx);
+                jj[u] = (size_t)gt * Nh + (gxE >> 1);
+                const bool inner = row >= 4 && row < 4 + TT && colE >= 4 && colE < 4 + TX;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const size_t i = off + c * Vh + jj[u];
+                    ra[u][c] = r[i];
+                    pa[u][c] = pold[i];
+                    if (PF) aa[u][c] = Ap_in[i];
+                    if (PF && inner) xa[u][c] = xs[i];
                 }
+                fe[u] = ph[(size_t)gt * Nx + gxE];
+                fo[u] = ph[(size_t)gt * Nx + gxO];
             }
         }
-        FE[w] = d0 + g * ph[(size_t)gt * Nx + gxE];
-        FO[w] = d0 + g * ph[(size_t)gt * Nx + gxO];
+#pragma unroll
+        for (int u = 0; u < M0; ++u) {
+            const int w = threadIdx.x + u * NT;
+            if (w < H * K) {
+                const int row = w / K, k = w - row * K;
+                const int colE = 2 * k + (row & 1);
+                const bool inner = row >= 4 && row < 4 + TT && colE >= 4 && colE < 4 + TX;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    const size_t i = off + c * Vh + jj[u];
+                    V2 a = ra[u][c];
+                    const V2 b = pa[u][c];
+                    if (PF) a = mk2<R>(a.x - alpha * aa[u][c].x, a.y - alpha * aa[u][c].y);
+                    const V2 pv = mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
+                    E[c][w] = pv;
+                    if (inner) {
+                        pnew[i] = pv;
+                        if (PF) {
+                            r_out[i] = a;
+                            xs[i] = mk2<R>(xa[u][c].x + alpha * b.x, xa[u][c].y + alpha * b.y);
+                            rho += (double)(a.x * a.x + a.y * a.y);
+                        }
+                    }
+                }
+                FE[w] = d0 + g * fe[u];
+                FO[w] = d0 + g * fo[u];
+            }
+        }
     }
     __syncthreads();
 
@@ -526,7 +552,21 @@ k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int
     const int skip = st[sys].done;
     const double2 *pp = part + 2 * (size_t)sys * nb;
     double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-    for (int i = threadIdx.x; i < nb; i += kNT) {
+    int i = threadIdx.x;
+    for (; i + 3 * kNT < nb; i += 4 * kNT) {   // four block partials in flight per thread
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[2 * u] = pp[2 * (i + u * kNT)];
+            v[2 * u + 1] = pp[2 * (i + u * kNT) + 1];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a0 = cadd(a0, v[2 * u]);
+            a1 = cadd(a1, v[2 * u + 1]);
+        }
+    }
+    for (; i < nb; i += kNT) {
         a0 = cadd(a0, pp[2 * i]);
         a1 = cadd(a1, pp[2 * i + 1]);
     }
@@ -613,7 +653,7 @@ k_fin_eo_pf_true(const double2 *__restrict__ partRes, const double2 *__restrict_
 // ---------------------------------------------------------------------------
 // at least 64 sites per tile: the partials buffer holds V/64 entries per system
 constexpr TileShape kEoTiles64[] = {{8, 32}, {16, 16}, {4, 64}, {4, 32}, {2, 32}};
-constexpr TileShape kEoTiles32[] = {{8, 32}, {16, 32}, {8, 64}, {16, 16}, {4, 64}, {2, 32}};
+constexpr TileShape kEoTiles32[] = {{8, 32}, {16, 32}, {8, 64}, {16, 16}, {4, 64}, {2, 32}, {32, 32}, {16, 64}};
 constexpr int kNumEo64 = sizeof(kEoTiles64) / sizeof(kEoTiles64[0]);
 constexpr int kNumEo32 = sizeof(kEoTiles32) / sizeof(kEoTiles32[0]);
 
@@ -705,7 +745,9 @@ static int launch_eo_pf(Ctx &c, int ti, int nsys, int par) {
     case 2: launch_eo_tt<float, 8, 64>(c, nsys, A, c.phis, true, b); break;
     case 3: launch_eo_tt<float, 16, 16>(c, nsys, A, c.phis, true, b); break;
     case 4: launch_eo_tt<float, 4, 64>(c, nsys, A, c.phis, true, b); break;
-    default: launch_eo_tt<float, 2, 32>(c, nsys, A, c.phis, true, b); break;
+    case 5: launch_eo_tt<float, 2, 32>(c, nsys, A, c.phis, true, b); break;
+    case 6: launch_eo_tt<float, 32, 32>(c, nsys, A, c.phis, true, b); break;
+    default: launch_eo_tt<float, 16, 64>(c, nsys, A, c.phis, true, b); break;
     }
     prof_after(c, PK_NORMAL_OP);
     return A.tilesPerSys;

@@ -4,6 +4,7 @@
 
 #include <algorithm>
 #include <cmath>
+#include <cstdlib>
 #include <cstring>
 #include <new>
 #include <string>
@@ -165,6 +166,9 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         c.err = "the learned preconditioner needs Nt % 16 == 0 and Nx % 16 == 0";
         return bail(YCN_ERR_UNSUPPORTED);
     }
+    // YCN_GRAPHS=0 launches CG chunks kernel by kernel instead of replaying captured
+    // graphs (for kernel-trace profilers); results are identical
+    if (const char *gr = std::getenv("YCN_GRAPHS")) c.useGraphs = std::atoi(gr) != 0;
     if ((rc = cg_autotune(c))) return bail(rc);
     if (c.p.evenOdd) {
         if (c.p.precond == YCN_PRECOND_CNN) {
