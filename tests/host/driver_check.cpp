@@ -61,6 +61,6 @@ int main(int argc, char **argv) {
     std::cout << n << "\n";
     std::cout << "cfg " << cfg.m2 << " " << cfg.lambda << " " << cfg.kappa << " " << cfg.yukawa_coupling << " "
               << cfg.Nt << " " << cfg.Nx << " " << cfg.cgTol << " " << cfg.cgTolMode << " " << cfg.precond << " "
-              << cfg.cnnTrainPairs << " " << cfg.cnnPairsPerSolve << " " << cfg.cnnLearningRate << "\n";
+              << cfg.cnnTrainPairs << " " << cfg.cnnPairsPerSolve << " " << cfg.cnnLearningRate << " " << cfg.evenOdd << "\n";
     return 0;
 }

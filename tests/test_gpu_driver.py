@@ -86,6 +86,11 @@ def test_driver_fp64(gpu, tmp_path):
     _check_files(tmp_path, out)
 
 
+def test_driver_even_odd_mixed(gpu, tmp_path):
+    out = _run(tmp_path, 'precision = "mixed"\nevenOdd = true\n')
+    _check_files(tmp_path, out)
+
+
 def test_driver_mixed_with_trained_cnn(gpu, tmp_path):
     out = _run(tmp_path, 'precision = "mixed"\nprecond = "cnn"\ncnnTrainPairs = 40\ncnnPairsPerSolve = 8\n')
     _check_files(tmp_path, out)

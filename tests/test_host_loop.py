@@ -176,7 +176,7 @@ def test_parameter_mapping(driver, tmp_path):
     assert float(cfg[2]) == pytest.approx(6 * L / k ** 2, rel=1e-5)
     assert float(cfg[4]) == pytest.approx(1.0 / math.sqrt(2 * k), rel=1e-5)
     assert cfg[5:9] == ["6", "8", "1e-08", "0"]
-    assert cfg[9:] == ["0", "2000", "16", "0.001"]          # no preconditioner; training defaults
+    assert cfg[9:] == ["0", "2000", "16", "0.001", "0"]     # no preconditioner; training defaults; no even-odd
     # useMass = "true": m2 = mass, lambda = g, kappa = FindKappa (main.cu:58-63,117-122)
     out, _ = run(driver, tmp_path, useMass="true")
     cfg = out[1].split()
@@ -196,11 +196,19 @@ def test_cnn_keys(driver, tmp_path):
     inp.write_text(text)
     r = subprocess.run([driver, str(inp), str(tmp_path)], check=True, capture_output=True, text=True)
     cfg = r.stdout.split("\n")[1].split()
-    assert cfg[9:] == ["1", "64", "8", "0.002"]
+    assert cfg[9:] == ["1", "64", "8", "0.002", "0"]
     bad = tmp_path / "bad.toml"
     bad.write_text(text.replace("cnnPairsPerSolve = 8", "cnnPairsPerSolve = 0"))
     r = subprocess.run([driver, str(bad), str(tmp_path)], capture_output=True, text=True)
     assert r.returncode != 0
+
+
+def test_even_odd_key(driver, tmp_path):
+    text = INPUT.format(useMass="false", tmax=1.0, adaptive="true") + '\n[cg]\nevenOdd = true\nprecision = "mixed"\n'
+    inp = tmp_path / "input.toml"
+    inp.write_text(text)
+    r = subprocess.run([driver, str(inp), str(tmp_path)], check=True, capture_output=True, text=True)
+    assert r.stdout.split("\n")[1].split()[13] == "1"
 
 
 def test_reference_input_parses(driver, tmp_path):

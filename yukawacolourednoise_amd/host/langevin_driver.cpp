@@ -84,6 +84,7 @@ RunConfig RunConfig::from_toml(const Toml &t) {
     c.cnnTrainPairs = (int)t.opt_int("cg", "cnnTrainPairs", 2000);
     c.cnnPairsPerSolve = (int)t.opt_int("cg", "cnnPairsPerSolve", 16);
     c.cnnLearningRate = t.opt_double("cg", "cnnLearningRate", 1e-3);
+    c.evenOdd = t.opt_bool("cg", "evenOdd", false);
     if (c.cnnTrainPairs < 0 || c.cnnPairsPerSolve < 1 || !(c.cnnLearningRate > 0))
         throw std::runtime_error("cg.cnnTrainPairs >= 0, cg.cnnPairsPerSolve >= 1 and cg.cnnLearningRate > 0 required");
     c.hotStart = parse_choice(t.opt_string("start", "mode", "hot"), {"cold", "hot"}, "start.mode") == 1;

@@ -44,6 +44,8 @@ struct RunConfig {
     // seen cnnTrainPairs (residual, error) pairs, cnnPairsPerSolve per fermion solve
     int cnnTrainPairs = 2000, cnnPairsPerSolve = 16;
     double cnnLearningRate = 1e-3;
+    // [cg] evenOdd: fermion force through the even-odd preconditioned solve
+    bool evenOdd = false;
     // [start]
     bool hotStart = true;
     long long initSeed = 0;

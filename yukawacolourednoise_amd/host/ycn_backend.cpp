@@ -46,6 +46,7 @@ ycn_params params_from_config(const RunConfig &cfg) {
     p.cgPrecision = cfg.cgPrecision;
     p.reliableEvery = cfg.reliableEvery;
     p.precond = cfg.precond;
+    p.evenOdd = cfg.evenOdd ? 1 : 0;
     return p;
 }
 
