@@ -80,3 +80,21 @@ def test_eo_fused_indexing(TT, TX):
     assert shapes
     for Nt, Nx in shapes:
         check(Nt, Nx, TT, TX)
+
+
+@pytest.mark.parametrize("nTiles,grid", [(2048, 1024), (2048, 512), (4096, 1024), (64, 64), (24, 8), (5, 5)])
+def test_persistent_tile_walk(nTiles, grid):
+    """Persistent launches: block b walks tiles b, b + grid, ...; with the XCD remap
+    (nTiles % 8 == 0, grid % 8 == 0) every tile is visited once and block b's tiles lie in
+    the band of XCD b % 8."""
+    remap = nTiles % 8 == 0
+    seen = []
+    for b in range(grid):
+        t = b
+        while t < nTiles:
+            u = (t % 8) * (nTiles // 8) + t // 8 if remap else t
+            if remap:
+                assert u // (nTiles // 8) == b % 8
+            seen.append(u)
+            t += grid
+    assert sorted(seen) == list(range(nTiles))

@@ -11,7 +11,8 @@ from test_gpu_parity import RTOL, mk, rel
 pytestmark = pytest.mark.gpu
 
 EO_TILES64 = [(8, 32), (16, 16), (4, 64), (4, 32), (2, 32)]
-EO_TILES32 = [(8, 32), (16, 32), (8, 64), (16, 16), (4, 64), (2, 32), (32, 32), (16, 64)]
+EO_TILES32 = [(8, 32), (16, 32), (8, 64), (16, 16), (4, 64), (2, 32), (32, 32), (16, 64),
+              (8, 32, "persistent"), (16, 32, "persistent"), (16, 16, "persistent"), (8, 64, "persistent")]
 
 
 @pytest.mark.parametrize("Nt,Nx", [(16, 16), (32, 32), (6, 32), (8, 64)])

@@ -347,7 +347,11 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // partials are (mu, nu) and (rho', 0), rho' the exact |r'|^2 that re-anchors the
 // recurrence every iteration.  r, Ap and p ping-pong between two buffers: blocks read the
 // old ones on their halo while other blocks write the new ones on their tiles.
-template <typename R, int TT, int TX, bool PF>
+// PERSIST = true: the grid holds only as many blocks as are resident; each block walks
+// the tiles blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next tile's stage-0
+// data into registers while it runs stages 1-4 of the current one, so global loads
+// overlap the LDS stages.  One partial per block (its tiles in order).
+template <typename R, int TT, int TX, bool PF, bool PERSIST = false>
 __global__ void __launch_bounds__(eo_threads(TT, TX))
 k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict__ st,
                   const typename Vec2<R>::T *__restrict__ r, typename Vec2<R>::T *__restrict__ r_out,
@@ -356,6 +360,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict
                   typename Vec2<R>::T *__restrict__ xs, double2 *__restrict__ part) {
     using V2 = typename Vec2<R>::T;
     constexpr int H = TT + 8, K = (TX + 8) / 2, NT = eo_threads(TT, TX);
+    constexpr int M0 = (H * K + NT - 1) / NT;
     __shared__ V2 E[4][H * K];
     __shared__ V2 O[4][H * K];
     __shared__ R FE[H * K], FO[H * K];
@@ -364,11 +369,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict
 
     const int sys = blockIdx.y;
     if (st[sys].done) return;
-    const int nb = gridDim.x;
-    int tile = blockIdx.x;
-    if (A.xcdRemap) tile = (tile % 8) * (nb / 8) + tile / 8;
-    const int ty = tile / A.tilesX, tx = tile - ty * A.tilesX;
-    const int t0 = ty * TT, x0 = tx * TX;
+    const int nTiles = A.tilesPerSys;
     const int Nt = A.Nt, Nx = A.Nx, Nh = A.Nh;
     const size_t Vh = A.Vh, off = (size_t)sys * 4 * Vh;
     const R beta = (R)st[sys].beta;
@@ -377,16 +378,23 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict
     const R hp = (R)(0.5 * (A.r + 1.0)), hm = (R)(0.5 * (A.r - 1.0)), hr = (R)(0.5 * A.r), h = (R)0.5;
     const R d0 = (R)(2.0 * A.r + A.M), g = (R)A.g;
 
-    // stage 0: p' on the even sites of the whole array, and d on both parities.
-    // Array (row, col) is lattice site (t0 - 4 + row, x0 - 4 + col); its parity is
-    // (row + col) & 1 because t0 + x0 is even.  A thread handles M0 sites: all their
-    // loads are issued before any is used.
-    constexpr int M0 = (H * K + NT - 1) / NT;
-    double rho = 0.0;
-    {
+    // tile origin of tile number t (contiguous bands of tiles per XCD)
+    auto origin = [&](int t, int &t0, int &x0) {
+        if (A.xcdRemap) t = (t % 8) * (nTiles / 8) + t / 8;
+        const int ty = t / A.tilesX, tx = t - ty * A.tilesX;
+        t0 = ty * TT;
+        x0 = tx * TX;
+    };
+
+    // stage-0 inputs of one tile, loaded into registers: a thread handles M0 sites of the
+    // array's even sites.  Array (row, col) is lattice site (t0 - 4 + row, x0 - 4 + col);
+    // its parity is (row + col) & 1 because t0 + x0 is even.
+    struct Regs {
         V2 ra[M0][4], pa[M0][4], aa[M0][4], xa[M0][4];
         R fe[M0], fo[M0];
         size_t jj[M0];
+    };
+    auto load0 = [&](int t0, int x0, Regs &G) {
 #pragma unroll
         for (int u = 0; u < M0; ++u) {
             const int w = threadIdx.x + u * NT;
@@ -395,20 +403,25 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict
                 const int gt = wrapm(t0 - 4 + row, Nt);
                 const int colE = 2 * k + (row & 1), colO = 2 * k + 1 - (row & 1);
                 const int gxE = wrapm(x0 - 4 + colE, Nx), gxO = wrapm(x0 - 4 + colO, Nx);
-                jj[u] = (size_t)gt * Nh + (gxE >> 1);
+                G.jj[u] = (size_t)gt * Nh + (gxE >> 1);
                 const bool inner = row >= 4 && row < 4 + TT && colE >= 4 && colE < 4 + TX;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const size_t i = off + c * Vh + jj[u];
-                    ra[u][c] = r[i];
-                    pa[u][c] = pold[i];
-                    if (PF) aa[u][c] = Ap_in[i];
-                    if (PF && inner) xa[u][c] = xs[i];
+                    const size_t i = off + c * Vh + G.jj[u];
+                    G.ra[u][c] = r[i];
+                    G.pa[u][c] = pold[i];
+                    if (PF) G.aa[u][c] = Ap_in[i];
+                    if (PF && inner) G.xa[u][c] = xs[i];
                 }
-                fe[u] = ph[(size_t)gt * Nx + gxE];
-                fo[u] = ph[(size_t)gt * Nx + gxO];
+                G.fe[u] = ph[(size_t)gt * Nx + gxE];
+                G.fo[u] = ph[(size_t)gt * Nx + gxO];
             }
         }
+    };
+    // stage 0 proper: p' (PF: r' = r - alpha Ap, x_s += alpha p) into LDS and, on the
+    // tile, to global memory; d on both parities
+    double rho = 0.0;
+    auto compute0 = [&](const Regs &G) {
 #pragma unroll
         for (int u = 0; u < M0; ++u) {
             const int w = threadIdx.x + u * NT;
@@ -418,27 +431,26 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict
                 const bool inner = row >= 4 && row < 4 + TT && colE >= 4 && colE < 4 + TX;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const size_t i = off + c * Vh + jj[u];
-                    V2 a = ra[u][c];
-                    const V2 b = pa[u][c];
-                    if (PF) a = mk2<R>(a.x - alpha * aa[u][c].x, a.y - alpha * aa[u][c].y);
+                    const size_t i = off + c * Vh + G.jj[u];
+                    V2 a = G.ra[u][c];
+                    const V2 b = G.pa[u][c];
+                    if (PF) a = mk2<R>(a.x - alpha * G.aa[u][c].x, a.y - alpha * G.aa[u][c].y);
                     const V2 pv = mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
                     E[c][w] = pv;
                     if (inner) {
                         pnew[i] = pv;
                         if (PF) {
                             r_out[i] = a;
-                            xs[i] = mk2<R>(xa[u][c].x + alpha * b.x, xa[u][c].y + alpha * b.y);
+                            xs[i] = mk2<R>(G.xa[u][c].x + alpha * b.x, G.xa[u][c].y + alpha * b.y);
                             rho += (double)(a.x * a.x + a.y * a.y);
                         }
                     }
                 }
-                FE[w] = d0 + g * fe[u];
-                FO[w] = d0 + g * fo[u];
+                FE[w] = d0 + g * G.fe[u];
+                FO[w] = d0 + g * G.fo[u];
             }
         }
-    }
-    __syncthreads();
+    };
 
     // the sites of parity `par` in rows/cols [4 - hh, 4 + T + hh): w -> (row, k) and the
     // array indices of the site and its four neighbours (other parity)
@@ -459,84 +471,114 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict
         q.ixb = q.row * Kw + ((q.col - 1) >> 1);
         return q;
     };
-    // hopping at site q from the other-parity array N
-    auto hop_at = [&](auto dag, const Site &q, const V2 (*N)[H * K], V2 o[4]) {
-        constexpr bool DAG = decltype(dag)::value;
-        const int gt = wrapm(t0 - 4 + q.row, Nt);
-        const R sf = (gt == Nt - 1) ? (R)-1 : (R)1, sb = (gt == 0) ? (R)-1 : (R)1;
+
+    double acc = 0.0, nu = 0.0;
+    // stages 1-4 of the tile at (t0, x0); stage 0 must be in LDS
+    auto stages = [&](int t0, int x0) {
+        // hopping at site q from the other-parity array N
+        auto hop_at = [&](auto dag, const Site &q, const V2 (*N)[H * K], V2 o[4]) {
+            constexpr bool DAG = decltype(dag)::value;
+            const int gt = wrapm(t0 - 4 + q.row, Nt);
+            const R sf = (gt == Nt - 1) ? (R)-1 : (R)1, sb = (gt == 0) ? (R)-1 : (R)1;
 #pragma unroll
-        for (int f = 0; f < 4; f += 2)
-            hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, N[f][q.itf], N[f + 1][q.itf], sf, N[f][q.itb], N[f + 1][q.itb],
-                                      sb, N[f][q.ixf], N[f + 1][q.ixf], N[f][q.ixb], N[f + 1][q.ixb], o[f], o[f + 1]);
+            for (int f = 0; f < 4; f += 2)
+                hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, N[f][q.itf], N[f + 1][q.itf], sf, N[f][q.itb],
+                                          N[f + 1][q.itb], sb, N[f][q.ixf], N[f + 1][q.ixf], N[f][q.ixb],
+                                          N[f + 1][q.ixb], o[f], o[f + 1]);
+        };
+        // stage 1: U = K p' / d on odd sites, 3-site halo
+        for (int w = threadIdx.x; w < (TT + 6) * ((TX + 6) / 2); w += NT) {
+            const Site q = site_of(w, 3, 1);
+            V2 o[4];
+            hop_at(std::integral_constant<bool, true>(), q, E, o);
+            const R inv = (R)1 / FO[q.i];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) O[c][q.i] = mk2<R>(o[c].x * inv, o[c].y * inv);
+        }
+        __syncthreads();
+        // stage 2: v = d p' - K U on even sites, 2-site halo; |v|^2 over the tile
+        for (int w = threadIdx.x; w < (TT + 4) * ((TX + 4) / 2); w += NT) {
+            const Site q = site_of(w, 2, 0);
+            V2 o[4];
+            hop_at(std::integral_constant<bool, true>(), q, O, o);
+            const R d = FE[q.i];
+            const bool inner = q.row >= 4 && q.row < 4 + TT && q.col >= 4 && q.col < 4 + TX;
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const V2 pv = E[c][q.i];
+                const V2 v = mk2<R>(d * pv.x - o[c].x, d * pv.y - o[c].y);
+                E[c][q.i] = v;
+                if (inner) acc += (double)(v.x * v.x + v.y * v.y);
+            }
+        }
+        __syncthreads();
+        // stage 3: W = H v / d on odd sites, 1-site halo
+        for (int w = threadIdx.x; w < (TT + 2) * ((TX + 2) / 2); w += NT) {
+            const Site q = site_of(w, 1, 1);
+            V2 o[4];
+            hop_at(std::integral_constant<bool, false>(), q, E, o);
+            const R inv = (R)1 / FO[q.i];
+#pragma unroll
+            for (int c = 0; c < 4; ++c) O[c][q.i] = mk2<R>(o[c].x * inv, o[c].y * inv);
+        }
+        __syncthreads();
+        // stage 4: Ap = d v - H W on the tile's even sites (PF: and |Ap|^2)
+        for (int w = threadIdx.x; w < TT * (TX / 2); w += NT) {
+            const Site q = site_of(w, 0, 0);
+            V2 o[4];
+            hop_at(std::integral_constant<bool, false>(), q, O, o);
+            const R d = FE[q.i];
+            const int gt = wrapm(t0 - 4 + q.row, Nt), gx = wrapm(x0 - 4 + q.col, Nx);
+            const size_t j = (size_t)gt * Nh + (gx >> 1);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) {
+                const V2 v = E[c][q.i];
+                const V2 a = mk2<R>(d * v.x - o[c].x, d * v.y - o[c].y);
+                Ap[off + c * Vh + j] = a;
+                if (PF) nu += (double)(a.x * a.x + a.y * a.y);
+            }
+        }
     };
 
-    // stage 1: U = K p' / d on odd sites, 3-site halo
-    for (int w = threadIdx.x; w < (TT + 6) * ((TX + 6) / 2); w += NT) {
-        const Site q = site_of(w, 3, 1);
-        V2 o[4];
-        hop_at(std::integral_constant<bool, true>(), q, E, o);
-        const R inv = (R)1 / FO[q.i];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) O[c][q.i] = mk2<R>(o[c].x * inv, o[c].y * inv);
-    }
-    __syncthreads();
-
-    // stage 2: v = d p' - K U on even sites, 2-site halo; |v|^2 over the tile
-    double acc = 0.0;
-    for (int w = threadIdx.x; w < (TT + 4) * ((TX + 4) / 2); w += NT) {
-        const Site q = site_of(w, 2, 0);
-        V2 o[4];
-        hop_at(std::integral_constant<bool, true>(), q, O, o);
-        const R d = FE[q.i];
-        const bool inner = q.row >= 4 && q.row < 4 + TT && q.col >= 4 && q.col < 4 + TX;
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const V2 pv = E[c][q.i];
-            const V2 v = mk2<R>(d * pv.x - o[c].x, d * pv.y - o[c].y);
-            E[c][q.i] = v;
-            if (inner) acc += (double)(v.x * v.x + v.y * v.y);
+    Regs G;
+    int t0, x0;
+    if (!PERSIST) {
+        origin(blockIdx.x, t0, x0);
+        load0(t0, x0, G);
+        compute0(G);
+        __syncthreads();
+        stages(t0, x0);
+    } else {
+        int t = blockIdx.x;
+        origin(t, t0, x0);
+        load0(t0, x0, G);
+        while (t < nTiles) {
+            compute0(G);
+            __syncthreads();
+            const int tn = t + (int)gridDim.x;
+            int n0 = 0, nx0 = 0;
+            if (tn < nTiles) {   // next tile's loads in flight during stages 1-4
+                origin(tn, n0, nx0);
+                load0(n0, nx0, G);
+            }
+            stages(t0, x0);
+            __syncthreads();     // stage 4 reads of E and O precede the next stage 0
+            t = tn;
+            t0 = n0;
+            x0 = nx0;
         }
     }
-    __syncthreads();
-
-    // stage 3: W = H v / d on odd sites, 1-site halo
-    for (int w = threadIdx.x; w < (TT + 2) * ((TX + 2) / 2); w += NT) {
-        const Site q = site_of(w, 1, 1);
-        V2 o[4];
-        hop_at(std::integral_constant<bool, false>(), q, E, o);
-        const R inv = (R)1 / FO[q.i];
-#pragma unroll
-        for (int c = 0; c < 4; ++c) O[c][q.i] = mk2<R>(o[c].x * inv, o[c].y * inv);
-    }
-    __syncthreads();
-
-    // stage 4: Ap = d v - H W on the tile's even sites (PF: and |Ap|^2)
-    double nu = 0.0;
-    for (int w = threadIdx.x; w < TT * (TX / 2); w += NT) {
-        const Site q = site_of(w, 0, 0);
-        V2 o[4];
-        hop_at(std::integral_constant<bool, false>(), q, O, o);
-        const R d = FE[q.i];
-        const int gt = wrapm(t0 - 4 + q.row, Nt), gx = wrapm(x0 - 4 + q.col, Nx);
-        const size_t j = (size_t)gt * Nh + (gx >> 1);
-#pragma unroll
-        for (int c = 0; c < 4; ++c) {
-            const V2 v = E[c][q.i];
-            const V2 a = mk2<R>(d * v.x - o[c].x, d * v.y - o[c].y);
-            Ap[off + c * Vh + j] = a;
-            if (PF) nu += (double)(a.x * a.x + a.y * a.y);
-        }
-    }
+    const size_t pidx = (size_t)sys * gridDim.x + blockIdx.x;
     if (PF) {
         const double2 t1 = block_sum2<NT>(make_double2(acc, nu), red);
         const double2 t2 = block_sum2<NT>(make_double2(rho, 0.0), red2);
         if (threadIdx.x == 0) {
-            part[2 * ((size_t)sys * nb + blockIdx.x)] = t1;
-            part[2 * ((size_t)sys * nb + blockIdx.x) + 1] = t2;
+            part[2 * pidx] = t1;
+            part[2 * pidx + 1] = t2;
         }
     } else {
         const double2 tot = block_sum2<NT>(make_double2(acc, 0.0), red);
-        if (threadIdx.x == 0) part[(size_t)sys * nb + blockIdx.x] = tot;
+        if (threadIdx.x == 0) part[pidx] = tot;
     }
 }
 
@@ -651,9 +693,17 @@ k_fin_eo_pf_true(const double2 *__restrict__ partRes, const double2 *__restrict_
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-// at least 64 sites per tile: the partials buffer holds V/64 entries per system
-constexpr TileShape kEoTiles64[] = {{8, 32}, {16, 16}, {4, 64}, {4, 32}, {2, 32}};
-constexpr TileShape kEoTiles32[] = {{8, 32}, {16, 32}, {8, 64}, {16, 16}, {4, 64}, {2, 32}, {32, 32}, {16, 64}};
+// at least 64 sites per tile: the partials buffer holds 2 V/64 entries per system.
+// FP32 entries with persist = true launch only the resident blocks, each walking several
+// tiles with the next tile's loads in flight.
+struct EoTileCfg {
+    int TT, TX;
+    bool persist;
+};
+constexpr EoTileCfg kEoTiles64[] = {{8, 32, false}, {16, 16, false}, {4, 64, false}, {4, 32, false}, {2, 32, false}};
+constexpr EoTileCfg kEoTiles32[] = {{8, 32, false},  {16, 32, false}, {8, 64, false},  {16, 16, false},
+                                    {4, 64, false},  {2, 32, false},  {32, 32, false}, {16, 64, false},
+                                    {8, 32, true},   {16, 32, true},  {16, 16, true},  {8, 64, true}};
 constexpr int kNumEo64 = sizeof(kEoTiles64) / sizeof(kEoTiles64[0]);
 constexpr int kNumEo32 = sizeof(kEoTiles32) / sizeof(kEoTiles32[0]);
 
@@ -674,7 +724,7 @@ static EoArgs eo_args(const Ctx &c, int TT, int TX) {
     return A;
 }
 
-static bool eo_fits(const Ctx &c, TileShape t) { return c.p.Nt % t.TT == 0 && c.p.Nx % t.TX == 0; }
+static bool eo_fits(const Ctx &c, EoTileCfg t) { return c.p.Nt % t.TT == 0 && c.p.Nx % t.TX == 0; }
 
 bool eo_available(const Ctx &c) { return c.p.Nt % 2 == 0 && c.p.Nx % 2 == 0 && c.tileEo64 >= 0; }
 
@@ -689,24 +739,46 @@ struct EoPtrs {
     V2 *pnew = nullptr, *Ap = nullptr, *xs = nullptr;
 };
 
+// resident blocks of a kernel on the whole device (persistent launches)
+static int resident_blocks(const void *kernel, int threads) {
+    int dev = 0, cus = 0, per = 0;
+    if (hipGetDevice(&dev) != hipSuccess || hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev) !=
+                                                 hipSuccess)
+        return 0;
+    if (hipOccupancyMaxActiveBlocksPerMultiprocessor(&per, kernel, threads, 0) != hipSuccess) return 0;
+    return per * cus;
+}
+
+// returns the number of blocks per system (= partial slots per system)
 template <typename R, int TT, int TX>
-static void launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, bool pf, const EoPtrs<R> &b) {
-    const dim3 grid(A.tilesPerSys, nsys), block(eo_threads(TT, TX));
+static int launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, bool pf, bool persist,
+                        const EoPtrs<R> &b) {
+    const dim3 block(eo_threads(TT, TX));
     if constexpr (std::is_same<R, float>::value) {
+        if (pf && persist) {
+            auto *k = &k_eo_normal_fused<R, TT, TX, true, true>;
+            static int resident = resident_blocks(reinterpret_cast<const void *>(k), eo_threads(TT, TX));
+            int g = std::min(A.tilesPerSys, std::max(resident, 8));
+            if (A.xcdRemap && g >= 8) g -= g % 8;   // keeps every block's tiles on one XCD's band
+            hipLaunchKernelGGL(k, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold,
+                               b.pnew, b.Ap, b.xs, c.partials);
+            return g;
+        }
         if (pf) {
-            hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, true>), grid, block, 0, c.stream, A, phi, c.cg_d, b.r,
-                               b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials);
-            return;
+            hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, true>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A,
+                               phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials);
+            return A.tilesPerSys;
         }
     }
-    hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), grid, block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out,
-                       b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials);
+    hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi,
+                       c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials);
+    return A.tilesPerSys;
 }
 
 // launch the fused Schur normal operator (FP64) with tile `ti`; returns the partial count
 static int launch_eo_fused(Ctx &c, int ti, int nsys, const double *phi, const double2 *r, const double2 *pold,
                            double2 *pnew, double2 *Ap) {
-    const TileShape t = kEoTiles64[ti];
+    const EoTileCfg t = kEoTiles64[ti];
     const EoArgs A = eo_args(c, t.TT, t.TX);
     EoPtrs<double> b;
     b.r = r;
@@ -714,12 +786,13 @@ static int launch_eo_fused(Ctx &c, int ti, int nsys, const double *phi, const do
     b.pnew = pnew;
     b.Ap = Ap;
     prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (2 * 64 + 8));
+    int nb = 0;
     switch (ti) {
-    case 0: launch_eo_tt<double, 8, 32>(c, nsys, A, phi, false, b); break;
-    case 1: launch_eo_tt<double, 16, 16>(c, nsys, A, phi, false, b); break;
-    case 2: launch_eo_tt<double, 4, 64>(c, nsys, A, phi, false, b); break;
-    case 3: launch_eo_tt<double, 4, 32>(c, nsys, A, phi, false, b); break;
-    default: launch_eo_tt<double, 2, 32>(c, nsys, A, phi, false, b); break;
+    case 0: nb = launch_eo_tt<double, 8, 32>(c, nsys, A, phi, false, false, b); break;
+    case 1: nb = launch_eo_tt<double, 16, 16>(c, nsys, A, phi, false, false, b); break;
+    case 2: nb = launch_eo_tt<double, 4, 64>(c, nsys, A, phi, false, false, b); break;
+    case 3: nb = launch_eo_tt<double, 4, 32>(c, nsys, A, phi, false, false, b); break;
+    default: nb = launch_eo_tt<double, 2, 32>(c, nsys, A, phi, false, false, b); break;
     }
     prof_after(c, PK_NORMAL_OP);
     return A.tilesPerSys;
@@ -727,7 +800,7 @@ static int launch_eo_fused(Ctx &c, int ti, int nsys, const double *phi, const do
 
 // the pipelined FP32 iteration's fused kernel with tile `ti` and ping-pong parity `par`
 static int launch_eo_pf(Ctx &c, int ti, int nsys, int par) {
-    const TileShape t = kEoTiles32[ti];
+    const EoTileCfg t = kEoTiles32[ti];
     const EoArgs A = eo_args(c, t.TT, t.TX);
     EoPtrs<float> b;
     b.r = par ? c.pfR1 : c.rs;
@@ -739,18 +812,20 @@ static int launch_eo_pf(Ctx &c, int ti, int nsys, int par) {
     b.xs = c.xs;
     // read r, Ap, p, x_s and write r, p, Ap, x_s (half-lattice FP32 vectors) and read phi
     prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (8 * 16 + 4));
-    switch (ti) {
-    case 0: launch_eo_tt<float, 8, 32>(c, nsys, A, c.phis, true, b); break;
-    case 1: launch_eo_tt<float, 16, 32>(c, nsys, A, c.phis, true, b); break;
-    case 2: launch_eo_tt<float, 8, 64>(c, nsys, A, c.phis, true, b); break;
-    case 3: launch_eo_tt<float, 16, 16>(c, nsys, A, c.phis, true, b); break;
-    case 4: launch_eo_tt<float, 4, 64>(c, nsys, A, c.phis, true, b); break;
-    case 5: launch_eo_tt<float, 2, 32>(c, nsys, A, c.phis, true, b); break;
-    case 6: launch_eo_tt<float, 32, 32>(c, nsys, A, c.phis, true, b); break;
-    default: launch_eo_tt<float, 16, 64>(c, nsys, A, c.phis, true, b); break;
+    const bool ps = t.persist;
+    int nb = 0;
+    switch (t.TT * 1000 + t.TX) {
+    case 8032: nb = launch_eo_tt<float, 8, 32>(c, nsys, A, c.phis, true, ps, b); break;
+    case 16032: nb = launch_eo_tt<float, 16, 32>(c, nsys, A, c.phis, true, ps, b); break;
+    case 8064: nb = launch_eo_tt<float, 8, 64>(c, nsys, A, c.phis, true, ps, b); break;
+    case 16016: nb = launch_eo_tt<float, 16, 16>(c, nsys, A, c.phis, true, ps, b); break;
+    case 4064: nb = launch_eo_tt<float, 4, 64>(c, nsys, A, c.phis, true, ps, b); break;
+    case 2032: nb = launch_eo_tt<float, 2, 32>(c, nsys, A, c.phis, true, ps, b); break;
+    case 32032: nb = launch_eo_tt<float, 32, 32>(c, nsys, A, c.phis, true, ps, b); break;
+    default: nb = launch_eo_tt<float, 16, 64>(c, nsys, A, c.phis, true, ps, b); break;
     }
     prof_after(c, PK_NORMAL_OP);
-    return A.tilesPerSys;
+    return nb;
 }
 
 static dim3 half_grid(const Ctx &c, int nsys) { return dim3((c.V / 2 + kNT - 1) / kNT, nsys); }
@@ -794,7 +869,8 @@ int eo_autotune(Ctx &c) {
         for (int i = 0; i < kNumEo32; ++i) {
             if (!eo_fits(c, kEoTiles32[i])) continue;
             const double us = time_it([&](int par) { (void)launch_eo_pf(c, i, c.nSys, par); });
-            std::snprintf(line, sizeof line, " %dx%d %.1f;", kEoTiles32[i].TT, kEoTiles32[i].TX, us);
+            std::snprintf(line, sizeof line, " %dx%d%s %.1f;", kEoTiles32[i].TT, kEoTiles32[i].TX,
+                          kEoTiles32[i].persist ? "p" : "", us);
             c.tuneReport += line;
             if (us < best) { best = us; c.tileEo32 = i; }
         }
