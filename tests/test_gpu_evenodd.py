@@ -84,9 +84,16 @@ def test_eo_every_fp64_tile(gpu, tile, monkeypatch):
     assert rel(d, dr) < RTOL, EO_TILES64[tile]
 
 
+FIN_REPORT = {"separate": "separate finalize", "fused": "in-launch finalize"}
+
+
+@pytest.mark.parametrize("fin", ["separate", "fused"])
 @pytest.mark.parametrize("tile", range(len(EO_TILES32)))
-def test_eo_every_fp32_tile(gpu, tile, monkeypatch):
+def test_eo_every_fp32_tile(gpu, tile, fin, monkeypatch):
+    """Every FP32 tile, with the finalize as its own kernel and inside the launch (the
+    last block of each group of 64 blocks, then the last group, sums the partials)."""
     monkeypatch.setenv("YCN_TILE_EO32", str(tile))
+    monkeypatch.setenv("YCN_EO_FIN", fin)
     Y = gpu
     Nt, Nx = 32, 64
     p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.1, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
@@ -94,11 +101,34 @@ def test_eo_every_fp32_tile(gpu, tile, monkeypatch):
     p.cgPrecision = Y.YCN_CG_MIXED
     phi = O.hot_start(9, Nt, Nx)
     with Y.Context(p) as c:
+        assert FIN_REPORT[fin] in c.tuning_report()
         c.set_field(phi)
         d, st = c.fermion_force(1)
     dr, _ = O.fermion_drift(orc, phi, 1)
     assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10
     assert rel(d, dr) < 1e-8, EO_TILES32[tile]
+
+
+@pytest.mark.parametrize("fin", ["separate", "fused"])
+def test_eo_finalize_ragged_groups(gpu, fin, monkeypatch):
+    """104 x 96 with 2 x 32 tiles: 156 blocks per system, arriving in groups of 64, 64
+    and 28, for two noise systems at once; both finalize forms reach the oracle's drift
+    and take the same number of iterations."""
+    monkeypatch.setenv("YCN_TILE_EO32", "5")
+    monkeypatch.setenv("YCN_EO_FIN", fin)
+    Y = gpu
+    Nt, Nx = 104, 96
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, nNoise=2, M=0.1, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    p.evenOdd = 1
+    p.cgPrecision = Y.YCN_CG_MIXED
+    phi = O.hot_start(19, Nt, Nx)
+    with Y.Context(p) as c:
+        assert "2x32, " + FIN_REPORT[fin] in c.tuning_report()
+        c.set_field(phi)
+        d, st = c.fermion_force(3)
+    dr, _ = O.fermion_drift(orc, phi, 3)
+    assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10 and st["systems"] == 2
+    assert rel(d, dr) < 1e-8
 
 
 def test_eo_batched_replicas_and_noise(gpu):

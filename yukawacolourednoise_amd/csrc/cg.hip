@@ -966,6 +966,8 @@ int mp_alloc(Ctx &c) {
             YCN_HIP(hipMalloc(b, sizeof(float2) * h));
             YCN_HIP(hipMemset(*b, 0, sizeof(float2) * h));
         }
+        int rc = eo_pf_alloc(c);
+        if (rc) return rc;
     }
     return YCN_OK;
 }
@@ -975,6 +977,8 @@ void mp_free(Ctx &c) {
     for (float2 **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (c.phis) { (void)hipFree(c.phis); c.phis = nullptr; }
+    if (c.pfCnt) { (void)hipFree(c.pfCnt); c.pfCnt = nullptr; }
+    if (c.pfGrp) { (void)hipFree(c.pfGrp); c.pfGrp = nullptr; }
 }
 
 static void launch_mp_iteration(Ctx &c, int nsys, int mtype, int parity) {

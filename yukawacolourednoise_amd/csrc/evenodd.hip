@@ -21,6 +21,7 @@
 #include <cmath>
 #include <cstdio>
 #include <cstdlib>
+#include <cstring>
 #include <type_traits>
 
 #include "cg_fused.hpp"
@@ -335,6 +336,96 @@ k_fin_eo_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
 // The diagonal d = 2r + M + g phi is staged per parity for the whole array.  |v|^2 over
 // the tile (= <p', Ap>) is the block's partial.  HBM traffic per launch: read r, p_old
 // (Vh sites each) and phi, write p', Ap: 4 S/2 + P per site of the full lattice.
+// finalize of the pipelined iteration: from (mu, nu) and the exact rho of the iteration
+// just done (block partials pp[2 b], pp[2 b + 1], summed in a fixed order), alpha = rho /
+// mu and beta = (alpha^2 nu - rho) / rho for the next one (a negative prediction, from
+// cancellation, restarts the direction).  Pauses at the stop after an even number of
+// iterations (the last direction applied is then in p[1]), at the cap, or on a
+// non-positive <p, Ap>; a reliable update follows.  commit = false only reduces (timing).
+// device-coherent (agent-scope) store and load of a double2: they bypass the per-XCD L2
+__device__ __forceinline__ void st_coherent(double2 *p, double2 v) {
+    unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
+    __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.y), __ATOMIC_RELAXED,
+                       __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ double2 ld_coherent(const double2 *p) {
+    const unsigned long long *q = reinterpret_cast<const unsigned long long *>(p);
+    const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    return make_double2(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
+}
+template <bool COH>
+__device__ __forceinline__ double2 ld_part(const double2 *p) {
+    if constexpr (COH) return ld_coherent(p);
+    else return *p;
+}
+// Arrival at a counter after this thread's device-coherent stores: they have completed
+// (vmcnt drained) before the increment, so whoever sees the final count sees them.
+// Only those few words are published; a device-wide fence would write back and
+// invalidate the whole L2 of this XCD.
+__device__ __forceinline__ int arrive(int *p) {
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    __builtin_amdgcn_s_waitcnt(0);
+    __atomic_signal_fence(__ATOMIC_SEQ_CST);
+    return __hip_atomic_fetch_add(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ void reset_counter(int *p) {
+    __hip_atomic_store(p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+// in-launch finalize: blocks arrive in groups of kFinGroup (one partial per lane of the
+// group's last block), then the groups at the system's counter; counters sit kCntStride
+// ints apart so that concurrent groups do not serialise on one memory line
+constexpr int kFinGroup = kWave, kCntStride = 64;
+
+template <int NT, bool COH>
+__device__ void pf_finalize(const double2 *pp, int nb, CgState &S, int *active, double2 *lds0, double2 *lds1,
+                            bool commit) {
+    const int skip = S.done;
+    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+    int i = threadIdx.x;
+    for (; i + 3 * NT < nb; i += 4 * NT) {   // four block partials in flight per thread
+        double2 v[8];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            v[2 * u] = ld_part<COH>(pp + 2 * (i + u * NT));
+            v[2 * u + 1] = ld_part<COH>(pp + 2 * (i + u * NT) + 1);
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            a0 = cadd(a0, v[2 * u]);
+            a1 = cadd(a1, v[2 * u + 1]);
+        }
+    }
+    for (; i < nb; i += NT) {
+        a0 = cadd(a0, ld_part<COH>(pp + 2 * i));
+        a1 = cadd(a1, ld_part<COH>(pp + 2 * i + 1));
+    }
+    const double2 mn = block_sum2<NT>(a0, lds0);
+    const double2 rh = block_sum2<NT>(a1, lds1);
+    if (skip || threadIdx.x) return;
+    if (!commit) {
+        S.rz = mn.x + mn.y + rh.x;   // a field the even-odd iteration never reads
+        return;
+    }
+    const double mu = mn.x, nu = mn.y, rho = rh.x;
+    S.iter += 1;
+    S.rr_prev = S.rr;
+    S.rr = rho;
+    const bool stop = (S.iter >= S.maxIter) || (sqrt(rho) <= S.stop && !(S.iter & 1)) ||
+                      (!(mu > 0.0) && !(S.iter & 1)) || !isfinite(mu) || !isfinite(rho);
+    if (stop) {
+        S.done = 1;
+        atomicSub(active, 1);
+        return;
+    }
+    const double al = (mu > 0.0) ? rho / mu : 0.0;
+    double rhoNext = al * al * nu - rho;
+    if (!(rhoNext > 0.0)) rhoNext = 0.0;
+    S.alpha = make_double2(al, 0.0);
+    S.beta = (rho > 0.0) ? rhoNext / rho : 0.0;
+}
+
 constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave : tt * tx / 2; }
 
 // PF = false: the operator alone, p' = r + beta p_old, partial <p', Ap> = |v|^2.
@@ -351,13 +442,21 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // the tiles blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next tile's stage-0
 // data into registers while it runs stages 1-4 of the current one, so global loads
 // overlap the LDS stages.  One partial per block (its tiles in order).
+// finMode (PF only): 0 = partials only (k_fin_pf follows); 1 = the launch finalizes
+// itself: the last block of each group of kFinGroup blocks sums the group's partials,
+// the last group of each system sums the group sums in group order and runs the
+// finalize, so one launch is one whole iteration; 2 = the same reductions without
+// touching the iteration state (the autotuner's timing launches).  cnt holds per
+// system 1 + maxGroups counters, kCntStride ints apart, zero between launches; gpart
+// holds per system maxGroups pairs of group sums.
 template <typename R, int TT, int TX, bool PF, bool PERSIST = false>
 __global__ void __launch_bounds__(eo_threads(TT, TX))
-k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict__ st,
+k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                   const typename Vec2<R>::T *__restrict__ r, typename Vec2<R>::T *__restrict__ r_out,
                   const typename Vec2<R>::T *__restrict__ Ap_in, const typename Vec2<R>::T *__restrict__ pold,
                   typename Vec2<R>::T *__restrict__ pnew, typename Vec2<R>::T *__restrict__ Ap,
-                  typename Vec2<R>::T *__restrict__ xs, double2 *__restrict__ part) {
+                  typename Vec2<R>::T *__restrict__ xs, double2 *__restrict__ part, int finMode, int *cnt,
+                  double2 *gpart, int maxGroups, int *active) {
     using V2 = typename Vec2<R>::T;
     constexpr int H = TT + 8, K = (TX + 8) / 2, NT = eo_threads(TT, TX);
     constexpr int M0 = (H * K + NT - 1) / NT;
@@ -572,66 +671,58 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, const CgState *__restrict
     if (PF) {
         const double2 t1 = block_sum2<NT>(make_double2(acc, nu), red);
         const double2 t2 = block_sum2<NT>(make_double2(rho, 0.0), red2);
-        if (threadIdx.x == 0) {
-            part[2 * pidx] = t1;
-            part[2 * pidx + 1] = t2;
+        if (finMode == 0) {
+            if (threadIdx.x == 0) {
+                part[2 * pidx] = t1;
+                part[2 * pidx + 1] = t2;
+            }
+            return;
         }
+        __shared__ int lastInGroup, lastGroup;
+        const int nb = (int)gridDim.x, grp = (int)blockIdx.x / kFinGroup;
+        const int gsize = min(kFinGroup, nb - grp * kFinGroup), ngrp = (nb + kFinGroup - 1) / kFinGroup;
+        int *cntSys = cnt + (size_t)sys * (1 + maxGroups) * kCntStride;
+        if (threadIdx.x == 0) {
+            st_coherent(part + 2 * pidx, t1);
+            st_coherent(part + 2 * pidx + 1, t2);
+            lastInGroup = arrive(cntSys + (1 + grp) * kCntStride) == gsize - 1;
+            lastGroup = 0;
+        }
+        __syncthreads();
+        if (!lastInGroup) return;
+        if (threadIdx.x < kWave) {   // the group's partials, one per lane, in lane order
+            double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+            if ((int)threadIdx.x < gsize) {
+                const double2 *q = part + 2 * ((size_t)sys * nb + grp * kFinGroup + threadIdx.x);
+                a0 = ld_coherent(q);
+                a1 = ld_coherent(q + 1);
+            }
+            a0 = make_double2(wave_sum(a0.x), wave_sum(a0.y));
+            a1 = make_double2(wave_sum(a1.x), wave_sum(a1.y));
+            if (threadIdx.x == 0) {
+                double2 *gq = gpart + 2 * ((size_t)sys * maxGroups + grp);
+                st_coherent(gq, a0);
+                st_coherent(gq + 1, a1);
+                reset_counter(cntSys + (1 + grp) * kCntStride);
+                lastGroup = arrive(cntSys) == ngrp - 1;
+            }
+        }
+        __syncthreads();
+        if (!lastGroup) return;
+        pf_finalize<NT, true>(gpart + 2 * (size_t)sys * maxGroups, ngrp, st[sys], active, red, red2, finMode == 1);
+        if (threadIdx.x == 0) reset_counter(cntSys);
     } else {
         const double2 tot = block_sum2<NT>(make_double2(acc, 0.0), red);
         if (threadIdx.x == 0) part[pidx] = tot;
     }
 }
 
-// pipelined iteration: from (mu, nu) and the exact rho of the iteration just done,
-// alpha = rho / mu and beta = (alpha^2 nu - rho) / rho for the next one (a negative
-// prediction, from cancellation, restarts the direction).  Pauses at the stop after an
-// even number of iterations (the last direction applied is then in p[1]), at the cap, or
-// on a non-positive <p, Ap>; a reliable update follows.
+// the separate finalize of the pipelined iteration (after a finMode 0 launch): one block
+// per system sums its nb block partials in block order
 __global__ void __launch_bounds__(kNT)
-k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active) {
+k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active, int commit) {
     __shared__ double2 lds0[kNT / kWave], lds1[kNT / kWave];
-    const int sys = blockIdx.x;
-    const int skip = st[sys].done;
-    const double2 *pp = part + 2 * (size_t)sys * nb;
-    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-    int i = threadIdx.x;
-    for (; i + 3 * kNT < nb; i += 4 * kNT) {   // four block partials in flight per thread
-        double2 v[8];
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            v[2 * u] = pp[2 * (i + u * kNT)];
-            v[2 * u + 1] = pp[2 * (i + u * kNT) + 1];
-        }
-#pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            a0 = cadd(a0, v[2 * u]);
-            a1 = cadd(a1, v[2 * u + 1]);
-        }
-    }
-    for (; i < nb; i += kNT) {
-        a0 = cadd(a0, pp[2 * i]);
-        a1 = cadd(a1, pp[2 * i + 1]);
-    }
-    const double2 mn = block_sum2<kNT>(a0, lds0);
-    const double2 rh = block_sum2<kNT>(a1, lds1);
-    if (skip || threadIdx.x) return;
-    CgState &S = st[sys];
-    const double mu = mn.x, nu = mn.y, rho = rh.x;
-    S.iter += 1;
-    S.rr_prev = S.rr;
-    S.rr = rho;
-    const bool stop = (S.iter >= S.maxIter) || (sqrt(rho) <= S.stop && !(S.iter & 1)) ||
-                      (!(mu > 0.0) && !(S.iter & 1)) || !isfinite(mu) || !isfinite(rho);
-    if (stop) {
-        S.done = 1;
-        atomicSub(active, 1);
-        return;
-    }
-    const double al = (mu > 0.0) ? rho / mu : 0.0;
-    double rhoNext = al * al * nu - rho;
-    if (!(rhoNext > 0.0)) rhoNext = 0.0;
-    S.alpha = make_double2(al, 0.0);
-    S.beta = (rho > 0.0) ? rhoNext / rho : 0.0;
+    pf_finalize<kNT, false>(part + 2 * (size_t)blockIdx.x * nb, nb, st[blockIdx.x], active, lds0, lds1, commit != 0);
 }
 
 // reliable update of the pipelined iteration: x += x_s (FP64), x_s = 0, and the last
@@ -726,6 +817,20 @@ static EoArgs eo_args(const Ctx &c, int TT, int TX) {
 
 static bool eo_fits(const Ctx &c, EoTileCfg t) { return c.p.Nt % t.TT == 0 && c.p.Nx % t.TX == 0; }
 
+// arrival counters and group sums of the in-launch finalize, sized for the smallest
+// tile (64 sites, so at most V / 64 blocks per system)
+int eo_pf_alloc(Ctx &c) {
+    const int maxBlocks = (c.V + 63) / 64;
+    c.pfGroups = (maxBlocks + kFinGroup - 1) / kFinGroup;
+    const size_t nc = (size_t)c.nSys * (1 + c.pfGroups) * kCntStride;
+    YCN_HIP(hipMalloc(&c.pfCnt, sizeof(int) * nc));
+    YCN_HIP(hipMemset(c.pfCnt, 0, sizeof(int) * nc));
+    const size_t ng = (size_t)c.nSys * c.pfGroups * 2;
+    YCN_HIP(hipMalloc(&c.pfGrp, sizeof(double2) * ng));
+    YCN_HIP(hipMemset(c.pfGrp, 0, sizeof(double2) * ng));
+    return YCN_OK;
+}
+
 bool eo_available(const Ctx &c) { return c.p.Nt % 2 == 0 && c.p.Nx % 2 == 0 && c.tileEo64 >= 0; }
 
 // buffers of one fused launch (PF: the pipelined iteration's ping-pong pairs and x_s)
@@ -752,7 +857,7 @@ static int resident_blocks(const void *kernel, int threads) {
 // returns the number of blocks per system (= partial slots per system)
 template <typename R, int TT, int TX>
 static int launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, bool pf, bool persist,
-                        const EoPtrs<R> &b) {
+                        const EoPtrs<R> &b, int finMode = 0) {
     const dim3 block(eo_threads(TT, TX));
     if constexpr (std::is_same<R, float>::value) {
         if (pf && persist) {
@@ -761,17 +866,19 @@ static int launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, b
             int g = std::min(A.tilesPerSys, std::max(resident, 8));
             if (A.xcdRemap && g >= 8) g -= g % 8;   // keeps every block's tiles on one XCD's band
             hipLaunchKernelGGL(k, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold,
-                               b.pnew, b.Ap, b.xs, c.partials);
+                               b.pnew, b.Ap, b.xs, c.partials, finMode, c.pfCnt, c.pfGrp, c.pfGroups, c.active);
             return g;
         }
         if (pf) {
             hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, true>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A,
-                               phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials);
+                               phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, finMode,
+                               c.pfCnt, c.pfGrp, c.pfGroups, c.active);
             return A.tilesPerSys;
         }
     }
     hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi,
-                       c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials);
+                       c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0, (int *)nullptr,
+                       (double2 *)nullptr, 0, (int *)nullptr);
     return A.tilesPerSys;
 }
 
@@ -786,20 +893,20 @@ static int launch_eo_fused(Ctx &c, int ti, int nsys, const double *phi, const do
     b.pnew = pnew;
     b.Ap = Ap;
     prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (2 * 64 + 8));
-    int nb = 0;
     switch (ti) {
-    case 0: nb = launch_eo_tt<double, 8, 32>(c, nsys, A, phi, false, false, b); break;
-    case 1: nb = launch_eo_tt<double, 16, 16>(c, nsys, A, phi, false, false, b); break;
-    case 2: nb = launch_eo_tt<double, 4, 64>(c, nsys, A, phi, false, false, b); break;
-    case 3: nb = launch_eo_tt<double, 4, 32>(c, nsys, A, phi, false, false, b); break;
-    default: nb = launch_eo_tt<double, 2, 32>(c, nsys, A, phi, false, false, b); break;
+    case 0: (void)launch_eo_tt<double, 8, 32>(c, nsys, A, phi, false, false, b); break;
+    case 1: (void)launch_eo_tt<double, 16, 16>(c, nsys, A, phi, false, false, b); break;
+    case 2: (void)launch_eo_tt<double, 4, 64>(c, nsys, A, phi, false, false, b); break;
+    case 3: (void)launch_eo_tt<double, 4, 32>(c, nsys, A, phi, false, false, b); break;
+    default: (void)launch_eo_tt<double, 2, 32>(c, nsys, A, phi, false, false, b); break;
     }
     prof_after(c, PK_NORMAL_OP);
     return A.tilesPerSys;
 }
 
-// the pipelined FP32 iteration's fused kernel with tile `ti` and ping-pong parity `par`
-static int launch_eo_pf(Ctx &c, int ti, int nsys, int par) {
+// the pipelined FP32 iteration's fused kernel (finMode as in k_eo_normal_fused) with
+// tile `ti` and ping-pong parity `par`; returns the block partials per system
+static int launch_eo_pf(Ctx &c, int ti, int nsys, int par, int finMode) {
     const EoTileCfg t = kEoTiles32[ti];
     const EoArgs A = eo_args(c, t.TT, t.TX);
     EoPtrs<float> b;
@@ -815,14 +922,14 @@ static int launch_eo_pf(Ctx &c, int ti, int nsys, int par) {
     const bool ps = t.persist;
     int nb = 0;
     switch (t.TT * 1000 + t.TX) {
-    case 8032: nb = launch_eo_tt<float, 8, 32>(c, nsys, A, c.phis, true, ps, b); break;
-    case 16032: nb = launch_eo_tt<float, 16, 32>(c, nsys, A, c.phis, true, ps, b); break;
-    case 8064: nb = launch_eo_tt<float, 8, 64>(c, nsys, A, c.phis, true, ps, b); break;
-    case 16016: nb = launch_eo_tt<float, 16, 16>(c, nsys, A, c.phis, true, ps, b); break;
-    case 4064: nb = launch_eo_tt<float, 4, 64>(c, nsys, A, c.phis, true, ps, b); break;
-    case 2032: nb = launch_eo_tt<float, 2, 32>(c, nsys, A, c.phis, true, ps, b); break;
-    case 32032: nb = launch_eo_tt<float, 32, 32>(c, nsys, A, c.phis, true, ps, b); break;
-    default: nb = launch_eo_tt<float, 16, 64>(c, nsys, A, c.phis, true, ps, b); break;
+    case 8032: nb = launch_eo_tt<float, 8, 32>(c, nsys, A, c.phis, true, ps, b, finMode); break;
+    case 16032: nb = launch_eo_tt<float, 16, 32>(c, nsys, A, c.phis, true, ps, b, finMode); break;
+    case 8064: nb = launch_eo_tt<float, 8, 64>(c, nsys, A, c.phis, true, ps, b, finMode); break;
+    case 16016: nb = launch_eo_tt<float, 16, 16>(c, nsys, A, c.phis, true, ps, b, finMode); break;
+    case 4064: nb = launch_eo_tt<float, 4, 64>(c, nsys, A, c.phis, true, ps, b, finMode); break;
+    case 2032: nb = launch_eo_tt<float, 2, 32>(c, nsys, A, c.phis, true, ps, b, finMode); break;
+    case 32032: nb = launch_eo_tt<float, 32, 32>(c, nsys, A, c.phis, true, ps, b, finMode); break;
+    default: nb = launch_eo_tt<float, 16, 64>(c, nsys, A, c.phis, true, ps, b, finMode); break;
     }
     prof_after(c, PK_NORMAL_OP);
     return nb;
@@ -863,16 +970,33 @@ int eo_autotune(Ctx &c) {
         c.tuneReport += line;
         if (us < best) { best = us; c.tileEo64 = i; }
     }
-    if (c.rs && c.phis && c.pfR1 && c.pfA1) {
-        c.tuneReport += " | even-odd fp32 pipelined tiles (us/launch):";
+    // FP32 pipelined iteration: each tile with the separate finalize and finalizing in
+    // the launch; YCN_EO_FIN=separate|fused restricts the choice to one form
+    const char *fin = std::getenv("YCN_EO_FIN");
+    const bool allowSep = !fin || std::strcmp(fin, "fused") != 0;
+    const bool allowFused = !fin || std::strcmp(fin, "separate") != 0;
+    double us32[kNumEo32][2] = {};
+    if (c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfCnt && c.pfGrp) {
+        c.tuneReport += " | even-odd fp32 pipelined tiles (us/iteration, separate | in-launch finalize):";
         best = 1e30;
         for (int i = 0; i < kNumEo32; ++i) {
+            us32[i][0] = us32[i][1] = 1e30;
             if (!eo_fits(c, kEoTiles32[i])) continue;
-            const double us = time_it([&](int par) { (void)launch_eo_pf(c, i, c.nSys, par); });
-            std::snprintf(line, sizeof line, " %dx%d%s %.1f;", kEoTiles32[i].TT, kEoTiles32[i].TX,
-                          kEoTiles32[i].persist ? "p" : "", us);
+            us32[i][0] = time_it([&](int par) {
+                const int nb = launch_eo_pf(c, i, c.nSys, par, 0);
+                hipLaunchKernelGGL(k_fin_pf, dim3(c.nSys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
+                                   c.cg_d, c.active, 0);
+            });
+            us32[i][1] = time_it([&](int par) { (void)launch_eo_pf(c, i, c.nSys, par, 2); });
+            std::snprintf(line, sizeof line, " %dx%d%s %.1f|%.1f;", kEoTiles32[i].TT, kEoTiles32[i].TX,
+                          kEoTiles32[i].persist ? "p" : "", us32[i][0], us32[i][1]);
             c.tuneReport += line;
-            if (us < best) { best = us; c.tileEo32 = i; }
+            for (int m = 0; m < 2; ++m)
+                if ((m ? allowFused : allowSep) && us32[i][m] < best) {
+                    best = us32[i][m];
+                    c.tileEo32 = i;
+                    c.eoFinFused = m == 1;
+                }
         }
     }
     // YCN_TILE_EO / YCN_TILE_EO32 force a tile of the tables (tests cover every shape)
@@ -883,8 +1007,17 @@ int eo_autotune(Ctx &c) {
     }
     if (const char *f = std::getenv("YCN_TILE_EO32")) {
         const int i = std::atoi(f);
-        if (c.rs && c.phis && c.pfR1 && i >= 0 && i < kNumEo32 && eo_fits(c, kEoTiles32[i])) c.tileEo32 = i;
+        if (c.rs && c.phis && c.pfR1 && c.pfCnt && c.pfGrp && i >= 0 && i < kNumEo32 && eo_fits(c, kEoTiles32[i])) {
+            c.tileEo32 = i;
+            c.eoFinFused = allowFused && (!allowSep || us32[i][1] < us32[i][0]);
+        }
         c.tuneReport += " (even-odd fp32 tile forced by YCN_TILE_EO32)";
+    }
+    if (c.tileEo32 >= 0) {
+        const EoTileCfg t = kEoTiles32[c.tileEo32];
+        std::snprintf(line, sizeof line, " | fp32 pipelined iteration: %dx%d%s, %s finalize", t.TT, t.TX,
+                      t.persist ? "p" : "", c.eoFinFused ? "in-launch" : "separate");
+        c.tuneReport += line;
     }
     (void)hipEventDestroy(e0);
     (void)hipEventDestroy(e1);
@@ -910,11 +1043,16 @@ static void launch_eo_iteration(Ctx &c, int nsys, const double *phi, int parity)
     launch_fin_beta(c, nsys, nbu, false, true);
 }
 
-// one pipelined FP32 iteration: fused kernel + one finalize
+// one pipelined FP32 iteration: one launch that finalizes itself, or the fused kernel
+// and k_fin_pf (whichever the autotuner timed faster)
 static void launch_eo_pf_iteration(Ctx &c, int nsys, int parity) {
-    const int nb = launch_eo_pf(c, c.tileEo32, nsys, parity);
+    if (c.eoFinFused) {
+        (void)launch_eo_pf(c, c.tileEo32, nsys, parity, 1);
+        return;
+    }
+    const int nb = launch_eo_pf(c, c.tileEo32, nsys, parity, 0);
     hipLaunchKernelGGL(k_fin_pf, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb, c.cg_d,
-                       c.active);
+                       c.active, 1);
 }
 
 // Solve S chi_e = b_e for every system (eta from the Philox streams when gen); chi_e is
@@ -925,7 +1063,8 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         c.err = "even-odd preconditioning needs even Nt and Nx and a fused tile that divides the lattice";
         return YCN_ERR_UNSUPPORTED;
     }
-    const bool mixed = c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfR1 && c.pfA1;
+    const bool mixed = c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfR1 && c.pfA1 &&
+                       c.pfCnt && c.pfGrp;
     const EoArgs A = eo_args(c, 0, 0);
     const int Vh = c.V / 2;
     const dim3 gF((c.V + kNT - 1) / kNT, nsys), gH = half_grid(c, nsys);

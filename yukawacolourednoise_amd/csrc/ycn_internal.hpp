@@ -101,6 +101,10 @@ struct Ctx {
     float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *Aps = nullptr, *xs = nullptr;
     // pipelined even-odd FP32 iteration: the second buffers of r and Ap ([nSys][4][V/2])
     float2 *pfR1 = nullptr, *pfA1 = nullptr;
+    int *pfCnt = nullptr;                 // arrival counters of the in-launch finalize (zero between launches)
+    double2 *pfGrp = nullptr;             // its group sums [nSys][pfGroups][2]
+    int pfGroups = 0;
+    bool eoFinFused = false;              // pipelined iteration finalizes in its own launch
     float *phis = nullptr;
 
     // learned preconditioner
@@ -210,6 +214,7 @@ int to_f32_n(Ctx &c, const double *a, float *b, int64_t n);
 
 // even-odd preconditioned fermion force (evenodd.hip)
 bool eo_available(const Ctx &c);
+int eo_pf_alloc(Ctx &c);
 int eo_autotune(Ctx &c);
 int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase, uint64_t step, SolveStats *st);
 int eo_drift(Ctx &c, const double *phi, double *drift);
