@@ -1,5 +1,6 @@
 """CPU check of the index arithmetic of the fused even-odd kernel (k_eo_normal_fused in
-yukawacolourednoise_amd/csrc/evenodd.hip), transcribed here: for every tile of its
+yukawacolourednoise_amd/csrc/evenodd.hip), transcribed here, with its division-free
+modulo (wrapf) emulated in single precision: for every tile of its
 tables and every lattice the tables accept, including time extents smaller than the
 4-site halo (Nt = 2), every global and LDS access stays in bounds, the parity-split LDS
 slots used for each neighbour hold exactly that neighbour, every stage covers its
@@ -7,15 +8,39 @@ region's sites of the right parity once, and stage 4 writes every even site of t
 lattice once."""
 import itertools
 
+import numpy as np
 import pytest
 
 EO64 = [(8, 32), (16, 16), (4, 64), (4, 32), (2, 32)]
 EO32 = [(8, 32), (16, 32), (8, 64), (16, 16), (4, 64), (2, 32), (32, 32), (16, 64)]
 
 
+def wrapf(a, n):
+    """The kernel's division-free a mod n: a - n floor(fl(fl(a + 1/2) * fl(1/n))), every
+    operation rounded to single precision as on the GPU."""
+    a = np.asarray(a, dtype=np.int64)
+    inv = np.float32(1.0) / np.float32(n)
+    q = np.floor((a.astype(np.float32) + np.float32(0.5)) * inv).astype(np.int64)
+    return a - n * q
+
+
 def wrapm(a, n):
-    m = a % n
+    m = int(wrapf(a, n))
+    assert m == a % n
     return m
+
+
+def test_wrapf_is_exact_modulo():
+    """Every extent up to 4096 over the coordinate range the kernel wraps (a tile's halo
+    reaches 4 sites past either end, also for extents below 4), and large extents near
+    the 2^21 limit the library accepts."""
+    for n in range(1, 4097):
+        a = np.arange(-4 * n - 8, 2 * n + 40)
+        assert np.array_equal(wrapf(a, n), a % n), n
+    rng = np.random.default_rng(0)
+    for n in rng.integers(4097, 1 << 21, 200):
+        a = np.concatenate([np.arange(-8, 40), np.arange(n - 40, n + 40), rng.integers(-(1 << 21), 1 << 22, 4000)])
+        assert np.array_equal(wrapf(a, int(n)), a % int(n)), n
 
 
 def slot_col(row, k, par):
@@ -64,8 +89,9 @@ def check(Nt, Nx, TT, TX):
                 assert slot_col(row, ixf % K, q) == col + 1 and ixf // K == row
                 assert slot_col(row, ixb % K, q) == col - 1 and ixb // K == row
                 cells.add((row, col))
-                if hh == 0:
-                    gt, gx = wrapm(t0 - 4 + row, Nt), wrapm(x0 - 4 + col, Nx)
+                if hh == 0:   # the tile's own sites: written without a wrap
+                    gt, gx = t0 - 4 + row, x0 - 4 + col
+                    assert 0 <= gt < Nt and 0 <= gx < Nx
                     j = gt * Nh + (gx >> 1)
                     assert 0 <= j < Vh and j not in out
                     out.add(j)

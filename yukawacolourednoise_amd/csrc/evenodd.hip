@@ -39,6 +39,7 @@ struct EoArgs {
     int Nt, Nx, Nh, V, Vh, nNoise;   // Nh = Nx / 2, Vh = V / 2
     double r, M, g;
     int tilesX, tilesPerSys, xcdRemap;
+    float invNt, invNx;              // 1 / Nt, 1 / Nx in single precision (wrapf)
 };
 
 // site (t, x) of compact index j on parity par
@@ -48,11 +49,19 @@ __device__ __forceinline__ void eo_site(const EoArgs &A, int par, int j, int &t,
 }
 __device__ __forceinline__ int eo_j(const EoArgs &A, int t, int x) { return t * A.Nh + (x >> 1); }
 
-// a mod n in [0, n) for any a: the 4-site halo of a tile spans several periods when an
-// extent is smaller than 4 (Nt = 2)
-__device__ __forceinline__ int wrapm(int a, int n) {
-    const int m = a % n;
-    return m < 0 ? m + n : m;
+// a mod n in [0, n) without an integer division (the 4-site halo of a tile spans several
+// periods when an extent is smaller than 4, Nt = 2).  For |a| < 2^22 and inv = fl(1 / n),
+// (a + 1/2) / n lies at least 1 / (2n) from an integer and the two single-precision
+// roundings move it by less than |a + 1/2| / n * 2^-23, so the floor is exact.
+__device__ __forceinline__ int wrapf(int a, int n, float inv) {
+    return a - n * (int)floorf(((float)a + 0.5f) * inv);
+}
+// element idx of a buffer through a 32-bit byte offset (the buffers of one system are
+// below 4 GiB), so that the load or store takes a scalar base and a vector offset
+template <typename T>
+__device__ __forceinline__ T &at(T *base, unsigned idx) {
+    using C = typename std::conditional<std::is_const<T>::value, const char, char>::type;
+    return *reinterpret_cast<T *>(reinterpret_cast<C *>(base) + (unsigned)(idx * (unsigned)sizeof(T)));
 }
 
 // Hopping (K for DAG, H otherwise) at site (t, x); ld(t', x', c) returns component c of
@@ -470,7 +479,17 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     if (st[sys].done) return;
     const int nTiles = A.tilesPerSys;
     const int Nt = A.Nt, Nx = A.Nx, Nh = A.Nh;
-    const size_t Vh = A.Vh, off = (size_t)sys * 4 * Vh;
+    const unsigned Vh = (unsigned)A.Vh;
+    const size_t off = (size_t)sys * 4 * A.Vh;   // this system's vectors
+    r += off;
+    pold += off;
+    pnew += off;
+    Ap += off;
+    if (PF) {
+        r_out += off;
+        Ap_in += off;
+        xs += off;
+    }
     const R beta = (R)st[sys].beta;
     const R alpha = PF ? (R)st[sys].alpha.x : (R)0;
     const R *ph = phi + (size_t)(sys / A.nNoise) * A.V;
@@ -491,7 +510,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     struct Regs {
         V2 ra[M0][4], pa[M0][4], aa[M0][4], xa[M0][4];
         R fe[M0], fo[M0];
-        size_t jj[M0];
+        unsigned jj[M0];
     };
     auto load0 = [&](int t0, int x0, Regs &G) {
 #pragma unroll
@@ -499,21 +518,21 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
             const int w = threadIdx.x + u * NT;
             if (w < H * K) {
                 const int row = w / K, k = w - row * K;
-                const int gt = wrapm(t0 - 4 + row, Nt);
+                const int gt = wrapf(t0 - 4 + row, Nt, A.invNt);
                 const int colE = 2 * k + (row & 1), colO = 2 * k + 1 - (row & 1);
-                const int gxE = wrapm(x0 - 4 + colE, Nx), gxO = wrapm(x0 - 4 + colO, Nx);
-                G.jj[u] = (size_t)gt * Nh + (gxE >> 1);
+                const int gxE = wrapf(x0 - 4 + colE, Nx, A.invNx), gxO = wrapf(x0 - 4 + colO, Nx, A.invNx);
+                G.jj[u] = (unsigned)(gt * Nh + (gxE >> 1));
                 const bool inner = row >= 4 && row < 4 + TT && colE >= 4 && colE < 4 + TX;
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const size_t i = off + c * Vh + G.jj[u];
-                    G.ra[u][c] = r[i];
-                    G.pa[u][c] = pold[i];
-                    if (PF) G.aa[u][c] = Ap_in[i];
-                    if (PF && inner) G.xa[u][c] = xs[i];
+                    const unsigned i = c * Vh + G.jj[u];
+                    G.ra[u][c] = at(r, i);
+                    G.pa[u][c] = at(pold, i);
+                    if (PF) G.aa[u][c] = at(Ap_in, i);
+                    if (PF && inner) G.xa[u][c] = at(xs, i);
                 }
-                G.fe[u] = ph[(size_t)gt * Nx + gxE];
-                G.fo[u] = ph[(size_t)gt * Nx + gxO];
+                G.fe[u] = at(ph, (unsigned)(gt * Nx + gxE));
+                G.fo[u] = at(ph, (unsigned)(gt * Nx + gxO));
             }
         }
     };
@@ -528,23 +547,25 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                 const int row = w / K, k = w - row * K;
                 const int colE = 2 * k + (row & 1);
                 const bool inner = row >= 4 && row < 4 + TT && colE >= 4 && colE < 4 + TX;
+                R rs = 0;   // |r'|^2 of the site (4 terms in R, then accumulated in FP64)
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
-                    const size_t i = off + c * Vh + G.jj[u];
+                    const unsigned i = c * Vh + G.jj[u];
                     V2 a = G.ra[u][c];
                     const V2 b = G.pa[u][c];
                     if (PF) a = mk2<R>(a.x - alpha * G.aa[u][c].x, a.y - alpha * G.aa[u][c].y);
                     const V2 pv = mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
                     E[c][w] = pv;
                     if (inner) {
-                        pnew[i] = pv;
+                        at(pnew, i) = pv;
                         if (PF) {
-                            r_out[i] = a;
-                            xs[i] = mk2<R>(G.xa[u][c].x + alpha * b.x, G.xa[u][c].y + alpha * b.y);
-                            rho += (double)(a.x * a.x + a.y * a.y);
+                            at(r_out, i) = a;
+                            at(xs, i) = mk2<R>(G.xa[u][c].x + alpha * b.x, G.xa[u][c].y + alpha * b.y);
+                            rs += a.x * a.x + a.y * a.y;
                         }
                     }
                 }
+                if (PF && inner) rho += (double)rs;
                 FE[w] = d0 + g * G.fe[u];
                 FO[w] = d0 + g * G.fo[u];
             }
@@ -577,7 +598,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         // hopping at site q from the other-parity array N
         auto hop_at = [&](auto dag, const Site &q, const V2 (*N)[H * K], V2 o[4]) {
             constexpr bool DAG = decltype(dag)::value;
-            const int gt = wrapm(t0 - 4 + q.row, Nt);
+            const int gt = wrapf(t0 - 4 + q.row, Nt, A.invNt);
             const R sf = (gt == Nt - 1) ? (R)-1 : (R)1, sb = (gt == 0) ? (R)-1 : (R)1;
 #pragma unroll
             for (int f = 0; f < 4; f += 2)
@@ -602,13 +623,15 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
             hop_at(std::integral_constant<bool, true>(), q, O, o);
             const R d = FE[q.i];
             const bool inner = q.row >= 4 && q.row < 4 + TT && q.col >= 4 && q.col < 4 + TX;
+            R vs = 0;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const V2 pv = E[c][q.i];
                 const V2 v = mk2<R>(d * pv.x - o[c].x, d * pv.y - o[c].y);
                 E[c][q.i] = v;
-                if (inner) acc += (double)(v.x * v.x + v.y * v.y);
+                vs += v.x * v.x + v.y * v.y;
             }
+            if (inner) acc += (double)vs;
         }
         __syncthreads();
         // stage 3: W = H v / d on odd sites, 1-site halo
@@ -627,15 +650,17 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
             V2 o[4];
             hop_at(std::integral_constant<bool, false>(), q, O, o);
             const R d = FE[q.i];
-            const int gt = wrapm(t0 - 4 + q.row, Nt), gx = wrapm(x0 - 4 + q.col, Nx);
-            const size_t j = (size_t)gt * Nh + (gx >> 1);
+            // stage-4 sites are the tile's own: no wrap
+            const unsigned j = (unsigned)((t0 - 4 + q.row) * Nh + ((x0 - 4 + q.col) >> 1));
+            R as = 0;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const V2 v = E[c][q.i];
                 const V2 a = mk2<R>(d * v.x - o[c].x, d * v.y - o[c].y);
-                Ap[off + c * Vh + j] = a;
-                if (PF) nu += (double)(a.x * a.x + a.y * a.y);
+                at(Ap, c * Vh + j) = a;
+                as += a.x * a.x + a.y * a.y;
             }
+            if (PF) nu += (double)as;
         }
     };
 
@@ -812,6 +837,8 @@ static EoArgs eo_args(const Ctx &c, int TT, int TX) {
     A.tilesX = TX > 0 ? c.p.Nx / TX : 0;
     A.tilesPerSys = (TT > 0 && TX > 0) ? (c.p.Nt / TT) * (c.p.Nx / TX) : 0;
     A.xcdRemap = (A.tilesPerSys % 8 == 0) ? 1 : 0;
+    A.invNt = 1.0f / (float)A.Nt;
+    A.invNx = 1.0f / (float)A.Nx;
     return A;
 }
 
@@ -831,7 +858,15 @@ int eo_pf_alloc(Ctx &c) {
     return YCN_OK;
 }
 
-bool eo_available(const Ctx &c) { return c.p.Nt % 2 == 0 && c.p.Nx % 2 == 0 && c.tileEo64 >= 0; }
+// even extents; the fused kernels address one system's four FP64 half-lattice
+// components and phi through 32-bit byte offsets and wrap halo coordinates with wrapf
+// (|a| < 2^22)
+static bool eo_shape_ok(const Ctx &c) {
+    return c.p.Nt % 2 == 0 && c.p.Nx % 2 == 0 && (size_t)4 * (c.V / 2) * sizeof(double2) < ((size_t)1 << 32) &&
+           c.p.Nt < (1 << 21) && c.p.Nx < (1 << 21);
+}
+
+bool eo_available(const Ctx &c) { return eo_shape_ok(c) && c.tileEo64 >= 0; }
 
 // buffers of one fused launch (PF: the pipelined iteration's ping-pong pairs and x_s)
 template <typename R>
@@ -941,7 +976,7 @@ static dim3 half_grid(const Ctx &c, int nsys) { return dim3((c.V / 2 + kNT - 1) 
 // the context's buffers and keeps the fastest.  The trial launches only touch CG scratch.
 int eo_autotune(Ctx &c) {
     c.tileEo64 = c.tileEo32 = -1;
-    if (c.p.Nt % 2 || c.p.Nx % 2) return YCN_OK;
+    if (!eo_shape_ok(c)) return YCN_OK;
     hipEvent_t e0, e1;
     YCN_HIP(hipEventCreate(&e0));
     YCN_HIP(hipEventCreate(&e1));
