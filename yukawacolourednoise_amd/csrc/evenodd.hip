@@ -391,24 +391,23 @@ template <int NT, bool COH>
 __device__ void pf_finalize(const double2 *pp, int nb, CgState &S, int *active, double2 *lds0, double2 *lds1,
                             bool commit) {
     const int skip = S.done;
+    // thread t adds entries t, t + NT, t + 2 NT, ...; each round issues all of its U loads
+    // before adding, so the partials' memory latency is paid once per U NT entries
+    constexpr int U = 4;
     double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-    int i = threadIdx.x;
-    for (; i + 3 * NT < nb; i += 4 * NT) {   // four block partials in flight per thread
-        double2 v[8];
+    for (int base = threadIdx.x; base < nb; base += U * NT) {
+        double2 v[2 * U];
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
-            v[2 * u] = ld_part<COH>(pp + 2 * (i + u * NT));
-            v[2 * u + 1] = ld_part<COH>(pp + 2 * (i + u * NT) + 1);
+        for (int u = 0; u < U; ++u) {
+            const int k = base + u * NT;
+            v[2 * u] = k < nb ? ld_part<COH>(pp + 2 * k) : make_double2(0.0, 0.0);
+            v[2 * u + 1] = k < nb ? ld_part<COH>(pp + 2 * k + 1) : make_double2(0.0, 0.0);
         }
 #pragma unroll
-        for (int u = 0; u < 4; ++u) {
+        for (int u = 0; u < U; ++u) {
             a0 = cadd(a0, v[2 * u]);
             a1 = cadd(a1, v[2 * u + 1]);
         }
-    }
-    for (; i < nb; i += NT) {
-        a0 = cadd(a0, ld_part<COH>(pp + 2 * i));
-        a1 = cadd(a1, ld_part<COH>(pp + 2 * i + 1));
     }
     const double2 mn = block_sum2<NT>(a0, lds0);
     const double2 rh = block_sum2<NT>(a1, lds1);
@@ -458,14 +457,20 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // touching the iteration state (the autotuner's timing launches).  cnt holds per
 // system 1 + maxGroups counters, kCntStride ints apart, zero between launches; gpart
 // holds per system maxGroups pairs of group sums.
-template <typename R, int TT, int TX, bool PF, bool PERSIST = false>
+// RU = true (FP64, PF = false): the reliable update's residual instead of an iteration.
+// Stage 0 stages x (r holds x, beta = 0), stage 2 forms res = b - S x with b (Ap_in) read
+// at its own sites, stage 4 writes r = S^T res (Ap) and its FP32 copy (out32); the
+// partials are |res|^2 (part) and |r|^2 (part2).  Systems already final are skipped.
+template <typename R, int TT, int TX, bool PF, bool PERSIST = false, bool RU = false>
 __global__ void __launch_bounds__(eo_threads(TT, TX))
 k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                   const typename Vec2<R>::T *__restrict__ r, typename Vec2<R>::T *__restrict__ r_out,
                   const typename Vec2<R>::T *__restrict__ Ap_in, const typename Vec2<R>::T *__restrict__ pold,
                   typename Vec2<R>::T *__restrict__ pnew, typename Vec2<R>::T *__restrict__ Ap,
                   typename Vec2<R>::T *__restrict__ xs, double2 *__restrict__ part, int finMode, int *cnt,
-                  double2 *gpart, int maxGroups, int *active) {
+                  double2 *gpart, int maxGroups, int *active, float2 *__restrict__ out32,
+                  double2 *__restrict__ part2) {
+    static_assert(!(PF && RU), "the reliable-update residual runs in FP64 only");
     using V2 = typename Vec2<R>::T;
     constexpr int H = TT + 8, K = (TX + 8) / 2, NT = eo_threads(TT, TX);
     constexpr int M0 = (H * K + NT - 1) / NT;
@@ -476,21 +481,24 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     __shared__ double2 red2[NT / kWave > 0 ? NT / kWave : 1];
 
     const int sys = blockIdx.y;
-    if (st[sys].done) return;
+    if (RU ? st[sys].final_ : st[sys].done) return;
     const int nTiles = A.tilesPerSys;
     const int Nt = A.Nt, Nx = A.Nx, Nh = A.Nh;
     const unsigned Vh = (unsigned)A.Vh;
     const size_t off = (size_t)sys * 4 * A.Vh;   // this system's vectors
     r += off;
-    pold += off;
-    pnew += off;
+    if (!RU) {
+        pold += off;
+        pnew += off;
+    }
     Ap += off;
     if (PF) {
         r_out += off;
-        Ap_in += off;
         xs += off;
     }
-    const R beta = (R)st[sys].beta;
+    if (PF || RU) Ap_in += off;
+    if (RU) out32 += off;
+    const R beta = RU ? (R)0 : (R)st[sys].beta;
     const R alpha = PF ? (R)st[sys].alpha.x : (R)0;
     const R *ph = phi + (size_t)(sys / A.nNoise) * A.V;
     const R hp = (R)(0.5 * (A.r + 1.0)), hm = (R)(0.5 * (A.r - 1.0)), hr = (R)(0.5 * A.r), h = (R)0.5;
@@ -527,7 +535,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                 for (int c = 0; c < 4; ++c) {
                     const unsigned i = c * Vh + G.jj[u];
                     G.ra[u][c] = at(r, i);
-                    G.pa[u][c] = at(pold, i);
+                    if (!RU) G.pa[u][c] = at(pold, i);
                     if (PF) G.aa[u][c] = at(Ap_in, i);
                     if (PF && inner) G.xa[u][c] = at(xs, i);
                 }
@@ -552,11 +560,11 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                 for (int c = 0; c < 4; ++c) {
                     const unsigned i = c * Vh + G.jj[u];
                     V2 a = G.ra[u][c];
-                    const V2 b = G.pa[u][c];
+                    const V2 b = RU ? a : G.pa[u][c];
                     if (PF) a = mk2<R>(a.x - alpha * G.aa[u][c].x, a.y - alpha * G.aa[u][c].y);
-                    const V2 pv = mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
+                    const V2 pv = RU ? a : mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
                     E[c][w] = pv;
-                    if (inner) {
+                    if (inner && !RU) {
                         at(pnew, i) = pv;
                         if (PF) {
                             at(r_out, i) = a;
@@ -623,11 +631,18 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
             hop_at(std::integral_constant<bool, true>(), q, O, o);
             const R d = FE[q.i];
             const bool inner = q.row >= 4 && q.row < 4 + TT && q.col >= 4 && q.col < 4 + TX;
+            unsigned jb = 0;   // RU: this site's entry of b
+            if (RU)
+                jb = (unsigned)(wrapf(t0 - 4 + q.row, Nt, A.invNt) * Nh + (wrapf(x0 - 4 + q.col, Nx, A.invNx) >> 1));
             R vs = 0;
 #pragma unroll
             for (int c = 0; c < 4; ++c) {
                 const V2 pv = E[c][q.i];
-                const V2 v = mk2<R>(d * pv.x - o[c].x, d * pv.y - o[c].y);
+                V2 v = mk2<R>(d * pv.x - o[c].x, d * pv.y - o[c].y);
+                if (RU) {
+                    const V2 bb = at(Ap_in, c * Vh + jb);
+                    v = mk2<R>(bb.x - v.x, bb.y - v.y);
+                }
                 E[c][q.i] = v;
                 vs += v.x * v.x + v.y * v.y;
             }
@@ -658,9 +673,10 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                 const V2 v = E[c][q.i];
                 const V2 a = mk2<R>(d * v.x - o[c].x, d * v.y - o[c].y);
                 at(Ap, c * Vh + j) = a;
+                if (RU) at(out32, c * Vh + j) = make_float2((float)a.x, (float)a.y);
                 as += a.x * a.x + a.y * a.y;
             }
-            if (PF) nu += (double)as;
+            if (PF || RU) nu += (double)as;
         }
     };
 
@@ -737,17 +753,30 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         pf_finalize<NT, true>(gpart + 2 * (size_t)sys * maxGroups, ngrp, st[sys], active, red, red2, finMode == 1);
         if (threadIdx.x == 0) reset_counter(cntSys);
     } else {
-        const double2 tot = block_sum2<NT>(make_double2(acc, 0.0), red);
-        if (threadIdx.x == 0) part[pidx] = tot;
+        const double2 tot = block_sum2<NT>(make_double2(acc, RU ? nu : 0.0), red);
+        if (threadIdx.x == 0) {
+            part[pidx] = make_double2(tot.x, 0.0);
+            if (RU) part2[pidx] = make_double2(tot.y, 0.0);
+        }
     }
 }
 
 // the separate finalize of the pipelined iteration (after a finMode 0 launch): one block
-// per system sums its nb block partials in block order
-__global__ void __launch_bounds__(kNT)
+// of NT threads per system sums its nb block partials
+template <int NT>
+__global__ void __launch_bounds__(NT)
 k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active, int commit) {
-    __shared__ double2 lds0[kNT / kWave], lds1[kNT / kWave];
-    pf_finalize<kNT, false>(part + 2 * (size_t)blockIdx.x * nb, nb, st[blockIdx.x], active, lds0, lds1, commit != 0);
+    __shared__ double2 lds0[NT / kWave], lds1[NT / kWave];
+    pf_finalize<NT, false>(part + 2 * (size_t)blockIdx.x * nb, nb, st[blockIdx.x], active, lds0, lds1, commit != 0);
+}
+// 1024 threads once a system has more partials than four rounds of 256 threads cover
+static void launch_fin_pf(Ctx &c, int nsys, int nb, int commit) {
+    if (nb > 4 * kNT)
+        hipLaunchKernelGGL(k_fin_pf<1024>, dim3(nsys), dim3(1024), 0, c.stream, (const double2 *)c.partials, nb,
+                           c.cg_d, c.active, commit);
+    else
+        hipLaunchKernelGGL(k_fin_pf<kNT>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
+                           c.cg_d, c.active, commit);
 }
 
 // reliable update of the pipelined iteration: x += x_s (FP64), x_s = 0, and the last
@@ -901,19 +930,20 @@ static int launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, b
             int g = std::min(A.tilesPerSys, std::max(resident, 8));
             if (A.xcdRemap && g >= 8) g -= g % 8;   // keeps every block's tiles on one XCD's band
             hipLaunchKernelGGL(k, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold,
-                               b.pnew, b.Ap, b.xs, c.partials, finMode, c.pfCnt, c.pfGrp, c.pfGroups, c.active);
+                               b.pnew, b.Ap, b.xs, c.partials, finMode, c.pfCnt, c.pfGrp, c.pfGroups, c.active,
+                               (float2 *)nullptr, (double2 *)nullptr);
             return g;
         }
         if (pf) {
             hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, true>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A,
                                phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, finMode,
-                               c.pfCnt, c.pfGrp, c.pfGroups, c.active);
+                               c.pfCnt, c.pfGrp, c.pfGroups, c.active, (float2 *)nullptr, (double2 *)nullptr);
             return A.tilesPerSys;
         }
     }
     hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi,
                        c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0, (int *)nullptr,
-                       (double2 *)nullptr, 0, (int *)nullptr);
+                       (double2 *)nullptr, 0, (int *)nullptr, (float2 *)nullptr, (double2 *)nullptr);
     return A.tilesPerSys;
 }
 
@@ -937,6 +967,31 @@ static int launch_eo_fused(Ctx &c, int ti, int nsys, const double *phi, const do
     }
     prof_after(c, PK_NORMAL_OP);
     return A.tilesPerSys;
+}
+
+// the reliable update's residual in one fused FP64 launch (tile of the FP64 table):
+// res = b - S x, r = S^T res (FP64 and FP32 copy), partials |res|^2 and |r|^2 of the
+// systems not yet final; returns the partials per system
+template <int TT, int TX>
+static int launch_eo_ru_tt(Ctx &c, int nsys, const EoArgs &A, const double *phi, const double2 *x, const double2 *b,
+                           double2 *r, float2 *r32, double2 *partRes, double2 *partN) {
+    hipLaunchKernelGGL((k_eo_normal_fused<double, TT, TX, false, false, true>), dim3(A.tilesPerSys, nsys),
+                       dim3(eo_threads(TT, TX)), 0, c.stream, A, phi, c.cg_d, x, (double2 *)nullptr, b,
+                       (const double2 *)nullptr, (double2 *)nullptr, r, (double2 *)nullptr, partRes, 0, (int *)nullptr,
+                       (double2 *)nullptr, 0, (int *)nullptr, r32, partN);
+    return A.tilesPerSys;
+}
+static int launch_eo_ru(Ctx &c, int nsys, const double *phi, const double2 *x, const double2 *b, double2 *r,
+                        float2 *r32, double2 *partRes, double2 *partN) {
+    const EoTileCfg t = kEoTiles64[c.tileEo64];
+    const EoArgs A = eo_args(c, t.TT, t.TX);
+    switch (c.tileEo64) {
+    case 0: return launch_eo_ru_tt<8, 32>(c, nsys, A, phi, x, b, r, r32, partRes, partN);
+    case 1: return launch_eo_ru_tt<16, 16>(c, nsys, A, phi, x, b, r, r32, partRes, partN);
+    case 2: return launch_eo_ru_tt<4, 64>(c, nsys, A, phi, x, b, r, r32, partRes, partN);
+    case 3: return launch_eo_ru_tt<4, 32>(c, nsys, A, phi, x, b, r, r32, partRes, partN);
+    default: return launch_eo_ru_tt<2, 32>(c, nsys, A, phi, x, b, r, r32, partRes, partN);
+    }
 }
 
 // the pipelined FP32 iteration's fused kernel (finMode as in k_eo_normal_fused) with
@@ -1017,11 +1072,7 @@ int eo_autotune(Ctx &c) {
         for (int i = 0; i < kNumEo32; ++i) {
             us32[i][0] = us32[i][1] = 1e30;
             if (!eo_fits(c, kEoTiles32[i])) continue;
-            us32[i][0] = time_it([&](int par) {
-                const int nb = launch_eo_pf(c, i, c.nSys, par, 0);
-                hipLaunchKernelGGL(k_fin_pf, dim3(c.nSys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
-                                   c.cg_d, c.active, 0);
-            });
+            us32[i][0] = time_it([&](int par) { launch_fin_pf(c, c.nSys, launch_eo_pf(c, i, c.nSys, par, 0), 0); });
             us32[i][1] = time_it([&](int par) { (void)launch_eo_pf(c, i, c.nSys, par, 2); });
             std::snprintf(line, sizeof line, " %dx%d%s %.1f|%.1f;", kEoTiles32[i].TT, kEoTiles32[i].TX,
                           kEoTiles32[i].persist ? "p" : "", us32[i][0], us32[i][1]);
@@ -1085,9 +1136,7 @@ static void launch_eo_pf_iteration(Ctx &c, int nsys, int parity) {
         (void)launch_eo_pf(c, c.tileEo32, nsys, parity, 1);
         return;
     }
-    const int nb = launch_eo_pf(c, c.tileEo32, nsys, parity, 0);
-    hipLaunchKernelGGL(k_fin_pf, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb, c.cg_d,
-                       c.active, 1);
+    launch_fin_pf(c, nsys, launch_eo_pf(c, c.tileEo32, nsys, parity, 0), 1);
 }
 
 // Solve S chi_e = b_e for every system (eta from the Philox streams when gen); chi_e is
@@ -1104,7 +1153,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
     const int Vh = c.V / 2;
     const dim3 gF((c.V + kNT - 1) / kNT, nsys), gH = half_grid(c, nsys);
     const int nbF = gF.x, nbH = gH.x;
-    double2 *partE = c.partials, *partH = c.partials + (size_t)nsys * nbF, *partH2 = partH + (size_t)nsys * nbH;
+    double2 *partE = c.partials, *partH = c.partials + (size_t)nsys * nbF;
     const EoBufs B = eo_bufs(c, nsys);
     int rc = set_int(c, c.active, nsys);
     if (rc) return rc;
@@ -1136,6 +1185,8 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
 
     int launched = 0, active = nsys;
     if (mixed) {
+        // reliable-update partials: at most V / 64 blocks per system each (smallest FP64 tile)
+        double2 *partRu = c.partials, *partRu2 = c.partials + (size_t)nsys * ((c.V + 63) / 64);
         const int every = std::max(1, c.p.reliableEvery);
         const int nIt = every + (every & 1);
         const int hardCap = maxIter + nIt + 8;
@@ -1150,14 +1201,9 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             // copy into r[0])
             hipLaunchKernelGGL(k_pf_reliable, gH, dim3(kNT), 0, c.stream, Vh, (const CgState *)c.cg_d, c.x, c.xs,
                                (const float2 *)c.ps[1], c.ps[0]);
-            hipLaunchKernelGGL(k_eo_hop_oe<true>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x, B.U);
-            hipLaunchKernelGGL((k_eo_hop_eo<true, 1>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x,
-                               (const double2 *)B.U, (const double2 *)B.b, B.res, (float2 *)nullptr, partH);
-            hipLaunchKernelGGL(k_eo_hop_oe<false>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.res, B.U);
-            hipLaunchKernelGGL((k_eo_hop_eo<false, 2>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.res,
-                               (const double2 *)B.U, (const double2 *)nullptr, c.r, c.rs, partH2);
-            hipLaunchKernelGGL(k_fin_eo_pf_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partH,
-                               (const double2 *)partH2, nbH, c.cg_d, c.active);
+            const int nbR = launch_eo_ru(c, nsys, phi, c.x, B.b, c.r, c.rs, partRu, partRu2);
+            hipLaunchKernelGGL(k_fin_eo_pf_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partRu,
+                               (const double2 *)partRu2, nbR, c.cg_d, c.active);
             YCN_HIP(hipGetLastError());
             rc = poll_active(c, active);
             if (rc) return rc;
