@@ -650,6 +650,7 @@ int cg_alloc(Ctx &c) {
     YCN_HIP(hipMalloc(&c.partials, sizeof(double2) * (size_t)std::max(c.nSys, c.nRep) * c.maxBlocks));
     YCN_HIP(hipMalloc(&c.active, sizeof(int)));
     YCN_HIP(hipHostMalloc((void **)&c.active_h, sizeof(int), hipHostMallocDefault));
+    YCN_HIP(hipHostMalloc((void **)&c.cg_h, sizeof(CgState) * c.nSys, hipHostMallocDefault));
     return YCN_OK;
 }
 
@@ -663,6 +664,8 @@ void cg_free(Ctx &c) {
     if (c.partials) (void)hipFree(c.partials);
     if (c.active) (void)hipFree(c.active);
     if (c.active_h) (void)hipHostFree(c.active_h);
+    if (c.cg_h) (void)hipHostFree(c.cg_h);
+    c.cg_h = nullptr;
     c.cg_d = nullptr;
     c.partials = nullptr;
     c.active = nullptr;
@@ -782,6 +785,18 @@ int poll_active(Ctx &c, int &active) {
     YCN_HIP(hipMemcpyAsync(c.active_h, c.active, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     YCN_HIP(hipStreamSynchronize(c.stream));
     active = *c.active_h;
+    return YCN_OK;
+}
+
+// active count and the largest iteration count of the systems not yet final
+int poll_iters(Ctx &c, int nsys, int &active, int &itMax) {
+    YCN_HIP(hipMemcpyAsync(c.active_h, c.active, sizeof(int), hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipMemcpyAsync(c.cg_h, c.cg_d, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipStreamSynchronize(c.stream));
+    active = *c.active_h;
+    itMax = 0;
+    for (int k = 0; k < nsys; ++k)
+        if (!c.cg_h[k].final_) itMax = std::max(itMax, c.cg_h[k].iter);
     return YCN_OK;
 }
 

@@ -1190,13 +1190,20 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         const int every = std::max(1, c.p.reliableEvery);
         const int nIt = every + (every & 1);
         const int hardCap = maxIter + nIt + 8;
+        int itDone = 0;
         rc = poll_active(c, active);
         if (rc) return rc;
         while (active > 0 && launched < hardCap) {
-            rc = run_chunk(c, phi, nsys, YCN_DAGGER, nIt, 4, c.tileEo32,
+            // chunks of the reliable-update interval until the previous solve's iteration
+            // count is near, then just past it (a multiple of 8), so that few launches run
+            // after the systems have paused
+            int n = nIt;
+            const int rem = c.eoPredIters - itDone + 4;
+            if (c.eoPredIters > 0 && rem < nIt) n = std::max(8, (rem + 7) / 8 * 8);
+            rc = run_chunk(c, phi, nsys, YCN_DAGGER, n, 4, c.tileEo32,
                            [&](int par) { launch_eo_pf_iteration(c, nsys, par); });
             if (rc) return rc;
-            launched += nIt;
+            launched += n;
             // reliable update: x += x_s, p[0] = p[1]; res = b - S x; r = S^T res (FP64, FP32
             // copy into r[0])
             hipLaunchKernelGGL(k_pf_reliable, gH, dim3(kNT), 0, c.stream, Vh, (const CgState *)c.cg_d, c.x, c.xs,
@@ -1205,10 +1212,15 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             hipLaunchKernelGGL(k_fin_eo_pf_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partRu,
                                (const double2 *)partRu2, nbR, c.cg_d, c.active);
             YCN_HIP(hipGetLastError());
-            rc = poll_active(c, active);
+            rc = poll_iters(c, nsys, active, itDone);
             if (rc) return rc;
         }
-        return read_stats(c, nsys, st);
+        SolveStats out;
+        rc = read_stats(c, nsys, &out);
+        if (rc) return rc;
+        c.eoPredIters = out.iters_max;
+        if (st) *st = out;
+        return YCN_OK;
     }
 
     int chunk = std::max(kGraphIters, std::min(c.lastIters, maxIter));

@@ -88,6 +88,8 @@ struct Ctx {
     double2 *partials = nullptr;          // [nSys][maxBlocks]
     int *active = nullptr;                // systems not yet converged
     int *active_h = nullptr;              // pinned host mirror
+    CgState *cg_h = nullptr;              // pinned host mirror of the CG states
+    int eoPredIters = 0;                  // iterations of the last even-odd mixed solve (chunk predictor)
     int maxBlocks = 0;
     int lastIters = 32;                   // predictor for the CG launch chunk
 
@@ -200,6 +202,7 @@ int field_slices(Ctx &c, int rep, double *slices_dev, double *sums_dev);
 int hot_start(Ctx &c, uint64_t initSeed);
 int set_int(Ctx &c, int *ptr, int v);
 int poll_active(Ctx &c, int &active);
+int poll_iters(Ctx &c, int nsys, int &active, int &itMax);
 int read_stats(Ctx &c, int nsys, SolveStats *st);
 // CG building blocks shared with the even-odd solver (cg.hip): finalize alpha / beta from
 // the partials in c.partials, the BLAS-1 updates x += alpha p, r -= alpha Ap on
