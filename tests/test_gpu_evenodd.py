@@ -84,14 +84,14 @@ def test_eo_every_fp64_tile(gpu, tile, monkeypatch):
     assert rel(d, dr) < RTOL, EO_TILES64[tile]
 
 
-FIN_REPORT = {"separate": "separate finalize", "fused": "in-launch finalize"}
+FIN_REPORT = {"separate": "separate finalize", "fused": "prologue finalize"}
 
 
 @pytest.mark.parametrize("fin", ["separate", "fused"])
 @pytest.mark.parametrize("tile", range(len(EO_TILES32)))
 def test_eo_every_fp32_tile(gpu, tile, fin, monkeypatch):
-    """Every FP32 tile, with the finalize as its own kernel and inside the launch (the
-    last block of each group of 64 blocks, then the last group, sums the partials)."""
+    """Every FP32 tile, with the finalize as its own kernel and in the next launch's
+    prologue (every block sums the previous launch's partials)."""
     monkeypatch.setenv("YCN_TILE_EO32", str(tile))
     monkeypatch.setenv("YCN_EO_FIN", fin)
     Y = gpu
@@ -110,10 +110,10 @@ def test_eo_every_fp32_tile(gpu, tile, fin, monkeypatch):
 
 
 @pytest.mark.parametrize("fin", ["separate", "fused"])
-def test_eo_finalize_ragged_groups(gpu, fin, monkeypatch):
-    """104 x 96 with 2 x 32 tiles: 156 blocks per system, arriving in groups of 64, 64
-    and 28, for two noise systems at once; both finalize forms reach the oracle's drift
-    and take the same number of iterations."""
+def test_eo_finalize_many_blocks(gpu, fin, monkeypatch):
+    """104 x 96 with 2 x 32 tiles: 156 block partials per system, more than the 64
+    threads that sum them, for two noise systems at once; both finalize forms reach the
+    oracle's drift."""
     monkeypatch.setenv("YCN_TILE_EO32", "5")
     monkeypatch.setenv("YCN_EO_FIN", fin)
     Y = gpu

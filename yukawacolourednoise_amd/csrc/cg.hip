@@ -926,7 +926,7 @@ int cg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
             const int n = std::min(chunk, hardCap - launched);
             for (int k = 0; k < n; k += kGraphIters) {
                 rc = run_chunk(c, phi, nsys, mtype, kGraphIters, YCN_CG_FP64, c.tile64,
-                               [&](int par) { launch_iteration(c, L, phi, nsys, mtype, par); });
+                               [&](int i) { launch_iteration(c, L, phi, nsys, mtype, i & 1); });
                 if (rc) return rc;
             }
             launched += ((n + kGraphIters - 1) / kGraphIters) * kGraphIters;
@@ -992,8 +992,8 @@ void mp_free(Ctx &c) {
     for (float2 **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (c.phis) { (void)hipFree(c.phis); c.phis = nullptr; }
-    if (c.pfCnt) { (void)hipFree(c.pfCnt); c.pfCnt = nullptr; }
-    if (c.pfGrp) { (void)hipFree(c.pfGrp); c.pfGrp = nullptr; }
+    for (double2 *&slot : c.pfSlot)
+        if (slot) { (void)hipFree(slot); slot = nullptr; }
 }
 
 static void launch_mp_iteration(Ctx &c, int nsys, int mtype, int parity) {
@@ -1047,7 +1047,7 @@ int mp_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
     if (rc) return rc;
     while (active > 0 && launched < hardCap) {
         rc = run_chunk(c, phi, nsys, mtype, nIt, YCN_CG_MIXED, c.tile32,
-                       [&](int par) { launch_mp_iteration(c, nsys, mtype, par); });
+                       [&](int i) { launch_mp_iteration(c, nsys, mtype, i & 1); });
         if (rc) return rc;
         launched += nIt;
         // reliable update: x += xs; rs = b - A x in FP64
@@ -1148,7 +1148,7 @@ int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t
     if (rc) return rc;
     while (active > 0 && launched < hardCap) {
         rc = run_chunk(c, phi, nsys, mtype, nIt, 2 /* flexible */, c.tile32,
-                       [&](int par) { launch_fcg_iteration(c, nsys, mtype, par); });
+                       [&](int i) { launch_fcg_iteration(c, nsys, mtype, i & 1); });
         if (rc) return rc;
         launched += nIt;
         hipLaunchKernelGGL(k_mp_accum, g, dim3(kNT), 0, c.stream, c.V, (const CgState *)c.cg_d, c.x, c.xs);
@@ -1212,7 +1212,7 @@ int train_preconditioner(Ctx &c, uint64_t step0, int nsteps, int pairsPerSolve, 
         int launched = 0, active = 1, taken = 0;
         while (active > 0 && launched < c.p.cgMaxIter) {
             rc = run_chunk(c, phi, 1, YCN_DAGGER, kGraphIters, YCN_CG_FP64, c.tile64,
-                           [&](int par) { launch_iteration(c, L, phi, 1, YCN_DAGGER, par); });
+                           [&](int i) { launch_iteration(c, L, phi, 1, YCN_DAGGER, i & 1); });
             if (rc) return rc;
             launched += kGraphIters;
             if (taken < pairsPerSolve && launched >= (taken + 1) * pred / (pairsPerSolve + 1)) {

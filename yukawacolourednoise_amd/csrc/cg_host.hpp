@@ -6,7 +6,8 @@ namespace ycn {
 
 constexpr int kGraphIters = 8;  // CG iterations per captured graph (even: keeps the p parity)
 
-// Launch n CG iterations (n even keeps the p ping-pong parity).  When graphs are on and
+// Launch n CG iterations, launch(i) for i = 0 .. n - 1 (i & 1 is the ping-pong parity; n
+// even keeps it across chunks).  When graphs are on and
 // no kernel is being profiled, the n iterations are captured once into a hipGraph per
 // (field, systems, operator, n, precision, tile) and replayed; every kernel reads its
 // scalars from device state and exits early for converged systems, so replays need no
@@ -14,7 +15,7 @@ constexpr int kGraphIters = 8;  // CG iterations per captured graph (even: keeps
 template <typename Launch>
 inline int run_chunk(Ctx &c, const void *phi, int nsys, int mtype, int n, int precision, int tile, Launch launch) {
     if (!c.useGraphs || c.profTarget != PK_NONE) {
-        for (int i = 0; i < n; ++i) launch(i & 1);
+        for (int i = 0; i < n; ++i) launch(i);
         YCN_HIP(hipGetLastError());
         return YCN_OK;
     }
@@ -26,7 +27,7 @@ inline int run_chunk(Ctx &c, const void *phi, int nsys, int mtype, int n, int pr
         }
     hipGraph_t g;
     YCN_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
-    for (int i = 0; i < n; ++i) launch(i & 1);
+    for (int i = 0; i < n; ++i) launch(i);
     const hipError_t le = hipGetLastError();
     YCN_HIP(hipStreamEndCapture(c.stream, &g));
     if (le != hipSuccess) {

@@ -288,6 +288,7 @@ k_fin_eo_setup(const double2 *__restrict__ partE, int nbE, const double2 *__rest
     S.beta = 0.0;
     S.alpha = make_double2(0.0, 0.0);
     S.iter = 0;
+    S.iter0 = 0;
     S.restarts = 0;
     S.true_rel = 0.0;
     S.rz = 0.0;
@@ -345,54 +346,11 @@ k_fin_eo_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
 // The diagonal d = 2r + M + g phi is staged per parity for the whole array.  |v|^2 over
 // the tile (= <p', Ap>) is the block's partial.  HBM traffic per launch: read r, p_old
 // (Vh sites each) and phi, write p', Ap: 4 S/2 + P per site of the full lattice.
-// finalize of the pipelined iteration: from (mu, nu) and the exact rho of the iteration
-// just done (block partials pp[2 b], pp[2 b + 1], summed in a fixed order), alpha = rho /
-// mu and beta = (alpha^2 nu - rho) / rho for the next one (a negative prediction, from
-// cancellation, restarts the direction).  Pauses at the stop after an even number of
-// iterations (the last direction applied is then in p[1]), at the cap, or on a
-// non-positive <p, Ap>; a reliable update follows.  commit = false only reduces (timing).
-// device-coherent (agent-scope) store and load of a double2: they bypass the per-XCD L2
-__device__ __forceinline__ void st_coherent(double2 *p, double2 v) {
-    unsigned long long *q = reinterpret_cast<unsigned long long *>(p);
-    __hip_atomic_store(q, (unsigned long long)__double_as_longlong(v.x), __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    __hip_atomic_store(q + 1, (unsigned long long)__double_as_longlong(v.y), __ATOMIC_RELAXED,
-                       __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ double2 ld_coherent(const double2 *p) {
-    const unsigned long long *q = reinterpret_cast<const unsigned long long *>(p);
-    const unsigned long long a = __hip_atomic_load(q, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    const unsigned long long b = __hip_atomic_load(q + 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-    return make_double2(__longlong_as_double((long long)a), __longlong_as_double((long long)b));
-}
-template <bool COH>
-__device__ __forceinline__ double2 ld_part(const double2 *p) {
-    if constexpr (COH) return ld_coherent(p);
-    else return *p;
-}
-// Arrival at a counter after this thread's device-coherent stores: they have completed
-// (vmcnt drained) before the increment, so whoever sees the final count sees them.
-// Only those few words are published; a device-wide fence would write back and
-// invalidate the whole L2 of this XCD.
-__device__ __forceinline__ int arrive(int *p) {
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    __builtin_amdgcn_s_waitcnt(0);
-    __atomic_signal_fence(__ATOMIC_SEQ_CST);
-    return __hip_atomic_fetch_add(p, 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-__device__ __forceinline__ void reset_counter(int *p) {
-    __hip_atomic_store(p, 0, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-}
-// in-launch finalize: blocks arrive in groups of kFinGroup (one partial per lane of the
-// group's last block), then the groups at the system's counter; counters sit kCntStride
-// ints apart so that concurrent groups do not serialise on one memory line
-constexpr int kFinGroup = kWave, kCntStride = 64;
-
-template <int NT, bool COH>
-__device__ void pf_finalize(const double2 *pp, int nb, CgState &S, int *active, double2 *lds0, double2 *lds1,
-                            bool commit) {
-    const int skip = S.done;
-    // thread t adds entries t, t + NT, t + 2 NT, ...; each round issues all of its U loads
-    // before adding, so the partials' memory latency is paid once per U NT entries
+// Sums of one system's nb block partials of the pipelined iteration (pp[2 b] = (mu, nu),
+// pp[2 b + 1] = (rho, 0)) by one block in a fixed order: thread t adds entries t, t + NT,
+// ..., each round issuing its U loads before adding; valid in thread 0.
+template <int NT>
+__device__ void sum_pf_partials(const double2 *pp, int nb, double2 *lds0, double2 *lds1, double2 &mn, double2 &rh) {
     constexpr int U = 4;
     double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
     for (int base = threadIdx.x; base < nb; base += U * NT) {
@@ -400,8 +358,8 @@ __device__ void pf_finalize(const double2 *pp, int nb, CgState &S, int *active, 
 #pragma unroll
         for (int u = 0; u < U; ++u) {
             const int k = base + u * NT;
-            v[2 * u] = k < nb ? ld_part<COH>(pp + 2 * k) : make_double2(0.0, 0.0);
-            v[2 * u + 1] = k < nb ? ld_part<COH>(pp + 2 * k + 1) : make_double2(0.0, 0.0);
+            v[2 * u] = k < nb ? pp[2 * k] : make_double2(0.0, 0.0);
+            v[2 * u + 1] = k < nb ? pp[2 * k + 1] : make_double2(0.0, 0.0);
         }
 #pragma unroll
         for (int u = 0; u < U; ++u) {
@@ -409,29 +367,82 @@ __device__ void pf_finalize(const double2 *pp, int nb, CgState &S, int *active, 
             a1 = cadd(a1, v[2 * u + 1]);
         }
     }
-    const double2 mn = block_sum2<NT>(a0, lds0);
-    const double2 rh = block_sum2<NT>(a1, lds1);
+    mn = block_sum2<NT>(a0, lds0);
+    rh = block_sum2<NT>(a1, lds1);
+}
+
+// The decision after iteration number `iter` of the pipelined iteration, from (mu, nu)
+// and the exact rho of that iteration: alpha = rho / mu and beta = (alpha^2 nu - rho) /
+// rho for the next one (a negative prediction, from cancellation, restarts the
+// direction).  Pause at the stop after an even number of iterations (the last direction
+// applied is then in p[1]), at the cap, or on a non-positive <p, Ap>; a reliable update
+// follows a pause.
+struct PfStep {
+    double alpha, beta;
+    bool stop;
+};
+__device__ __forceinline__ PfStep pf_decide(double mu, double nu, double rho, int iter, const CgState &S) {
+    PfStep d;
+    d.stop = (iter >= S.maxIter) || (sqrt(rho) <= S.stop && !(iter & 1)) || (!(mu > 0.0) && !(iter & 1)) ||
+             !isfinite(mu) || !isfinite(rho);
+    d.alpha = (mu > 0.0) ? rho / mu : 0.0;
+    double rhoNext = d.alpha * d.alpha * nu - rho;
+    if (!(rhoNext > 0.0)) rhoNext = 0.0;
+    d.beta = (rho > 0.0) ? rhoNext / rho : 0.0;
+    return d;
+}
+
+// the separate finalize of one iteration (one block per system, after a finMode 0
+// launch); commit = false only reduces (timing)
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active, int commit) {
+    __shared__ double2 lds0[NT / kWave], lds1[NT / kWave];
+    CgState &S = st[blockIdx.x];
+    const int skip = S.done;
+    double2 mn, rh;
+    sum_pf_partials<NT>(part + 2 * (size_t)blockIdx.x * nb, nb, lds0, lds1, mn, rh);
     if (skip || threadIdx.x) return;
     if (!commit) {
         S.rz = mn.x + mn.y + rh.x;   // a field the even-odd iteration never reads
         return;
     }
-    const double mu = mn.x, nu = mn.y, rho = rh.x;
     S.iter += 1;
+    const PfStep d = pf_decide(mn.x, mn.y, rh.x, S.iter, S);
     S.rr_prev = S.rr;
-    S.rr = rho;
-    const bool stop = (S.iter >= S.maxIter) || (sqrt(rho) <= S.stop && !(S.iter & 1)) ||
-                      (!(mu > 0.0) && !(S.iter & 1)) || !isfinite(mu) || !isfinite(rho);
-    if (stop) {
+    S.rr = rh.x;
+    if (d.stop) {
         S.done = 1;
         atomicSub(active, 1);
         return;
     }
-    const double al = (mu > 0.0) ? rho / mu : 0.0;
-    double rhoNext = al * al * nu - rho;
-    if (!(rhoNext > 0.0)) rhoNext = 0.0;
-    S.alpha = make_double2(al, 0.0);
-    S.beta = (rho > 0.0) ? rhoNext / rho : 0.0;
+    S.alpha = make_double2(d.alpha, 0.0);
+    S.beta = d.beta;
+}
+
+// Finalize at the end of a chunk of n launches that finalized each other in their
+// prologues (finMode 1): launch n - 1's partials in slotLast, launch n - 2's in slotPrev.
+// Systems that paused inside the chunk already hold their state.
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_fin_pf_chunk(const double2 *__restrict__ slotLast, const double2 *__restrict__ slotPrev, int nb, int n,
+               CgState *__restrict__ st, int *__restrict__ active) {
+    __shared__ double2 lds0[NT / kWave], lds1[NT / kWave];
+    CgState &S = st[blockIdx.x];
+    if (S.done) return;
+    double2 mn, rh, mp, rp;
+    sum_pf_partials<NT>(slotLast + 2 * (size_t)blockIdx.x * nb, nb, lds0, lds1, mn, rh);
+    __syncthreads();
+    sum_pf_partials<NT>(slotPrev + 2 * (size_t)blockIdx.x * nb, nb, lds0, lds1, mp, rp);
+    if (threadIdx.x) return;
+    S.iter = S.iter0 + n;
+    const PfStep d = pf_decide(mn.x, mn.y, rh.x, S.iter, S);
+    S.rr_prev = rp.x;
+    S.rr = rh.x;
+    if (d.stop) {
+        S.done = 1;
+        atomicSub(active, 1);
+    }
 }
 
 constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave : tt * tx / 2; }
@@ -450,13 +461,15 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // the tiles blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next tile's stage-0
 // data into registers while it runs stages 1-4 of the current one, so global loads
 // overlap the LDS stages.  One partial per block (its tiles in order).
-// finMode (PF only): 0 = partials only (k_fin_pf follows); 1 = the launch finalizes
-// itself: the last block of each group of kFinGroup blocks sums the group's partials,
-// the last group of each system sums the group sums in group order and runs the
-// finalize, so one launch is one whole iteration; 2 = the same reductions without
-// touching the iteration state (the autotuner's timing launches).  cnt holds per
-// system 1 + maxGroups counters, kCntStride ints apart, zero between launches; gpart
-// holds per system maxGroups pairs of group sums.
+// finMode (PF only): 0 = alpha and beta from the state, partials to part (k_fin_pf
+// follows).  1 = launch idx of a chunk finalizes launch idx - 1 in its prologue: every
+// block sums that launch's partials (prevPart) in the same fixed order and derives the
+// same alpha, beta and stop decision, so no finalize launch runs between iterations; a
+// stopping launch's block 0 records the pause and every block returns.  Launch 0 of a
+// chunk starts from the state, k_fin_pf_chunk closes the chunk, and the state is not
+// written otherwise during a chunk (blocks read iter0, not iter).  The per-block reads
+// of prevPart suit grids of resident blocks (PERSIST).  2 = the prologue's reductions
+// without pausing or writing the state (the autotuner's timing launches).
 // RU = true (FP64, PF = false): the reliable update's residual instead of an iteration.
 // Stage 0 stages x (r holds x, beta = 0), stage 2 forms res = b - S x with b (Ap_in) read
 // at its own sites, stage 4 writes r = S^T res (Ap) and its FP32 copy (out32); the
@@ -467,8 +480,8 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                   const typename Vec2<R>::T *__restrict__ r, typename Vec2<R>::T *__restrict__ r_out,
                   const typename Vec2<R>::T *__restrict__ Ap_in, const typename Vec2<R>::T *__restrict__ pold,
                   typename Vec2<R>::T *__restrict__ pnew, typename Vec2<R>::T *__restrict__ Ap,
-                  typename Vec2<R>::T *__restrict__ xs, double2 *__restrict__ part, int finMode, int *cnt,
-                  double2 *gpart, int maxGroups, int *active, float2 *__restrict__ out32,
+                  typename Vec2<R>::T *__restrict__ xs, double2 *__restrict__ part, int finMode, int idx,
+                  const double2 *__restrict__ prevPart, int *active, float2 *__restrict__ out32,
                   double2 *__restrict__ part2) {
     static_assert(!(PF && RU), "the reliable-update residual runs in FP64 only");
     using V2 = typename Vec2<R>::T;
@@ -498,8 +511,8 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     }
     if (PF || RU) Ap_in += off;
     if (RU) out32 += off;
-    const R beta = RU ? (R)0 : (R)st[sys].beta;
-    const R alpha = PF ? (R)st[sys].alpha.x : (R)0;
+    R beta = RU ? (R)0 : (R)st[sys].beta;
+    R alpha = PF ? (R)st[sys].alpha.x : (R)0;
     const R *ph = phi + (size_t)(sys / A.nNoise) * A.V;
     const R hp = (R)(0.5 * (A.r + 1.0)), hm = (R)(0.5 * (A.r - 1.0)), hr = (R)(0.5 * A.r), h = (R)0.5;
     const R d0 = (R)(2.0 * A.r + A.M), g = (R)A.g;
@@ -680,11 +693,53 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         }
     };
 
+    // finMode 1, 2: finalize launch idx - 1 (its iteration is number iter0 + idx); false:
+    // the system pauses here
+    auto prologue = [&]() -> bool {
+        if (!PF || finMode == 0 || idx == 0) return true;
+        __shared__ double bc[3];   // alpha, beta, rho of launch idx - 1
+        __shared__ int pause;
+        const int nb = (int)gridDim.x;
+        double2 mn, rh;
+        sum_pf_partials<NT>(prevPart + 2 * (size_t)sys * nb, nb, red, red2, mn, rh);
+        if (threadIdx.x == 0) {
+            const PfStep d = pf_decide(mn.x, mn.y, rh.x, st[sys].iter0 + idx, st[sys]);
+            bc[0] = d.alpha;
+            bc[1] = d.beta;
+            bc[2] = rh.x;
+            pause = finMode == 1 && d.stop;
+        }
+        __syncthreads();
+        if (!pause) {
+            alpha = (R)bc[0];
+            beta = (R)bc[1];
+            return true;
+        }
+        if (blockIdx.x == 0) {   // record the pause; rho of the iteration before it
+            double rprev = st[sys].rr;   // iteration iter0 (chunk start) when idx == 1
+            if (idx >= 2) {              // launch idx - 2's partials, in this launch's slot
+                double2 m2, r2;
+                sum_pf_partials<NT>(part + 2 * (size_t)sys * nb, nb, red, red2, m2, r2);
+                rprev = r2.x;
+            }
+            if (threadIdx.x == 0) {
+                CgState &S = st[sys];
+                S.iter = S.iter0 + idx;
+                S.rr_prev = rprev;
+                S.rr = bc[2];
+                S.done = 1;
+                atomicSub(active, 1);
+            }
+        }
+        return false;
+    };
+
     Regs G;
     int t0, x0;
     if (!PERSIST) {
         origin(blockIdx.x, t0, x0);
         load0(t0, x0, G);
+        if (!prologue()) return;
         compute0(G);
         __syncthreads();
         stages(t0, x0);
@@ -692,6 +747,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         int t = blockIdx.x;
         origin(t, t0, x0);
         load0(t0, x0, G);
+        if (!prologue()) return;
         while (t < nTiles) {
             compute0(G);
             __syncthreads();
@@ -712,46 +768,10 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     if (PF) {
         const double2 t1 = block_sum2<NT>(make_double2(acc, nu), red);
         const double2 t2 = block_sum2<NT>(make_double2(rho, 0.0), red2);
-        if (finMode == 0) {
-            if (threadIdx.x == 0) {
-                part[2 * pidx] = t1;
-                part[2 * pidx + 1] = t2;
-            }
-            return;
-        }
-        __shared__ int lastInGroup, lastGroup;
-        const int nb = (int)gridDim.x, grp = (int)blockIdx.x / kFinGroup;
-        const int gsize = min(kFinGroup, nb - grp * kFinGroup), ngrp = (nb + kFinGroup - 1) / kFinGroup;
-        int *cntSys = cnt + (size_t)sys * (1 + maxGroups) * kCntStride;
         if (threadIdx.x == 0) {
-            st_coherent(part + 2 * pidx, t1);
-            st_coherent(part + 2 * pidx + 1, t2);
-            lastInGroup = arrive(cntSys + (1 + grp) * kCntStride) == gsize - 1;
-            lastGroup = 0;
+            part[2 * pidx] = t1;
+            part[2 * pidx + 1] = t2;
         }
-        __syncthreads();
-        if (!lastInGroup) return;
-        if (threadIdx.x < kWave) {   // the group's partials, one per lane, in lane order
-            double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
-            if ((int)threadIdx.x < gsize) {
-                const double2 *q = part + 2 * ((size_t)sys * nb + grp * kFinGroup + threadIdx.x);
-                a0 = ld_coherent(q);
-                a1 = ld_coherent(q + 1);
-            }
-            a0 = make_double2(wave_sum(a0.x), wave_sum(a0.y));
-            a1 = make_double2(wave_sum(a1.x), wave_sum(a1.y));
-            if (threadIdx.x == 0) {
-                double2 *gq = gpart + 2 * ((size_t)sys * maxGroups + grp);
-                st_coherent(gq, a0);
-                st_coherent(gq + 1, a1);
-                reset_counter(cntSys + (1 + grp) * kCntStride);
-                lastGroup = arrive(cntSys) == ngrp - 1;
-            }
-        }
-        __syncthreads();
-        if (!lastGroup) return;
-        pf_finalize<NT, true>(gpart + 2 * (size_t)sys * maxGroups, ngrp, st[sys], active, red, red2, finMode == 1);
-        if (threadIdx.x == 0) reset_counter(cntSys);
     } else {
         const double2 tot = block_sum2<NT>(make_double2(acc, RU ? nu : 0.0), red);
         if (threadIdx.x == 0) {
@@ -761,14 +781,6 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     }
 }
 
-// the separate finalize of the pipelined iteration (after a finMode 0 launch): one block
-// of NT threads per system sums its nb block partials
-template <int NT>
-__global__ void __launch_bounds__(NT)
-k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active, int commit) {
-    __shared__ double2 lds0[NT / kWave], lds1[NT / kWave];
-    pf_finalize<NT, false>(part + 2 * (size_t)blockIdx.x * nb, nb, st[blockIdx.x], active, lds0, lds1, commit != 0);
-}
 // 1024 threads once a system has more partials than four rounds of 256 threads cover
 static void launch_fin_pf(Ctx &c, int nsys, int nb, int commit) {
     if (nb > 4 * kNT)
@@ -828,6 +840,7 @@ k_fin_eo_pf_true(const double2 *__restrict__ partRes, const double2 *__restrict_
     S.alpha = make_double2(0.0, 0.0);
     S.beta = (S.rr_prev > 0.0) ? rn.x / S.rr_prev : 0.0;
     S.rr = rn.x;
+    S.iter0 = S.iter;   // the next chunk's launches count from here
     if (S.mode == YCN_TOL_REL_TRUE && sqrt(rn.x) <= S.stop) S.stop = sqrt(rn.x) * 0.5 * S.tol / S.true_rel;
     if (S.done) {
         S.done = 0;
@@ -873,17 +886,14 @@ static EoArgs eo_args(const Ctx &c, int TT, int TX) {
 
 static bool eo_fits(const Ctx &c, EoTileCfg t) { return c.p.Nt % t.TT == 0 && c.p.Nx % t.TX == 0; }
 
-// arrival counters and group sums of the in-launch finalize, sized for the smallest
-// tile (64 sites, so at most V / 64 blocks per system)
+// the two partial slots of prologue-finalized launches (launch idx writes slot idx & 1
+// and reads the other), sized for the smallest tile (64 sites: at most V / 64 blocks)
 int eo_pf_alloc(Ctx &c) {
-    const int maxBlocks = (c.V + 63) / 64;
-    c.pfGroups = (maxBlocks + kFinGroup - 1) / kFinGroup;
-    const size_t nc = (size_t)c.nSys * (1 + c.pfGroups) * kCntStride;
-    YCN_HIP(hipMalloc(&c.pfCnt, sizeof(int) * nc));
-    YCN_HIP(hipMemset(c.pfCnt, 0, sizeof(int) * nc));
-    const size_t ng = (size_t)c.nSys * c.pfGroups * 2;
-    YCN_HIP(hipMalloc(&c.pfGrp, sizeof(double2) * ng));
-    YCN_HIP(hipMemset(c.pfGrp, 0, sizeof(double2) * ng));
+    const size_t n = (size_t)c.nSys * ((c.V + 63) / 64) * 2;
+    for (double2 *&slot : c.pfSlot) {
+        YCN_HIP(hipMalloc(&slot, sizeof(double2) * n));
+        YCN_HIP(hipMemset(slot, 0, sizeof(double2) * n));
+    }
     return YCN_OK;
 }
 
@@ -906,6 +916,9 @@ struct EoPtrs {
     const V2 *Ap_in = nullptr;
     const V2 *pold = nullptr;
     V2 *pnew = nullptr, *Ap = nullptr, *xs = nullptr;
+    double2 *part = nullptr;            // this launch's block partials
+    const double2 *prevPart = nullptr;  // finMode 1, 2: the previous launch's
+    int idx = 0;                        // launch index in the chunk
 };
 
 // resident blocks of a kernel on the whole device (persistent launches)
@@ -930,20 +943,20 @@ static int launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, b
             int g = std::min(A.tilesPerSys, std::max(resident, 8));
             if (A.xcdRemap && g >= 8) g -= g % 8;   // keeps every block's tiles on one XCD's band
             hipLaunchKernelGGL(k, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold,
-                               b.pnew, b.Ap, b.xs, c.partials, finMode, c.pfCnt, c.pfGrp, c.pfGroups, c.active,
-                               (float2 *)nullptr, (double2 *)nullptr);
+                               b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
+                               (double2 *)nullptr);
             return g;
         }
         if (pf) {
             hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, true>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A,
-                               phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, finMode,
-                               c.pfCnt, c.pfGrp, c.pfGroups, c.active, (float2 *)nullptr, (double2 *)nullptr);
+                               phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, b.part, finMode, b.idx,
+                               b.prevPart, c.active, (float2 *)nullptr, (double2 *)nullptr);
             return A.tilesPerSys;
         }
     }
     hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi,
-                       c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0, (int *)nullptr,
-                       (double2 *)nullptr, 0, (int *)nullptr, (float2 *)nullptr, (double2 *)nullptr);
+                       c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0, 0,
+                       (const double2 *)nullptr, (int *)nullptr, (float2 *)nullptr, (double2 *)nullptr);
     return A.tilesPerSys;
 }
 
@@ -977,8 +990,8 @@ static int launch_eo_ru_tt(Ctx &c, int nsys, const EoArgs &A, const double *phi,
                            double2 *r, float2 *r32, double2 *partRes, double2 *partN) {
     hipLaunchKernelGGL((k_eo_normal_fused<double, TT, TX, false, false, true>), dim3(A.tilesPerSys, nsys),
                        dim3(eo_threads(TT, TX)), 0, c.stream, A, phi, c.cg_d, x, (double2 *)nullptr, b,
-                       (const double2 *)nullptr, (double2 *)nullptr, r, (double2 *)nullptr, partRes, 0, (int *)nullptr,
-                       (double2 *)nullptr, 0, (int *)nullptr, r32, partN);
+                       (const double2 *)nullptr, (double2 *)nullptr, r, (double2 *)nullptr, partRes, 0, 0,
+                       (const double2 *)nullptr, (int *)nullptr, r32, partN);
     return A.tilesPerSys;
 }
 static int launch_eo_ru(Ctx &c, int nsys, const double *phi, const double2 *x, const double2 *b, double2 *r,
@@ -995,11 +1008,16 @@ static int launch_eo_ru(Ctx &c, int nsys, const double *phi, const double2 *x, c
 }
 
 // the pipelined FP32 iteration's fused kernel (finMode as in k_eo_normal_fused) with
-// tile `ti` and ping-pong parity `par`; returns the block partials per system
-static int launch_eo_pf(Ctx &c, int ti, int nsys, int par, int finMode) {
+// tile `ti` as launch idx of a chunk (idx & 1 is the ping-pong parity); returns the block
+// partials per system
+static int launch_eo_pf(Ctx &c, int ti, int nsys, int idx, int finMode) {
     const EoTileCfg t = kEoTiles32[ti];
     const EoArgs A = eo_args(c, t.TT, t.TX);
+    const int par = idx & 1;
     EoPtrs<float> b;
+    b.idx = idx;
+    b.part = finMode ? c.pfSlot[par] : c.partials;
+    b.prevPart = c.pfSlot[par ^ 1];
     b.r = par ? c.pfR1 : c.rs;
     b.r_out = par ? c.rs : c.pfR1;
     b.Ap_in = par ? c.pfA1 : c.Aps;
@@ -1037,11 +1055,11 @@ int eo_autotune(Ctx &c) {
     YCN_HIP(hipEventCreate(&e1));
     YCN_HIP(hipMemsetAsync(c.cg_d, 0, sizeof(CgState) * c.nSys, c.stream));
     const int reps = c.V >= (1 << 16) ? 10 : 3;
-    auto time_it = [&](auto launch) -> double {
+    auto time_it = [&](auto launch) -> double {   // launch(k), k = 0, 1, ... (k & 1: parity)
         launch(0);
         launch(1);
         (void)hipEventRecord(e0, c.stream);
-        for (int i = 0; i < reps; ++i) launch(i & 1);
+        for (int i = 0; i < reps; ++i) launch(2 + i);
         (void)hipEventRecord(e1, c.stream);
         (void)hipEventSynchronize(e1);
         float ms = 0;
@@ -1053,27 +1071,28 @@ int eo_autotune(Ctx &c) {
     double best = 1e30;
     for (int i = 0; i < kNumEo64; ++i) {
         if (!eo_fits(c, kEoTiles64[i])) continue;
-        const double us = time_it([&](int par) {
-            (void)launch_eo_fused(c, i, c.nSys, c.phi[c.cur], c.r, c.pv[par], c.pv[par ^ 1], c.Ap);
+        const double us = time_it([&](int k) {
+            (void)launch_eo_fused(c, i, c.nSys, c.phi[c.cur], c.r, c.pv[k & 1], c.pv[(k & 1) ^ 1], c.Ap);
         });
         std::snprintf(line, sizeof line, " %dx%d %.1f;", kEoTiles64[i].TT, kEoTiles64[i].TX, us);
         c.tuneReport += line;
         if (us < best) { best = us; c.tileEo64 = i; }
     }
-    // FP32 pipelined iteration: each tile with the separate finalize and finalizing in
-    // the launch; YCN_EO_FIN=separate|fused restricts the choice to one form
+    // FP32 pipelined iteration: each tile with the separate finalize and with the
+    // finalize in the next launch's prologue; YCN_EO_FIN=separate|fused restricts the
+    // choice to one form
     const char *fin = std::getenv("YCN_EO_FIN");
     const bool allowSep = !fin || std::strcmp(fin, "fused") != 0;
     const bool allowFused = !fin || std::strcmp(fin, "separate") != 0;
     double us32[kNumEo32][2] = {};
-    if (c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfCnt && c.pfGrp) {
-        c.tuneReport += " | even-odd fp32 pipelined tiles (us/iteration, separate | in-launch finalize):";
+    if (c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfSlot[0] && c.pfSlot[1]) {
+        c.tuneReport += " | even-odd fp32 pipelined tiles (us/iteration, separate | prologue finalize):";
         best = 1e30;
         for (int i = 0; i < kNumEo32; ++i) {
             us32[i][0] = us32[i][1] = 1e30;
             if (!eo_fits(c, kEoTiles32[i])) continue;
-            us32[i][0] = time_it([&](int par) { launch_fin_pf(c, c.nSys, launch_eo_pf(c, i, c.nSys, par, 0), 0); });
-            us32[i][1] = time_it([&](int par) { (void)launch_eo_pf(c, i, c.nSys, par, 2); });
+            us32[i][0] = time_it([&](int k) { launch_fin_pf(c, c.nSys, launch_eo_pf(c, i, c.nSys, k, 0), 0); });
+            us32[i][1] = time_it([&](int k) { (void)launch_eo_pf(c, i, c.nSys, k, 2); });
             std::snprintf(line, sizeof line, " %dx%d%s %.1f|%.1f;", kEoTiles32[i].TT, kEoTiles32[i].TX,
                           kEoTiles32[i].persist ? "p" : "", us32[i][0], us32[i][1]);
             c.tuneReport += line;
@@ -1093,7 +1112,7 @@ int eo_autotune(Ctx &c) {
     }
     if (const char *f = std::getenv("YCN_TILE_EO32")) {
         const int i = std::atoi(f);
-        if (c.rs && c.phis && c.pfR1 && c.pfCnt && c.pfGrp && i >= 0 && i < kNumEo32 && eo_fits(c, kEoTiles32[i])) {
+        if (c.rs && c.phis && c.pfR1 && c.pfSlot[0] && i >= 0 && i < kNumEo32 && eo_fits(c, kEoTiles32[i])) {
             c.tileEo32 = i;
             c.eoFinFused = allowFused && (!allowSep || us32[i][1] < us32[i][0]);
         }
@@ -1102,7 +1121,7 @@ int eo_autotune(Ctx &c) {
     if (c.tileEo32 >= 0) {
         const EoTileCfg t = kEoTiles32[c.tileEo32];
         std::snprintf(line, sizeof line, " | fp32 pipelined iteration: %dx%d%s, %s finalize", t.TT, t.TX,
-                      t.persist ? "p" : "", c.eoFinFused ? "in-launch" : "separate");
+                      t.persist ? "p" : "", c.eoFinFused ? "prologue" : "separate");
         c.tuneReport += line;
     }
     (void)hipEventDestroy(e0);
@@ -1129,14 +1148,20 @@ static void launch_eo_iteration(Ctx &c, int nsys, const double *phi, int parity)
     launch_fin_beta(c, nsys, nbu, false, true);
 }
 
-// one pipelined FP32 iteration: one launch that finalizes itself, or the fused kernel
-// and k_fin_pf (whichever the autotuner timed faster)
-static void launch_eo_pf_iteration(Ctx &c, int nsys, int parity) {
+// launch i of a chunk of the pipelined FP32 iteration: one launch that finalizes its
+// predecessor in the prologue, or the fused kernel and k_fin_pf (whichever the autotuner
+// timed faster)
+static void launch_eo_pf_iteration(Ctx &c, int nsys, int i) {
     if (c.eoFinFused) {
-        (void)launch_eo_pf(c, c.tileEo32, nsys, parity, 1);
+        c.eoPfBlocks = launch_eo_pf(c, c.tileEo32, nsys, i, 1);
         return;
     }
-    launch_fin_pf(c, nsys, launch_eo_pf(c, c.tileEo32, nsys, parity, 0), 1);
+    launch_fin_pf(c, nsys, launch_eo_pf(c, c.tileEo32, nsys, i, 0), 1);
+}
+// after a chunk of n prologue-finalized launches (n even: launch n - 1 wrote slot 1)
+static void launch_fin_pf_chunk(Ctx &c, int nsys, int n) {
+    hipLaunchKernelGGL(k_fin_pf_chunk<kNT>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.pfSlot[(n - 1) & 1],
+                       (const double2 *)c.pfSlot[n & 1], c.eoPfBlocks, n, c.cg_d, c.active);
 }
 
 // Solve S chi_e = b_e for every system (eta from the Philox streams when gen); chi_e is
@@ -1148,7 +1173,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         return YCN_ERR_UNSUPPORTED;
     }
     const bool mixed = c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfR1 && c.pfA1 &&
-                       c.pfCnt && c.pfGrp;
+                       c.pfSlot[0] && c.pfSlot[1];
     const EoArgs A = eo_args(c, 0, 0);
     const int Vh = c.V / 2;
     const dim3 gF((c.V + kNT - 1) / kNT, nsys), gH = half_grid(c, nsys);
@@ -1201,8 +1226,9 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             const int rem = c.eoPredIters - itDone + 4;
             if (c.eoPredIters > 0 && rem < nIt) n = std::max(8, (rem + 7) / 8 * 8);
             rc = run_chunk(c, phi, nsys, YCN_DAGGER, n, 4, c.tileEo32,
-                           [&](int par) { launch_eo_pf_iteration(c, nsys, par); });
+                           [&](int i) { launch_eo_pf_iteration(c, nsys, i); });
             if (rc) return rc;
+            if (c.eoFinFused) launch_fin_pf_chunk(c, nsys, n);
             launched += n;
             // reliable update: x += x_s, p[0] = p[1]; res = b - S x; r = S^T res (FP64, FP32
             // copy into r[0])
@@ -1230,7 +1256,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             const int n = std::min(chunk, hardCap - launched);
             for (int k = 0; k < n; k += kGraphIters) {
                 rc = run_chunk(c, phi, nsys, YCN_DAGGER, kGraphIters, 3, c.tileEo64,
-                               [&](int par) { launch_eo_iteration(c, nsys, phi, par); });
+                               [&](int i) { launch_eo_iteration(c, nsys, phi, i & 1); });
                 if (rc) return rc;
             }
             launched += ((n + kGraphIters - 1) / kGraphIters) * kGraphIters;

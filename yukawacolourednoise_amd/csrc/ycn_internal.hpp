@@ -46,6 +46,7 @@ struct CgState {
     int precOff;       // flexible CG: 1 = preconditioner switched off (plain CG fallback)
     int restartNext;   // flexible CG: next beta = 0
     double enorm2;     // even-odd solve: |eta|^2 of the unpreconditioned system
+    int iter0;         // pipelined even-odd iteration: iterations done when the chunk began
 };
 
 // Per-replica Langevin state (device resident).
@@ -103,10 +104,9 @@ struct Ctx {
     float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *Aps = nullptr, *xs = nullptr;
     // pipelined even-odd FP32 iteration: the second buffers of r and Ap ([nSys][4][V/2])
     float2 *pfR1 = nullptr, *pfA1 = nullptr;
-    int *pfCnt = nullptr;                 // arrival counters of the in-launch finalize (zero between launches)
-    double2 *pfGrp = nullptr;             // its group sums [nSys][pfGroups][2]
-    int pfGroups = 0;
-    bool eoFinFused = false;              // pipelined iteration finalizes in its own launch
+    double2 *pfSlot[2] = {nullptr, nullptr};   // block partials of prologue-finalized launches
+    bool eoFinFused = false;              // pipelined iteration finalizes in the next launch's prologue
+    int eoPfBlocks = 0;                   // blocks per system of those launches (same tile throughout)
     float *phis = nullptr;
 
     // learned preconditioner
