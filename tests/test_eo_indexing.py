@@ -60,6 +60,7 @@ def site_of(w, hh, par, TX, K):
 
 def check(Nt, Nx, TT, TX):
     H, K = TT + 8, (TX + 8) // 2
+    HO = H - 2   # the odd-parity arrays hold rows 1 .. H - 2: odd entry (row, k) at (row - 1) K + k
     Nh, Vh = Nx // 2, Nt * Nx // 2
     out = set()
     for ty, tx in itertools.product(range(Nt // TT), range(Nx // TX)):
@@ -73,6 +74,8 @@ def check(Nt, Nx, TT, TX):
             assert (gt + gxE) % 2 == 0
             assert 0 <= gt * Nh + (gxE >> 1) < Vh
             assert 0 <= gt * Nx + gxE < Nt * Nx and 0 <= gt * Nx + gxO < Nt * Nx
+            if 1 <= row <= HO:   # stage 0 stores the odd diagonal of rows 1 .. H - 2
+                assert 0 <= w - K < HO * K
         for hh, par in [(3, 1), (2, 0), (1, 1), (0, 0)]:
             n = (TT + 2 * hh) * ((TX + 2 * hh) // 2)
             cells = set()
@@ -82,6 +85,11 @@ def check(Nt, Nx, TT, TX):
                 assert slot_col(row, col >> 1, par) == col and i == row * K + (col >> 1)
                 for idx in (i, itf, itb, ixf, ixb):
                     assert 0 <= idx < H * K
+                # odd-parity accesses (the site itself in stages 1 and 3, the neighbours in
+                # stages 2 and 4) land in the odd arrays' rows 1 .. H - 2
+                odd = (i,) if par == 1 else (itf, itb, ixf, ixb)
+                for idx in odd:
+                    assert 0 <= idx - K < HO * K
                 # neighbour slots of the other parity hold (row +- 1, col), (row, col +- 1)
                 q = 1 - par
                 assert slot_col(row + 1, itf % K, q) == col and itf // K == row + 1

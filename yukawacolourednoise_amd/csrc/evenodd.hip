@@ -487,9 +487,12 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     using V2 = typename Vec2<R>::T;
     constexpr int H = TT + 8, K = (TX + 8) / 2, NT = eo_threads(TT, TX);
     constexpr int M0 = (H * K + NT - 1) / NT;
+    // O and FO hold only array rows 1 .. H - 2 (odd sites are needed on the 3-site halo at
+    // most): odd-site entry (row, k) sits at (row - 1) K + k, 7-8 % less LDS per block
+    constexpr int HO = H - 2;
     __shared__ V2 E[4][H * K];
-    __shared__ V2 O[4][H * K];
-    __shared__ R FE[H * K], FO[H * K];
+    __shared__ V2 O[4][HO * K];
+    __shared__ R FE[H * K], FO[HO * K];
     __shared__ double2 red[NT / kWave > 0 ? NT / kWave : 1];
     __shared__ double2 red2[NT / kWave > 0 ? NT / kWave : 1];
 
@@ -588,7 +591,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                 }
                 if (PF && inner) rho += (double)rs;
                 FE[w] = d0 + g * G.fe[u];
-                FO[w] = d0 + g * G.fo[u];
+                if (row >= 1 && row <= HO) FO[w - K] = d0 + g * G.fo[u];
             }
         }
     };
@@ -616,32 +619,40 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     double acc = 0.0, nu = 0.0;
     // stages 1-4 of the tile at (t0, x0); stage 0 must be in LDS
     auto stages = [&](int t0, int x0) {
-        // hopping at site q from the other-parity array N
-        auto hop_at = [&](auto dag, const Site &q, const V2 (*N)[H * K], V2 o[4]) {
+        // hopping at site q from the other-parity array N (component stride NS, entries
+        // shifted down by SH: 0 for E, K for O)
+        auto hop_at = [&](auto dag, auto ns, auto sh, const Site &q, const V2 *N, V2 o[4]) {
             constexpr bool DAG = decltype(dag)::value;
+            constexpr int NS = decltype(ns)::value, SH = decltype(sh)::value;
             const int gt = wrapf(t0 - 4 + q.row, Nt, A.invNt);
             const R sf = (gt == Nt - 1) ? (R)-1 : (R)1, sb = (gt == 0) ? (R)-1 : (R)1;
+            const int itf = q.itf - SH, itb = q.itb - SH, ixf = q.ixf - SH, ixb = q.ixb - SH;
 #pragma unroll
-            for (int f = 0; f < 4; f += 2)
-                hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, N[f][q.itf], N[f + 1][q.itf], sf, N[f][q.itb],
-                                          N[f + 1][q.itb], sb, N[f][q.ixf], N[f + 1][q.ixf], N[f][q.ixb],
-                                          N[f + 1][q.ixb], o[f], o[f + 1]);
+            for (int f = 0; f < 4; f += 2) {
+                const V2 *N0 = N + f * NS, *N1 = N + (f + 1) * NS;
+                hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, N0[itf], N1[itf], sf, N0[itb], N1[itb], sb, N0[ixf], N1[ixf],
+                                          N0[ixb], N1[ixb], o[f], o[f + 1]);
+            }
         };
+        using FromE = std::integral_constant<int, H * K>;
+        using FromO = std::integral_constant<int, HO * K>;
+        using Sh0 = std::integral_constant<int, 0>;
+        using ShK = std::integral_constant<int, K>;
         // stage 1: U = K p' / d on odd sites, 3-site halo
         for (int w = threadIdx.x; w < (TT + 6) * ((TX + 6) / 2); w += NT) {
             const Site q = site_of(w, 3, 1);
             V2 o[4];
-            hop_at(std::integral_constant<bool, true>(), q, E, o);
-            const R inv = (R)1 / FO[q.i];
+            hop_at(std::true_type(), FromE(), Sh0(), q, &E[0][0], o);
+            const R inv = (R)1 / FO[q.i - K];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) O[c][q.i] = mk2<R>(o[c].x * inv, o[c].y * inv);
+            for (int c = 0; c < 4; ++c) O[c][q.i - K] = mk2<R>(o[c].x * inv, o[c].y * inv);
         }
         __syncthreads();
         // stage 2: v = d p' - K U on even sites, 2-site halo; |v|^2 over the tile
         for (int w = threadIdx.x; w < (TT + 4) * ((TX + 4) / 2); w += NT) {
             const Site q = site_of(w, 2, 0);
             V2 o[4];
-            hop_at(std::integral_constant<bool, true>(), q, O, o);
+            hop_at(std::true_type(), FromO(), ShK(), q, &O[0][0], o);
             const R d = FE[q.i];
             const bool inner = q.row >= 4 && q.row < 4 + TT && q.col >= 4 && q.col < 4 + TX;
             unsigned jb = 0;   // RU: this site's entry of b
@@ -666,17 +677,17 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         for (int w = threadIdx.x; w < (TT + 2) * ((TX + 2) / 2); w += NT) {
             const Site q = site_of(w, 1, 1);
             V2 o[4];
-            hop_at(std::integral_constant<bool, false>(), q, E, o);
-            const R inv = (R)1 / FO[q.i];
+            hop_at(std::false_type(), FromE(), Sh0(), q, &E[0][0], o);
+            const R inv = (R)1 / FO[q.i - K];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) O[c][q.i] = mk2<R>(o[c].x * inv, o[c].y * inv);
+            for (int c = 0; c < 4; ++c) O[c][q.i - K] = mk2<R>(o[c].x * inv, o[c].y * inv);
         }
         __syncthreads();
         // stage 4: Ap = d v - H W on the tile's even sites (PF: and |Ap|^2)
         for (int w = threadIdx.x; w < TT * (TX / 2); w += NT) {
             const Site q = site_of(w, 0, 0);
             V2 o[4];
-            hop_at(std::integral_constant<bool, false>(), q, O, o);
+            hop_at(std::false_type(), FromO(), ShK(), q, &O[0][0], o);
             const R d = FE[q.i];
             // stage-4 sites are the tile's own: no wrap
             const unsigned j = (unsigned)((t0 - 4 + q.row) * Nh + ((x0 - 4 + q.col) >> 1));
