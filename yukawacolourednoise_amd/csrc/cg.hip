@@ -872,14 +872,15 @@ int cg_autotune(Ctx &c) {
     return YCN_OK;
 }
 
+// the systems' final states into the pinned mirror c.cg_h, and their summary
 int read_stats(Ctx &c, int nsys, SolveStats *st) {
-    std::vector<CgState> S(nsys);
-    YCN_HIP(hipMemcpyAsync(S.data(), c.cg_d, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream));
+    YCN_HIP(hipMemcpyAsync(c.cg_h, c.cg_d, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream));
     YCN_HIP(hipStreamSynchronize(c.stream));
     SolveStats out;
     out.systems = nsys;
     double sum = 0;
-    for (const CgState &s : S) {
+    for (int k = 0; k < nsys; ++k) {
+        const CgState &s = c.cg_h[k];
         out.iters_max = std::max(out.iters_max, s.iter);
         out.restarts = std::max(out.restarts, s.restarts);
         out.converged &= s.converged;

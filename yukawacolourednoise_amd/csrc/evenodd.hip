@@ -269,7 +269,8 @@ __device__ __forceinline__ double2 eo_fin_sum(const double2 *part, int nb, doubl
 // |eta|^2 and |c|^2 = |S^T b|^2: rr, thresholds and run flags of a new solve
 __global__ void __launch_bounds__(kNT)
 k_fin_eo_setup(const double2 *__restrict__ partE, int nbE, const double2 *__restrict__ partC, int nbC,
-               CgState *__restrict__ st, int *__restrict__ active, double tol, int mode, int maxIter, int maxRestarts) {
+               CgState *__restrict__ st, int *__restrict__ active, double tol, int mode, int maxIter, int maxRestarts,
+               double stopScale) {
     __shared__ double2 lds0[kNT / kWave], lds1[kNT / kWave];
     const int sys = blockIdx.x;
     const double2 e = eo_fin_sum(partE + (size_t)sys * nbE, nbE, lds0);
@@ -284,7 +285,9 @@ k_fin_eo_setup(const double2 *__restrict__ partE, int nbE, const double2 *__rest
     S.mode = mode;
     S.maxIter = maxIter;
     S.maxRestarts = maxRestarts;
-    S.stop = (mode == YCN_TOL_ABS_RECURSIVE) ? tol : tol * sqrt(cc.x);
+    S.stop0 = (mode == YCN_TOL_ABS_RECURSIVE) ? tol : tol * sqrt(cc.x);
+    S.stop = (mode == YCN_TOL_REL_TRUE) ? S.stop0 * stopScale : S.stop0;
+    S.need = 0.0;
     S.beta = 0.0;
     S.alpha = make_double2(0.0, 0.0);
     S.iter = 0;
@@ -314,6 +317,7 @@ k_fin_eo_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
     CgState &S = st[sys];
     S.true_rel = S.enorm2 > 0.0 ? sqrt(tot.x / S.enorm2) : sqrt(tot.x);
     if (S.mode != YCN_TOL_REL_TRUE) return;
+    if (S.done && S.true_rel > 0.0) S.need = sqrt(S.rr) * S.tol / S.true_rel;   // paused at the stop
     if (S.true_rel > S.tol && S.iter < S.maxIter && S.restarts < S.maxRestarts) {
         S.restarts += 1;
         S.stop = sqrt(S.rr) * 0.5 * S.tol / S.true_rel;
@@ -838,6 +842,8 @@ k_fin_eo_pf_true(const double2 *__restrict__ partRes, const double2 *__restrict_
     CgState &S = st[sys];
     S.true_rel = S.enorm2 > 0.0 ? sqrt(res.x / S.enorm2) : sqrt(res.x);
     S.restarts += 1;
+    // paused at the stop: the normal-equation threshold the true residual asks for
+    if (S.done && S.true_rel > 0.0) S.need = sqrt(rn.x) * S.tol / S.true_rel;
     const bool conv = (S.mode == YCN_TOL_REL_TRUE) ? (S.true_rel <= S.tol) : (sqrt(rn.x) <= S.stop);
     if (conv || S.iter >= S.maxIter) {
         S.converged = conv ? 1 : 0;
@@ -1175,6 +1181,19 @@ static void launch_fin_pf_chunk(Ctx &c, int nsys, int n) {
                        (const double2 *)c.pfSlot[n & 1], c.eoPfBlocks, n, c.cg_d, c.active);
 }
 
+// Relative mode: the recursive (normal-equation) residual at which the FP64 true
+// residual reaches tol varies little between consecutive solves, so the next solve
+// starts from 0.9 times the largest ratio need / stop0 the systems of this one reported
+// (the first solve starts from tol |r_0|).  A threshold that proves too loose is
+// tightened by the true-residual check as before; one too tight costs a few iterations.
+static void eo_learn_stop_scale(Ctx &c, int nsys) {
+    if (c.p.cgTolMode != YCN_TOL_REL_TRUE) return;
+    double r = 0.0;
+    for (int k = 0; k < nsys; ++k)
+        if (c.cg_h[k].need > 0.0 && c.cg_h[k].stop0 > 0.0) r = std::max(r, c.cg_h[k].need / c.cg_h[k].stop0);
+    if (r > 0.0) c.eoStopScale = std::min(1.0, std::max(1e-3, 0.9 * r));
+}
+
 // Solve S chi_e = b_e for every system (eta from the Philox streams when gen); chi_e is
 // left in c.x, eta in c.eta.  FP64 CG, or FP32 inner iterations with FP64 reliable
 // updates when cgPrecision is mixed.
@@ -1216,7 +1235,8 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_fin_eo_setup, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partE, nbF,
-                       (const double2 *)partH, nbH, c.cg_d, c.active, c.p.cgTol, c.p.cgTolMode, maxIter, maxRestarts);
+                       (const double2 *)partH, nbH, c.cg_d, c.active, c.p.cgTol, c.p.cgTolMode, maxIter, maxRestarts,
+                       c.eoStopScale);
     YCN_HIP(hipGetLastError());
 
     int launched = 0, active = nsys;
@@ -1256,6 +1276,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         rc = read_stats(c, nsys, &out);
         if (rc) return rc;
         c.eoPredIters = out.iters_max;
+        eo_learn_stop_scale(c, nsys);
         if (st) *st = out;
         return YCN_OK;
     }
@@ -1287,7 +1308,10 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         if (active == 0 || launched >= hardCap) break;
         chunk = kGraphIters;
     }
-    return read_stats(c, nsys, st);
+    rc = read_stats(c, nsys, st);
+    if (rc) return rc;
+    eo_learn_stop_scale(c, nsys);
+    return YCN_OK;
 }
 
 int eo_drift(Ctx &c, const double *phi, double *drift) {

@@ -47,6 +47,8 @@ struct CgState {
     int restartNext;   // flexible CG: next beta = 0
     double enorm2;     // even-odd solve: |eta|^2 of the unpreconditioned system
     int iter0;         // pipelined even-odd iteration: iterations done when the chunk began
+    double stop0;      // even-odd solve: the unscaled starting threshold tol |r_0|
+    double need;       // even-odd solve: threshold the true residual asked for at the last check at the stop
 };
 
 // Per-replica Langevin state (device resident).
@@ -91,6 +93,7 @@ struct Ctx {
     int *active_h = nullptr;              // pinned host mirror
     CgState *cg_h = nullptr;              // pinned host mirror of the CG states
     int eoPredIters = 0;                  // iterations of the last even-odd mixed solve (chunk predictor)
+    double eoStopScale = 1.0;             // starting threshold of even-odd solves / tol |r_0| (relative mode)
     int maxBlocks = 0;
     int lastIters = 32;                   // predictor for the CG launch chunk
 
