@@ -651,6 +651,7 @@ int cg_alloc(Ctx &c) {
     YCN_HIP(hipMalloc(&c.active, sizeof(int)));
     YCN_HIP(hipHostMalloc((void **)&c.active_h, sizeof(int), hipHostMallocDefault));
     YCN_HIP(hipHostMalloc((void **)&c.cg_h, sizeof(CgState) * c.nSys, hipHostMallocDefault));
+    YCN_HIP(hipEventCreateWithFlags(&c.pollEv, hipEventDisableTiming));
     return YCN_OK;
 }
 
@@ -666,6 +667,8 @@ void cg_free(Ctx &c) {
     if (c.active_h) (void)hipHostFree(c.active_h);
     if (c.cg_h) (void)hipHostFree(c.cg_h);
     c.cg_h = nullptr;
+    if (c.pollEv) (void)hipEventDestroy(c.pollEv);
+    c.pollEv = nullptr;
     c.cg_d = nullptr;
     c.partials = nullptr;
     c.active = nullptr;
@@ -781,18 +784,33 @@ int to_f32_n(Ctx &c, const double *a, float *b, int64_t n) {
     return YCN_OK;
 }
 
+// Wait for the stream by spinning on an event: a solve polls a few times per Langevin
+// step, and each blocking wait leaves the GPU idle until the host thread is scheduled
+// again.
+int spin_sync(Ctx &c) {
+    YCN_HIP(hipEventRecord(c.pollEv, c.stream));
+    hipError_t e;
+    while ((e = hipEventQuery(c.pollEv)) == hipErrorNotReady) {
+    }
+    YCN_HIP(e);
+    return YCN_OK;
+}
+
 int poll_active(Ctx &c, int &active) {
     YCN_HIP(hipMemcpyAsync(c.active_h, c.active, sizeof(int), hipMemcpyDeviceToHost, c.stream));
-    YCN_HIP(hipStreamSynchronize(c.stream));
+    int rc = spin_sync(c);
+    if (rc) return rc;
     active = *c.active_h;
     return YCN_OK;
 }
 
-// active count and the largest iteration count of the systems not yet final
+// active count and the largest iteration count of the systems not yet final (the
+// states stay in the pinned mirror c.cg_h)
 int poll_iters(Ctx &c, int nsys, int &active, int &itMax) {
     YCN_HIP(hipMemcpyAsync(c.active_h, c.active, sizeof(int), hipMemcpyDeviceToHost, c.stream));
     YCN_HIP(hipMemcpyAsync(c.cg_h, c.cg_d, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream));
-    YCN_HIP(hipStreamSynchronize(c.stream));
+    int rc = spin_sync(c);
+    if (rc) return rc;
     active = *c.active_h;
     itMax = 0;
     for (int k = 0; k < nsys; ++k)
@@ -875,7 +893,13 @@ int cg_autotune(Ctx &c) {
 // the systems' final states into the pinned mirror c.cg_h, and their summary
 int read_stats(Ctx &c, int nsys, SolveStats *st) {
     YCN_HIP(hipMemcpyAsync(c.cg_h, c.cg_d, sizeof(CgState) * nsys, hipMemcpyDeviceToHost, c.stream));
-    YCN_HIP(hipStreamSynchronize(c.stream));
+    int rc = spin_sync(c);
+    if (rc) return rc;
+    return stats_from_mirror(c, nsys, st);
+}
+
+// summary of the states already in the pinned mirror
+int stats_from_mirror(Ctx &c, int nsys, SolveStats *st) {
     SolveStats out;
     out.systems = nsys;
     double sum = 0;

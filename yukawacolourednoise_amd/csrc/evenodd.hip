@@ -1246,9 +1246,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         const int every = std::max(1, c.p.reliableEvery);
         const int nIt = every + (every & 1);
         const int hardCap = maxIter + nIt + 8;
-        int itDone = 0;
-        rc = poll_active(c, active);
-        if (rc) return rc;
+        int itDone = 0;   // systems already converged at setup exit the chunk kernels at once
         while (active > 0 && launched < hardCap) {
             // chunks of the reliable-update interval until the previous solve's iteration
             // count is near, then just past it (a multiple of 8), so that few launches run
@@ -1272,8 +1270,8 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             rc = poll_iters(c, nsys, active, itDone);
             if (rc) return rc;
         }
-        SolveStats out;
-        rc = read_stats(c, nsys, &out);
+        SolveStats out;   // the last poll left the final states in the mirror
+        rc = stats_from_mirror(c, nsys, &out);
         if (rc) return rc;
         c.eoPredIters = out.iters_max;
         eo_learn_stop_scale(c, nsys);

@@ -92,6 +92,7 @@ struct Ctx {
     int *active = nullptr;                // systems not yet converged
     int *active_h = nullptr;              // pinned host mirror
     CgState *cg_h = nullptr;              // pinned host mirror of the CG states
+    hipEvent_t pollEv = nullptr;          // spin-waited by the solvers' polls
     int eoPredIters = 0;                  // iterations of the last even-odd mixed solve (chunk predictor)
     double eoStopScale = 1.0;             // starting threshold of even-odd solves / tol |r_0| (relative mode)
     int maxBlocks = 0;
@@ -206,6 +207,8 @@ int hot_start(Ctx &c, uint64_t initSeed);
 int set_int(Ctx &c, int *ptr, int v);
 int poll_active(Ctx &c, int &active);
 int poll_iters(Ctx &c, int nsys, int &active, int &itMax);
+int spin_sync(Ctx &c);
+int stats_from_mirror(Ctx &c, int nsys, SolveStats *st);
 int read_stats(Ctx &c, int nsys, SolveStats *st);
 // CG building blocks shared with the even-odd solver (cg.hip): finalize alpha / beta from
 // the partials in c.partials, the BLAS-1 updates x += alpha p, r -= alpha Ap on
