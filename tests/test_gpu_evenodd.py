@@ -168,3 +168,29 @@ def test_eo_small_time_extents(gpu, Nt):
             c.langevin_step(step)
             phi, _, _ = O.langevin_step(orc, phi, step)
             assert rel(c.get_field(), phi) < RTOL, step
+
+
+@pytest.mark.parametrize("precision", ["fp64", "mixed"])
+def test_eo_second_solve_starts_from_learned_threshold(gpu, precision):
+    """Relative mode: a context's next solve starts from the recursive threshold its
+    previous solve's true-residual check asked for.  Solving the same system twice, the
+    second solve needs no more true-residual checks than the first, both meet the 1e-10
+    bar, and the second drift still equals the oracle's to the solver accuracy."""
+    Y = gpu
+    Nt, Nx = 64, 64
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.05, g=0.5, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    p.evenOdd = 1
+    if precision == "mixed":
+        p.cgPrecision = Y.YCN_CG_MIXED
+    phi = O.hot_start(3, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        d1, s1 = c.fermion_force(2)
+        d2, s2 = c.fermion_force(2)   # same eta and field: the same system
+    dr, _ = O.fermion_drift(orc, phi, 2)
+    print(f"# {precision}: first {s1['iters']} iterations / {s1['restarts']} checks, "
+          f"second {s2['iters']} / {s2['restarts']}", flush=True)
+    for s in (s1, s2):
+        assert s["converged"] == 1 and s["true_rel_resid"] <= 1e-10, s
+    assert s2["restarts"] <= s1["restarts"]
+    assert rel(d1, dr) < 1e-8 and rel(d2, dr) < 1e-8
