@@ -14,9 +14,12 @@
 // blocks (commented out in Dirac.cu:256-379): site (t, x) of a parity at t * Nx/2 +
 // (x >> 1), SoA planes [sys][c][Vh].  Needs even Nt and Nx.
 //
-// One CG iteration = k_eo_normal_fused (p' = r + beta p, v = S p', Ap = S^T v, |v|^2:
-// four half-lattice hops through LDS) + the BLAS-1 update on Vh sites: 5S + P bytes per
-// site of the full lattice, against 10S + P for the unpreconditioned iteration.
+// FP64: one CG iteration = k_eo_normal_fused (p' = r + beta p, v = S p', Ap = S^T v,
+// |v|^2: four half-lattice hops through LDS) + the BLAS-1 update on Vh sites: 5S + P
+// bytes per site of the full lattice, against 10S + P for the unpreconditioned
+// iteration.  Mixed precision: one launch per pipelined FP32 iteration (132 B per site),
+// its finalize in the next launch's prologue or a separate kernel, and FP64 reliable
+// updates whose residual b - S x and S^T of it take one fused launch.
 #include <algorithm>
 #include <cmath>
 #include <cstdio>
@@ -332,24 +335,8 @@ k_fin_eo_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
 }
 
 // ---------------------------------------------------------------------------
-// fused Schur normal operator: p' = r + beta p_old; v = S p'; Ap = S^T v; |v|^2
+// finalize of the pipelined FP32 iteration
 // ---------------------------------------------------------------------------
-// One block owns a TT x TX tile of the full lattice (TT, TX even) and produces Ap on the
-// tile's TT*TX/2 even sites.  LDS holds the tile plus a 4-site halo, (TT+8) x (TX+8)
-// sites in full-lattice coordinates, split by lattice parity: E[c][row][k] and
-// O[c][row][k] hold the site of that parity in array row `row`, column 2k or 2k+1
-// (k = column >> 1), four SoA planes each.  Lanes of a wave take consecutive k, so every
-// neighbour read (rows row +- 1 at the same k, the same row at k or k +- 1) is a
-// contiguous run: no bank conflicts.  Each stage fills the parity it produces on a halo
-// one site smaller than the stage before:
-//   0: p'  on even sites, 4-site halo (from HBM/L2; p' of the tile also written out)
-//   1: U = K p' / d on odd sites, 3-site halo          (= A_o^-1 K_oe p')
-//   2: v = d p' - K U on even sites, 2-site halo        (= S p', overwrites p')
-//   3: W = H v / d on odd sites, 1-site halo            (overwrites U)
-//   4: Ap = d v - H W on the tile's even sites          (= S^T S p')
-// The diagonal d = 2r + M + g phi is staged per parity for the whole array.  |v|^2 over
-// the tile (= <p', Ap>) is the block's partial.  HBM traffic per launch: read r, p_old
-// (Vh sites each) and phi, write p', Ap: 4 S/2 + P per site of the full lattice.
 // Sums of one system's nb block partials of the pipelined iteration (pp[2 b] = (mu, nu),
 // pp[2 b + 1] = (rho, 0)) by one block in a fixed order: thread t adds entries t, t + NT,
 // ..., each round issuing its U loads before adding; valid in thread 0.
@@ -449,6 +436,25 @@ k_fin_pf_chunk(const double2 *__restrict__ slotLast, const double2 *__restrict__
     }
 }
 
+// ---------------------------------------------------------------------------
+// fused Schur normal operator: p' = r + beta p_old; v = S p'; Ap = S^T v; |v|^2
+// ---------------------------------------------------------------------------
+// One block owns a TT x TX tile of the full lattice (TT, TX even) and produces Ap on the
+// tile's TT*TX/2 even sites.  LDS holds the tile plus a 4-site halo, (TT+8) x (TX+8)
+// sites in full-lattice coordinates, split by lattice parity: E[c][row][k] and
+// O[c][row][k] hold the site of that parity in array row `row`, column 2k or 2k+1
+// (k = column >> 1), four SoA planes each.  Lanes of a wave take consecutive k, so every
+// neighbour read (rows row +- 1 at the same k, the same row at k or k +- 1) is a
+// contiguous run: no bank conflicts.  Each stage fills the parity it produces on a halo
+// one site smaller than the stage before:
+//   0: p'  on even sites, 4-site halo (from HBM/L2; p' of the tile also written out)
+//   1: U = K p' / d on odd sites, 3-site halo          (= A_o^-1 K_oe p')
+//   2: v = d p' - K U on even sites, 2-site halo        (= S p', overwrites p')
+//   3: W = H v / d on odd sites, 1-site halo            (overwrites U)
+//   4: Ap = d v - H W on the tile's even sites          (= S^T S p')
+// The diagonal d = 2r + M + g phi is staged per parity for the whole array.  |v|^2 over
+// the tile (= <p', Ap>) is the block's partial.  HBM traffic per launch: read r, p_old
+// (Vh sites each) and phi, write p', Ap: 4 S/2 + P per site of the full lattice.
 constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave : tt * tx / 2; }
 
 // PF = false: the operator alone, p' = r + beta p_old, partial <p', Ap> = |v|^2.
