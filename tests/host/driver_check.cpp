@@ -38,6 +38,11 @@ struct FakeBackend : ycn::StepBackend {
             m.corr.push_back(std::exp(-0.1 * t) + 1e-3 * (double)c);
         }
         m.trace = 2.0 * (double)c;
+        m.phi2 = 3.0 * (double)c;
+        m.cgSolves = 2;
+        m.cgItersMean = 100.0 + (double)c;
+        m.cgTrueRelMax = 1e-11 * (double)c;
+        m.cgNotConverged = (int)(c % 2);
         log << "measure " << c << "\n";
         return m;
     }

@@ -150,3 +150,6 @@ def test_dataset_reads_driver_output(tmp_path):
     assert np.allclose(D.Sq_t, np.exp(-0.1 * np.arange(Nt))[None, :] + 1e-3 * c[:, None], rtol=1e-5)
     mag, _ = D.compute_mag()
     assert mag == pytest.approx(tr["sigma"].mean())
+    assert D.stats is not None and len(D.stats["phi2"]) == n
+    assert np.allclose(D.stats["phi2"], 3 * c / V, rtol=1e-5)
+    assert np.allclose(D.stats["cg_iters_mean"], 100 + c, rtol=1e-5)

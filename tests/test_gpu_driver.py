@@ -79,6 +79,11 @@ def _check_files(tmp_path, stdout):
     assert np.allclose(D.S_t.mean(axis=1), D.fields_data["sigma"], rtol=1e-4, atol=1e-6)
     mag, err = D.compute_mag()
     assert np.isfinite(mag) and err >= 0
+    # stats.csv: <phi^2> positive, every solve converged to the 1e-10 bar
+    s = D.stats
+    assert s is not None and len(s["phi2"]) == n
+    assert np.all(s["phi2"] > 0) and np.all(s["cg_solves"] >= 1) and np.all(s["cg_iters_mean"] > 0)
+    assert np.all(s["cg_not_converged"] == 0) and np.all(s["cg_true_rel_max"] <= 1e-10)
 
 
 def test_driver_fp64(gpu, tmp_path):

@@ -153,6 +153,16 @@ def test_control_flow_and_files(driver, tmp_path, adaptive):
     first = [float(v) for v in blocks[0].strip("\n").split("\t") if v]
     assert first == pytest.approx([(t + 0.25 * meas[0]) / 8 for t in range(6)], rel=1e-5)
     assert "#Number of measurements: %d" % len(meas) in stdout
+    # stats.csv: time, eps, <phi^2> = phi2 / V, and the backend's CG statistics
+    st = (d / "stats.csv").read_text().split("\n")
+    assert st[0] == "time,eps,phi2,cg_solves,cg_iters_mean,cg_true_rel_max,cg_not_converged"
+    rows = [list(map(float, x.split(","))) for x in st[1:] if x]
+    assert len(rows) == len(meas)
+    times = [r[0] for r in rows]
+    assert all(b > a for a, b in zip(times, times[1:]))
+    for row, cnt in zip(rows, meas):
+        assert row[2] == pytest.approx(3.0 * cnt / 48, rel=1e-5)
+        assert row[3:] == pytest.approx([2, 100.0 + cnt, 1e-11 * cnt, cnt % 2], rel=1e-5)
 
 
 def test_sigusr2_early_finish(driver, tmp_path):

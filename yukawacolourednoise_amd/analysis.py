@@ -206,6 +206,16 @@ def read_traces(path):
     return {"tr": rows[:, 0], "sigma": rows[:, 1], "phi": rows[:, 2]}
 
 
+def read_stats(path):
+    """``stats.csv`` written by the C++ driver beside the reference's files (Langevin
+    time, eps, <phi^2>, and the CG statistics of the solves since the previous
+    measurement) as a dict of arrays."""
+    with open(path) as f:
+        names = f.readline().strip().split(",")
+    rows = np.loadtxt(path, delimiter=",", skiprows=1, ndmin=2)
+    return {n: rows[:, k] for k, n in enumerate(names)}
+
+
 def read_time_slices(path, Nt=None):
     """``slice.dat``: one line of Nt time-slice sums per measurement; an incomplete last
     line (a run ended mid-write) is dropped (analysis/read_in_data.py:65-95)."""
@@ -235,6 +245,8 @@ class Dataset:
         self.fields_data = read_traces(os.path.join(folder, "traces.csv"))
         self.S_t = read_time_slices(os.path.join(folder, "slice.dat"), self.Nt)
         self.Sq_t = read_fermion_correlator(os.path.join(folder, "data.csv"), self.Nt)
+        st = os.path.join(folder, "stats.csv")
+        self.stats = read_stats(st) if os.path.exists(st) else None
         return self
 
     def compute_mag(self):

@@ -157,6 +157,8 @@ int run_langevin(const RunConfig &cfg, StepBackend &B, DriverIO &io) {
     nMeasurements = oldMeasurements;
     elapsedLangevinTime = nMeasurements * cfg.ExportTime;
     std::ofstream timeSliceFile(io.dir + "/" + cfg.timeSliceFileName);
+    std::ofstream statsFile(io.dir + "/stats.csv");
+    statsFile << "time,eps,phi2,cg_solves,cg_iters_mean,cg_true_rel_max,cg_not_converged\n";
     const auto timerStart = std::chrono::high_resolution_clock::now();
     double avg_magnetisation = 0.0;
     double trace = 0.0;   // printed one measurement late, as in main.cu:438 vs :472-482
@@ -195,10 +197,13 @@ int run_langevin(const RunConfig &cfg, StepBackend &B, DriverIO &io) {
         trace = (cfg.yukawa_coupling != 0.0) ? m.trace : 0.0;
         tracefile << (double)trace / vol << "," << (double)(m.magnetisation[0] / vol) << ","
                   << (double)(std::sqrt(sum2) / vol) << "\n";
+        statsFile << elapsedLangevinTime << ',' << h_eps << ',' << m.phi2 / vol << ',' << m.cgSolves << ','
+                  << m.cgItersMean << ',' << m.cgTrueRelMax << ',' << m.cgNotConverged << '\n';
         out.flush();
         datafile.flush();
         tracefile.flush();
         timeSliceFile.flush();
+        statsFile.flush();
     }
     out << "Final abs. magnetisation: " << (double)avg_magnetisation / (double)nMeasurements / (double)vol << std::endl;
     const auto timerStop = std::chrono::high_resolution_clock::now();

@@ -63,6 +63,10 @@ struct Measurement {
     std::vector<double> slices;          // Nt sums over x of phi
     std::vector<double> corr;            // Nt point-source correlator
     double trace = 0;                    // sum_x drift_x / g
+    double phi2 = 0;                     // sum_x phi^2
+    // fermion solves of the Langevin steps since the previous measurement
+    int cgSolves = 0, cgNotConverged = 0;
+    double cgItersMean = 0, cgTrueRelMax = 0;
 };
 
 class StepBackend {
@@ -84,7 +88,9 @@ struct DriverIO {
 extern std::atomic<bool> g_early_finish;   // set by SIGUSR2
 void install_signal_handler();
 
-// Runs main.cu's control flow; returns the number of measurements.
+// Runs main.cu's control flow; returns the number of measurements.  Besides the
+// reference's outputs it writes stats.csv: per measurement the Langevin time, eps,
+// <phi^2> and the CG statistics of the solves since the previous measurement.
 int run_langevin(const RunConfig &cfg, StepBackend &backend, DriverIO &io);
 
 }  // namespace ycn

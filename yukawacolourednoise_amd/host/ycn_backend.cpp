@@ -1,6 +1,8 @@
 // ycn_backend.cpp -- see ycn_backend.hpp.
 #include "ycn_backend.hpp"
 
+#include <algorithm>
+
 #include <sstream>
 
 namespace ycn {
@@ -94,6 +96,10 @@ double YcnBackend::step(uint64_t counter) {
         return ts.eps_last;
     }
     check(ctx_.get(), ycn_langevin_step(ctx_.get(), counter, eps_.data(), &last_), "ycn_langevin_step", true);
+    solves_ += 1;
+    itersSum_ += last_.iters_mean;
+    trueRelMax_ = std::max(trueRelMax_, last_.true_rel_resid);
+    notConverged_ += last_.converged ? 0 : 1;
     return eps_[0];
 }
 
@@ -109,6 +115,13 @@ Measurement YcnBackend::measure(uint64_t counter) {
     check(ctx_.get(), ycn_measure(ctx_.get(), 0, counter, &ob, m.slices.data(), m.corr.data()), "ycn_measure");
     m.magnetisation = {ob.magnetisation};
     m.trace = ob.trace;
+    m.phi2 = ob.phi2;
+    m.cgSolves = solves_;
+    m.cgNotConverged = notConverged_;
+    m.cgItersMean = solves_ ? itersSum_ / solves_ : 0.0;
+    m.cgTrueRelMax = trueRelMax_;
+    solves_ = notConverged_ = 0;
+    itersSum_ = trueRelMax_ = 0.0;
     return m;
 }
 

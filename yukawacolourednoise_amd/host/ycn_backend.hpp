@@ -101,6 +101,9 @@ private:
     ycn_cg_stats last_{};
     std::vector<double> eps_;
     int trainedPairs_ = 0;   // pairs the learned preconditioner has been trained on
+    // solves since the last measurement
+    int solves_ = 0, notConverged_ = 0;
+    double itersSum_ = 0, trueRelMax_ = 0;
 };
 
 }  // namespace ycn
