@@ -80,6 +80,18 @@ def cpu_baseline(gpu_iters_per_solve: float, budget_s: float = 20.0):
     solves = iters / 4
     t_iter = t_cg / (iters + solves)
     t_step = t_noise + t_upd + (gpu_iters_per_solve + 1.0) * t_iter
+    # the same sample on one thread (two 4-iteration solves)
+    O.set_threads(1)
+    t0 = time.perf_counter()
+    O.coloured_noise(p, 0)
+    O.bosonic_update(p, phi, np.zeros((Nt, Nx)), noise)
+    t_fixed1 = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    for _ in range(2):
+        O.cg(q, phi, eta, 1)
+    t_iter1 = (time.perf_counter() - t0) / (8 + 2)
+    O.set_threads(cores)
+    t_step1 = t_fixed1 + (gpu_iters_per_solve + 1.0) * t_iter1
     return {
         "value": 1.0 / t_step,
         "unit": "steps/s",
@@ -89,8 +101,22 @@ def cpu_baseline(gpu_iters_per_solve: float, budget_s: float = 20.0):
                    f"({t_noise:.2f} s), 1 bosonic update ({t_upd:.3f} s), {iters} CG iterations "
                    f"({t_iter * 1e3:.1f} ms/iter incl. its share of true-residual checks); step priced "
                    f"at the GPU's {gpu_iters_per_solve:.0f} iterations/solve + 1"),
-        "cpu": platform.processor() or platform.machine(),
+        "single_thread_value": 1.0 / t_step1,
+        "single_thread_sample": (f"the same on 1 thread: noise + update {t_fixed1:.2f} s, 8 CG iterations "
+                                 f"({t_iter1 * 1e3:.0f} ms/iter)"),
+        "cpu": cpu_model(),
     }
+
+
+def cpu_model():
+    try:
+        with open("/proc/cpuinfo") as f:
+            for line in f:
+                if line.startswith("model name"):
+                    return line.split(":", 1)[1].strip()
+    except OSError:
+        pass
+    return platform.processor() or platform.machine()
 
 
 THERM_STEPS = 125           # shared thermalisation before any method is timed
