@@ -95,13 +95,15 @@ def test_eo_every_fp32_tile(gpu, tile, fin, monkeypatch):
     monkeypatch.setenv("YCN_TILE_EO32", str(tile))
     monkeypatch.setenv("YCN_EO_FIN", fin)
     Y = gpu
-    Nt, Nx = 32, 64
+    Nt, Nx = 32, 128
     p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.1, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
     p.evenOdd = 1
     p.cgPrecision = Y.YCN_CG_MIXED
     phi = O.hot_start(9, Nt, Nx)
+    t = EO_TILES32[tile]
+    name = f"{t[0]}x{t[1]}" + ("p" if "persistent" in t else "")
     with Y.Context(p) as c:
-        assert FIN_REPORT[fin] in c.tuning_report()
+        assert f"fp32 pipelined iteration: {name}, {FIN_REPORT[fin]}" in c.tuning_report()
         c.set_field(phi)
         d, st = c.fermion_force(1)
     dr, _ = O.fermion_drift(orc, phi, 1)
