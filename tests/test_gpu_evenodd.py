@@ -196,3 +196,21 @@ def test_eo_second_solve_starts_from_learned_threshold(gpu, precision):
         assert s["converged"] == 1 and s["true_rel_resid"] <= 1e-10, s
     assert s2["restarts"] <= s1["restarts"]
     assert rel(d1, dr) < 1e-8 and rel(d2, dr) < 1e-8
+
+
+@pytest.mark.parametrize("r", [0.7, 1.0])
+def test_eo_mixed_wilson_parameter(gpu, r):
+    """The pipelined FP32 kernels have a specialised hop for Wilson r = 1 (projector
+    form) and the general one otherwise; both reach the oracle's drift."""
+    Y = gpu
+    Nt, Nx = 32, 64
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, r=r, M=0.1, g=0.5, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    p.evenOdd = 1
+    p.cgPrecision = Y.YCN_CG_MIXED
+    phi = O.hot_start(21, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        d, st = c.fermion_force(4)
+    dr, _ = O.fermion_drift(orc, phi, 4)
+    assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+    assert rel(d, dr) < 1e-8

@@ -63,6 +63,23 @@ __device__ __forceinline__ void hop_flavour_r(R hp, R hm, R hr, R h, V2 tf0, V2 
     o1.y = -(b_hm * Tb1.y + b_hp * Tf1.y + hr * (Xf1.y + Xb1.y) + h * (Xb0.y - Xf0.y));
 }
 
+// hop_flavour_r at Wilson parameter r = 1, where (r -+ gamma_t) / 2 are projectors: the
+// (r - 1) / 2 terms vanish and (r + 1) / 2 = 1, r / 2 = 1 / 2, so each output component
+// takes one time neighbour and half the x-neighbour combination
+template <bool DAG, typename R, typename V2>
+__device__ __forceinline__ void hop_flavour_r1(V2 tf0, V2 tf1, R sf, V2 tb0, V2 tb1, R sb, V2 xf0, V2 xf1, V2 xb0,
+                                               V2 xb1, V2 &o0, V2 &o1) {
+    const V2 Tb0 = DAG ? tf0 : tb0, Tf1 = DAG ? tb1 : tf1;
+    const R sTb = DAG ? sf : sb, sTf = DAG ? sb : sf;
+    const V2 Xf0 = DAG ? xb0 : xf0, Xf1 = DAG ? xb1 : xf1;
+    const V2 Xb0 = DAG ? xf0 : xb0, Xb1 = DAG ? xf1 : xb1;
+    const R hf = (R)0.5;
+    o0.x = -(sTb * Tb0.x + hf * ((Xf0.x + Xb0.x) + (Xb1.x - Xf1.x)));
+    o0.y = -(sTb * Tb0.y + hf * ((Xf0.y + Xb0.y) + (Xb1.y - Xf1.y)));
+    o1.x = -(sTf * Tf1.x + hf * ((Xf1.x + Xb1.x) + (Xb0.x - Xf0.x)));
+    o1.y = -(sTf * Tf1.y + hf * ((Xf1.y + Xb1.y) + (Xb0.y - Xf0.y)));
+}
+
 struct FusedArgs {
     int Nt, Nx, V, nNoise;
     double r, M, g;

@@ -484,7 +484,8 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // Stage 0 stages x (r holds x, beta = 0), stage 2 forms res = b - S x with b (Ap_in) read
 // at its own sites, stage 4 writes r = S^T res (Ap) and its FP32 copy (out32); the
 // partials are |res|^2 (part) and |r|^2 (part2).  Systems already final are skipped.
-template <typename R, int TT, int TX, bool PF, bool PERSIST = false, bool RU = false>
+// WR1 = true: Wilson parameter r = 1 (hop_flavour_r1), chosen at launch when A.r == 1.
+template <typename R, int TT, int TX, bool PF, bool PERSIST = false, bool RU = false, bool WR1 = false>
 __global__ void __launch_bounds__(eo_threads(TT, TX))
 k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                   const typename Vec2<R>::T *__restrict__ r, typename Vec2<R>::T *__restrict__ r_out,
@@ -640,8 +641,12 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
 #pragma unroll
             for (int f = 0; f < 4; f += 2) {
                 const V2 *N0 = N + f * NS, *N1 = N + (f + 1) * NS;
-                hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, N0[itf], N1[itf], sf, N0[itb], N1[itb], sb, N0[ixf], N1[ixf],
-                                          N0[ixb], N1[ixb], o[f], o[f + 1]);
+                if constexpr (WR1)
+                    hop_flavour_r1<DAG, R, V2>(N0[itf], N1[itf], sf, N0[itb], N1[itb], sb, N0[ixf], N1[ixf], N0[ixb],
+                                               N1[ixb], o[f], o[f + 1]);
+                else
+                    hop_flavour_r<DAG, R, V2>(hp, hm, hr, h, N0[itf], N1[itf], sf, N0[itb], N1[itb], sb, N0[ixf],
+                                              N1[ixf], N0[ixb], N1[ixb], o[f], o[f + 1]);
             }
         };
         using FromE = std::integral_constant<int, H * K>;
@@ -955,26 +960,36 @@ static int resident_blocks(const void *kernel, int threads) {
 }
 
 // returns the number of blocks per system (= partial slots per system)
+// the pipelined FP32 iteration's launch (persistent or one block per tile), WR1 as in
+// k_eo_normal_fused; returns the blocks per system
+template <int TT, int TX, bool WR1>
+static int launch_eo_pf_tt(const Ctx &c, int nsys, const EoArgs &A, const float *phi, bool persist,
+                           const EoPtrs<float> &b, int finMode) {
+    const dim3 block(eo_threads(TT, TX));
+    if (persist) {
+        auto *k = &k_eo_normal_fused<float, TT, TX, true, true, false, WR1>;
+        static int resident = resident_blocks(reinterpret_cast<const void *>(k), eo_threads(TT, TX));
+        int g = std::min(A.tilesPerSys, std::max(resident, 8));
+        if (A.xcdRemap && g >= 8) g -= g % 8;   // keeps every block's tiles on one XCD's band
+        hipLaunchKernelGGL(k, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew,
+                           b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
+                           (double2 *)nullptr);
+        return g;
+    }
+    hipLaunchKernelGGL((k_eo_normal_fused<float, TT, TX, true, false, false, WR1>), dim3(A.tilesPerSys, nsys), block, 0,
+                       c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, b.part, finMode,
+                       b.idx, b.prevPart, c.active, (float2 *)nullptr, (double2 *)nullptr);
+    return A.tilesPerSys;
+}
+
 template <typename R, int TT, int TX>
 static int launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, bool pf, bool persist,
                         const EoPtrs<R> &b, int finMode = 0) {
     const dim3 block(eo_threads(TT, TX));
     if constexpr (std::is_same<R, float>::value) {
-        if (pf && persist) {
-            auto *k = &k_eo_normal_fused<R, TT, TX, true, true>;
-            static int resident = resident_blocks(reinterpret_cast<const void *>(k), eo_threads(TT, TX));
-            int g = std::min(A.tilesPerSys, std::max(resident, 8));
-            if (A.xcdRemap && g >= 8) g -= g % 8;   // keeps every block's tiles on one XCD's band
-            hipLaunchKernelGGL(k, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold,
-                               b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
-                               (double2 *)nullptr);
-            return g;
-        }
         if (pf) {
-            hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, true>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A,
-                               phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, b.part, finMode, b.idx,
-                               b.prevPart, c.active, (float2 *)nullptr, (double2 *)nullptr);
-            return A.tilesPerSys;
+            return A.r == 1.0 ? launch_eo_pf_tt<TT, TX, true>(c, nsys, A, phi, persist, b, finMode)
+                              : launch_eo_pf_tt<TT, TX, false>(c, nsys, A, phi, persist, b, finMode);
         }
     }
     hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi,
