@@ -134,14 +134,14 @@ def make_ctx(Y, size, method):
     return Y.Context(p)
 
 
-def thermalise(Y, size, therm, train):
-    """Hot start (Philox, seed 7) and `therm` Langevin steps.  With `train` the steps
+def thermalise(Y, size, therm, train, seed=7):
+    """Hot start (Philox, `seed`) and `therm` Langevin steps.  With `train` the steps
     are run by the learned method's context, which trains its preconditioner online on
     the fermion solves of those steps (its FP64 CG leaves the (r, e) pairs); otherwise a
     mixed-precision context runs them.  Returns (context or None, phi after the steps,
     training stats)."""
     ctx = make_ctx(Y, size, "learned" if train else "mixed")
-    ctx.hot_start(7)
+    ctx.hot_start(seed)
     t0 = time.perf_counter()
     info = None
     if train:
@@ -285,6 +285,7 @@ def main():
     ap.add_argument("--therm", type=int, default=THERM_STEPS,
                     help="untimed thermalisation steps shared by every method")
     ap.add_argument("--method", choices=["fp64", "mixed", "learned", "eo-fp64", "eo-mixed", "all"], default="all")
+    ap.add_argument("--seed", type=int, default=7, help="hot-start seed of the shared starting field")
     ap.add_argument("--profile-kernel", default="cg_normal_op")
     ap.add_argument("--prof-steps", type=int, default=3,
                     help="steps of the profiled pass after the timed region (roofline)")
@@ -310,7 +311,7 @@ def main():
 
     methods = ["learned", "fp64", "mixed", "eo-fp64", "eo-mixed"] if args.method == "all" else [args.method]
     train = "learned" in methods
-    lctx, phi0, train_info, therm_s = thermalise(Y, args.size, args.therm, train)
+    lctx, phi0, train_info, therm_s = thermalise(Y, args.size, args.therm, train, args.seed)
     print(f"# thermalised {args.therm} steps in {therm_s:.1f} s"
           + (f" (training on {train_info['pairs']} pairs, loss {train_info['loss_first']:.3f} -> "
              f"{train_info['loss_last']:.3f})" if train else ""), file=sys.stderr, flush=True)
@@ -347,7 +348,7 @@ def main():
         "vs_baseline": None,
         "dtype": {"fp64": "f64", "mixed": "f32+f64", "learned": "bf16+f32+f64", "eo-fp64": "f64",
                   "eo-mixed": "f32+f64"}[head],
-        "data": f"synthetic (Philox hot start, seed 7, {args.therm} thermalisation steps)",
+        "data": f"synthetic (Philox hot start, seed {args.seed}, {args.therm} thermalisation steps)",
         "config": {
             "workload": f"{args.size}x{args.size} near-critical Langevin step, {cg_desc[head]} on D^dag D "
                         "to FP64 true relative residual 1e-10, 1 noise vector, fixed eps 0.005",

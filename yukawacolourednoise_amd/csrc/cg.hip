@@ -214,9 +214,15 @@ k_fin_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, i
 // |r_prev|^2.  Flexible CG: beta follows from z = CNN(r_true) (k_fin_fcg); the network
 // is switched off, and the direction restarted, when a reliable interval did not
 // reduce the true residual.
+// FCG guard: refRate (< 0) is the decades of true-residual reduction per iteration of
+// the context's last plain solve (0: none known).  From max(100, 0.3 n_plain) iterations
+// on, a flexible solve that has reduced its true residual at less than half that rate
+// switches the network off: a preconditioner that slows CG down costs at most that
+// stretch of iterations.
 template <bool FCG>
 __global__ void __launch_bounds__(kNT)
-k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active) {
+k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active,
+              double refRate) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.x;
     const int skip = st[sys].final_;   // loaded together with the partials
@@ -238,7 +244,12 @@ k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
         // the direction is kept (beta follows from the new z in k_fin_fcg); a reliable
         // interval that did not reduce the true residual switches the network off and
         // restarts the direction (plain CG from here on)
-        if (!(S.true_rel < S.true_prev) && !S.precOff) {
+        bool slow = false;
+        if (refRate < 0.0 && S.true_rel > 0.0) {
+            const double nPlain = log10(S.tol) / refRate;
+            if (S.iter >= fmax(100.0, 0.3 * nPlain)) slow = log10(S.true_rel) / S.iter > 0.5 * refRate;
+        }
+        if ((!(S.true_rel < S.true_prev) || slow) && !S.precOff) {
             S.precOff = 1;
             S.restartNext = 1;
         }
@@ -977,7 +988,13 @@ int cg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
         if (active == 0 || launched >= hardCap) break;
         chunk = kGraphIters;
     }
-    return read_stats(c, nsys, st);
+    SolveStats out;
+    rc = read_stats(c, nsys, &out);
+    if (rc) return rc;
+    if (mtype == YCN_DAGGER && c.p.cgTolMode == YCN_TOL_REL_TRUE && out.converged)
+        c.plainIters = out.iters_max;   // the flexible solver's yardstick
+    if (st) *st = out;
+    return YCN_OK;
 }
 
 // ===========================================================================
@@ -1087,7 +1104,7 @@ int mp_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
                                (const double2 *)c.eta, (const double2 *)c.t, c.r, c.rs, c.partials);
         }
         hipLaunchKernelGGL(k_fin_mp_true<false>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials,
-                           nb, c.cg_d, c.active);
+                           nb, c.cg_d, c.active, 0.0);
         YCN_HIP(hipGetLastError());
         rc = poll_active(c, active);
         if (rc) return rc;
@@ -1180,8 +1197,9 @@ int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t
         hipLaunchKernelGGL(k_true_a<false>, g, dim3(kNT), 0, c.stream, L, phi, (const double2 *)c.x, c.t);
         hipLaunchKernelGGL((k_true_b<true, true>), g, dim3(kNT), 0, c.stream, L, phi, (const CgState *)c.cg_d,
                            (const double2 *)c.eta, (const double2 *)c.t, c.r, c.rs, c.partials);
+        const double refRate = (c.plainIters > 0 && c.p.cgTol > 0.0) ? std::log10(c.p.cgTol) / c.plainIters : 0.0;
         hipLaunchKernelGGL(k_fin_mp_true<true>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nb,
-                           c.cg_d, c.active);
+                           c.cg_d, c.active, refRate);
         nbC = cnn_apply(c, nsys, c.cg_d, c.rs, c.Aps, c.zs, c.partials);
         if (nbC < 0) return nbC;
         hipLaunchKernelGGL(k_fin_fcg<1>, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbC,
@@ -1253,6 +1271,7 @@ int train_preconditioner(Ctx &c, uint64_t step0, int nsteps, int pairsPerSolve, 
         SolveStats st;
         rc = read_stats(c, 1, &st);
         if (rc) return rc;
+        if (st.iters_max > 0) c.plainIters = st.iters_max;   // the flexible solver's yardstick
         // fermion drift and update: the chain advances exactly as a plain step
         rc = fermion_drift(c, phi, c.drift);
         if (rc) return rc;

@@ -484,7 +484,8 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // Stage 0 stages x (r holds x, beta = 0), stage 2 forms res = b - S x with b (Ap_in) read
 // at its own sites, stage 4 writes r = S^T res (Ap) and its FP32 copy (out32); the
 // partials are |res|^2 (part) and |r|^2 (part2).  Systems already final are skipped.
-// WR1 = true: Wilson parameter r = 1 (hop_flavour_r1), chosen at launch when A.r == 1.
+// WR1 = true: Wilson parameter r = 1 (hop_flavour_r1), chosen at launch when A.r == 1 (every
+// precision and mode).
 template <typename R, int TT, int TX, bool PF, bool PERSIST = false, bool RU = false, bool WR1 = false>
 __global__ void __launch_bounds__(eo_threads(TT, TX))
 k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
@@ -992,9 +993,14 @@ static int launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, b
                               : launch_eo_pf_tt<TT, TX, false>(c, nsys, A, phi, persist, b, finMode);
         }
     }
-    hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi,
-                       c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0, 0,
-                       (const double2 *)nullptr, (int *)nullptr, (float2 *)nullptr, (double2 *)nullptr);
+    if (A.r == 1.0)
+        hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false, false, false, true>), dim3(A.tilesPerSys, nsys), block,
+                           0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0,
+                           0, (const double2 *)nullptr, (int *)nullptr, (float2 *)nullptr, (double2 *)nullptr);
+    else
+        hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A,
+                           phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0, 0,
+                           (const double2 *)nullptr, (int *)nullptr, (float2 *)nullptr, (double2 *)nullptr);
     return A.tilesPerSys;
 }
 
@@ -1026,10 +1032,11 @@ static int launch_eo_fused(Ctx &c, int ti, int nsys, const double *phi, const do
 template <int TT, int TX>
 static int launch_eo_ru_tt(Ctx &c, int nsys, const EoArgs &A, const double *phi, const double2 *x, const double2 *b,
                            double2 *r, float2 *r32, double2 *partRes, double2 *partN) {
-    hipLaunchKernelGGL((k_eo_normal_fused<double, TT, TX, false, false, true>), dim3(A.tilesPerSys, nsys),
-                       dim3(eo_threads(TT, TX)), 0, c.stream, A, phi, c.cg_d, x, (double2 *)nullptr, b,
-                       (const double2 *)nullptr, (double2 *)nullptr, r, (double2 *)nullptr, partRes, 0, 0,
-                       (const double2 *)nullptr, (int *)nullptr, r32, partN);
+    auto *k = A.r == 1.0 ? &k_eo_normal_fused<double, TT, TX, false, false, true, true>
+                         : &k_eo_normal_fused<double, TT, TX, false, false, true, false>;
+    hipLaunchKernelGGL(k, dim3(A.tilesPerSys, nsys), dim3(eo_threads(TT, TX)), 0, c.stream, A, phi, c.cg_d, x,
+                       (double2 *)nullptr, b, (const double2 *)nullptr, (double2 *)nullptr, r, (double2 *)nullptr,
+                       partRes, 0, 0, (const double2 *)nullptr, (int *)nullptr, r32, partN);
     return A.tilesPerSys;
 }
 static int launch_eo_ru(Ctx &c, int nsys, const double *phi, const double2 *x, const double2 *b, double2 *r,

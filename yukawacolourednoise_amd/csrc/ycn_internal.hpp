@@ -97,6 +97,7 @@ struct Ctx {
     double eoStopScale = 1.0;             // starting threshold of even-odd solves / tol |r_0| (relative mode)
     int maxBlocks = 0;
     int lastIters = 32;                   // predictor for the CG launch chunk
+    int plainIters = 0;                   // iterations of the last converged plain FP64 fermion solve (relative mode)
 
     // fused normal-operator tiles chosen by the autotuner (-1: three-kernel path)
     int tile64 = -1, tile32 = -1;
