@@ -220,7 +220,7 @@ C4 = dict(Nt=64, Nx=64, nReplicas=64, nNoise=8, lam=0.5, g=0.3, M=0.3, r=1.0, cu
           epsBar=0.01, adaptive=0, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
 
 
-def bench_c4(Y, steps, warmup, therm):
+def bench_c4(Y, steps, warmup, therm, methods=("fp64", "mixed", "eo-fp64", "eo-mixed")):
     """The batched workload: 64 chains of 64^2 in one context, m^2_k = -1 + 1.5 k / 63,
     seeds 0..63, 8 noise vectors per drift estimate solved together (512 systems per
     CG launch), eps = 0.01 fixed.  Reports chain-steps/s per method from a common
@@ -229,7 +229,7 @@ def bench_c4(Y, steps, warmup, therm):
     seeds = list(range(64))
     out = {}
     phi0 = None
-    for method in ["fp64", "mixed", "eo-fp64", "eo-mixed"]:
+    for method in methods:
         p = Y.default_params(**C4)
         p.cgPrecision = Y.YCN_CG_FP64 if method in ("fp64", "eo-fp64") else Y.YCN_CG_MIXED
         p.evenOdd = 1 if method.startswith("eo-") else 0
@@ -250,10 +250,12 @@ def bench_c4(Y, steps, warmup, therm):
         st = ctx.langevin_run(step, steps)
         ctx.sync()
         dt = time.perf_counter() - t0
+        tuning = ctx.tuning_report()
         ctx.close()
         out[method] = {"chain_steps_per_s": 64 * steps / dt, "ms_per_batch_step": 1e3 * dt / steps,
                        "cg_iters_per_solve_max": st["iters"] / max(1, steps),
-                       "cg_iters_per_solve_mean": st["iters_mean"], "true_rel_resid_max": st["true_rel_resid"]}
+                       "cg_iters_per_solve_mean": st["iters_mean"], "true_rel_resid_max": st["true_rel_resid"],
+                       "tuning": tuning}
         print(f"# c4 {method}: {out[method]['chain_steps_per_s']:.1f} chain-steps/s", file=sys.stderr, flush=True)
     head = max(out, key=lambda m: out[m]["chain_steps_per_s"])
     res = {
@@ -306,7 +308,10 @@ def main():
     import yukawacolourednoise_amd as Y
 
     if args.workload == "c4":
-        bench_c4(Y, args.steps, args.warmup, args.therm)
+        if args.method == "learned":
+            sys.exit("the batched workload runs fp64, mixed, eo-fp64 and eo-mixed")
+        bench_c4(Y, args.steps, args.warmup, args.therm,
+                 ("fp64", "mixed", "eo-fp64", "eo-mixed") if args.method == "all" else (args.method,))
         return
 
     methods = ["learned", "fp64", "mixed", "eo-fp64", "eo-mixed"] if args.method == "all" else [args.method]

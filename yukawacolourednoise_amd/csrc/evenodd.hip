@@ -195,8 +195,8 @@ __global__ void __launch_bounds__(kNT) k_eo_zero(int64_t n, double2 *__restrict_
     if (i >= n) return;
     const double2 z = make_double2(0.0, 0.0);
     a[i] = z;
-    b[i] = z;
-    c[i] = z;
+    if (b) b[i] = z;   // b, c may be null (the FP64 solver's directions, unused in mixed precision)
+    if (c) c[i] = z;
 }
 
 __global__ void __launch_bounds__(kNT) k_eo_zero32(int64_t n, float2 *__restrict__ a, float2 *__restrict__ b,
@@ -383,17 +383,9 @@ __device__ __forceinline__ PfStep pf_decide(double mu, double nu, double rho, in
     return d;
 }
 
-// the separate finalize of one iteration (one block per system, after a finMode 0
-// launch); commit = false only reduces (timing)
-template <int NT>
-__global__ void __launch_bounds__(NT)
-k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active, int commit) {
-    __shared__ double2 lds0[NT / kWave], lds1[NT / kWave];
-    CgState &S = st[blockIdx.x];
-    const int skip = S.done;
-    double2 mn, rh;
-    sum_pf_partials<NT>(part + 2 * (size_t)blockIdx.x * nb, nb, lds0, lds1, mn, rh);
-    if (skip || threadIdx.x) return;
+// the finalize of one iteration from its sums (thread 0): iteration count, rho, pause or
+// the next alpha, beta; commit = false only keeps the sums (timing launches)
+__device__ __forceinline__ void pf_commit(CgState &S, double2 mn, double2 rh, int *active, int commit) {
     if (!commit) {
         S.rz = mn.x + mn.y + rh.x;   // a field the even-odd iteration never reads
         return;
@@ -409,6 +401,42 @@ k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int
     }
     S.alpha = make_double2(d.alpha, 0.0);
     S.beta = d.beta;
+}
+
+// the separate finalize of one iteration (one block per system, after a finMode 0
+// launch); commit = false only reduces (timing)
+template <int NT>
+__global__ void __launch_bounds__(NT)
+k_fin_pf(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, int *__restrict__ active, int commit) {
+    __shared__ double2 lds0[NT / kWave], lds1[NT / kWave];
+    CgState &S = st[blockIdx.x];
+    const int skip = S.done;
+    double2 mn, rh;
+    sum_pf_partials<NT>(part + 2 * (size_t)blockIdx.x * nb, nb, lds0, lds1, mn, rh);
+    if (skip || threadIdx.x) return;
+    pf_commit(S, mn, rh, active, commit);
+}
+
+// the same with one wave per system (systems with at most kNT block partials: one or a
+// few loads per lane and a shuffle reduction, no LDS or barriers)
+__global__ void __launch_bounds__(kNT)
+k_fin_pf_wave(const double2 *__restrict__ part, int nb, int nsys, CgState *__restrict__ st, int *__restrict__ active,
+              int commit) {
+    const int sys = blockIdx.x * (kNT / kWave) + (int)threadIdx.x / kWave;
+    const int lane = threadIdx.x & (kWave - 1);
+    if (sys >= nsys) return;
+    CgState &S = st[sys];
+    const int skip = S.done;
+    const double2 *pp = part + 2 * (size_t)sys * nb;
+    double2 a0 = make_double2(0.0, 0.0), a1 = make_double2(0.0, 0.0);
+    for (int k = lane; k < nb; k += kWave) {
+        a0 = cadd(a0, pp[2 * k]);
+        a1 = cadd(a1, pp[2 * k + 1]);
+    }
+    const double2 mn = make_double2(wave_sum(a0.x), wave_sum(a0.y));
+    const double2 rh = make_double2(wave_sum(a1.x), wave_sum(a1.y));
+    if (skip || lane) return;
+    pf_commit(S, mn, rh, active, commit);
 }
 
 // Finalize at the end of a chunk of n launches that finalized each other in their
@@ -463,7 +491,7 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // rho = |r|^2, mu = <p, Ap>, nu = |Ap|^2, alpha_k = rho_k / mu_k and <r_k, Ap_k> = mu_k,
 // rho_{k+1} = alpha_k^2 nu_k - rho_k, so beta_{k+1} is known before r_{k+1} exists and
 // the x, r updates move into the next operator application.  Stage 0 then forms, on its
-// even sites, r' = r - alpha Ap, p' = r' + beta p and (tile only) x += alpha p; the block
+// even sites, r' = r - alpha Ap, p' = r' + beta p and (tile only) x_s += alpha p; the block
 // partials are (mu, nu) and (rho', 0), rho' the exact |r'|^2 that re-anchors the
 // recurrence every iteration.  r, Ap and p ping-pong between two buffers: blocks read the
 // old ones on their halo while other blocks write the new ones on their tiles.
@@ -808,9 +836,13 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     }
 }
 
-// 1024 threads once a system has more partials than four rounds of 256 threads cover
+// one wave per system for few partials, 1024 threads once a system has more partials
+// than four rounds of 256 threads cover
 static void launch_fin_pf(Ctx &c, int nsys, int nb, int commit) {
-    if (nb > 4 * kNT)
+    if (nb <= kNT)
+        hipLaunchKernelGGL(k_fin_pf_wave, dim3((nsys + kNT / kWave - 1) / (kNT / kWave)), dim3(kNT), 0, c.stream,
+                           (const double2 *)c.partials, nb, nsys, c.cg_d, c.active, commit);
+    else if (nb > 4 * kNT)
         hipLaunchKernelGGL(k_fin_pf<1024>, dim3(nsys), dim3(1024), 0, c.stream, (const double2 *)c.partials, nb,
                            c.cg_d, c.active, commit);
     else
@@ -1253,8 +1285,8 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
     hipLaunchKernelGGL((k_eo_hop_eo<false, 2>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.b,
                        (const double2 *)B.U, (const double2 *)nullptr, c.r, mixed ? c.rs : (float2 *)nullptr, partH);
     const int64_t nh = (int64_t)nsys * 4 * Vh;
-    hipLaunchKernelGGL(k_eo_zero, dim3((unsigned)((nh + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, nh, c.x, c.pv[0],
-                       c.pv[1]);
+    hipLaunchKernelGGL(k_eo_zero, dim3((unsigned)((nh + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, nh, c.x,
+                       mixed ? (double2 *)nullptr : c.pv[0], mixed ? (double2 *)nullptr : c.pv[1]);
     if (mixed) {
         hipLaunchKernelGGL(k_eo_zero32, dim3((unsigned)((nh + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, nh, c.xs,
                            c.ps[0], c.Aps);
