@@ -214,3 +214,20 @@ def test_eo_mixed_wilson_parameter(gpu, r):
     dr, _ = O.fermion_drift(orc, phi, 4)
     assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
     assert rel(d, dr) < 1e-8
+
+
+@pytest.mark.parametrize("Nt", [2, 6, 10])
+def test_eo_mixed_small_time_extents(gpu, Nt):
+    """The fastest path on the sweep's small time extents (Nt = 2 .. 10 at Nx = 32):
+    converged pipelined mixed-precision drift against the oracle's FP64 drift."""
+    Y = gpu
+    p, orc = mk(Y, Nt=Nt, Nx=32, M=0.1, g=0.3, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    p.evenOdd = 1
+    p.cgPrecision = Y.YCN_CG_MIXED
+    phi = O.hot_start(13, Nt, 32)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        d, st = c.fermion_force(2)
+    dr, _ = O.fermion_drift(orc, phi, 2)
+    assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+    assert rel(d, dr) < 1e-8
