@@ -85,11 +85,12 @@ __device__ __forceinline__ void eo_hop(const EoArgs &A, int t, int x, Ld ld, V2 
 // setup, Schur blocks for setup and reliable updates, drift (FP64, untiled)
 // ---------------------------------------------------------------------------
 
-// eta generated (GEN) or given; partial |eta|^2 over the full lattice
+// eta generated (GEN) or given; u_o = A_o^-1 eta_o on the odd sites (compact layout);
+// partial |eta|^2 over the full lattice
 template <bool GEN>
 __global__ void __launch_bounds__(kNT)
-k_eo_eta(EoArgs A, const uint64_t *__restrict__ seeds, uint32_t streamBase, uint64_t step, double2 *__restrict__ eta,
-         double2 *__restrict__ part) {
+k_eo_eta(EoArgs A, const uint64_t *__restrict__ seeds, uint32_t streamBase, uint64_t step, const double *__restrict__ phi,
+         double2 *__restrict__ eta, double2 *__restrict__ u, double2 *__restrict__ part) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.y, s = blockIdx.x * kNT + threadIdx.x;
     const size_t V = A.V, off = (size_t)sys * 4 * V;
@@ -106,29 +107,30 @@ k_eo_eta(EoArgs A, const uint64_t *__restrict__ seeds, uint32_t streamBase, uint
         }
 #pragma unroll
         for (int c = 0; c < 4; ++c) acc.x += b[c].x * b[c].x + b[c].y * b[c].y;
+        const int t = s / A.Nx, x = s - t * A.Nx;
+        if ((t + x) & 1) {
+            const double inv = 1.0 / (2.0 * A.r + A.M + A.g * phi[(size_t)(sys / A.nNoise) * V + s]);
+            const size_t j = (size_t)sys * 4 * A.Vh + eo_j(A, t, x);
+#pragma unroll
+            for (int c = 0; c < 4; ++c) u[j + (size_t)c * A.Vh] = make_double2(b[c].x * inv, b[c].y * inv);
+        }
     }
     acc = block_sum2<kNT>(acc, lds);
     if (threadIdx.x == 0) part[(size_t)sys * gridDim.x + blockIdx.x] = acc;
 }
 
-// b_e = eta_e - K_eo A_o^-1 eta_o
+// b_e = eta_e - K_eo u_o with u_o = A_o^-1 eta_o (from k_eo_eta)
 __global__ void __launch_bounds__(kNT)
-k_eo_rhs(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__ eta, double2 *__restrict__ b) {
+k_eo_rhs(EoArgs A, const double2 *__restrict__ eta, const double2 *__restrict__ u, double2 *__restrict__ b) {
     const int sys = blockIdx.y, j = blockIdx.x * kNT + threadIdx.x;
     if (j >= A.Vh) return;
     int t, x;
     eo_site(A, 0, j, t, x);
     const size_t V = A.V, Vh = A.Vh;
     const double2 *e = eta + (size_t)sys * 4 * V;
-    const double *ph = phi + (size_t)(sys / A.nNoise) * V;
-    const double d0 = 2.0 * A.r + A.M;
+    const double2 *uo = u + (size_t)sys * 4 * Vh;
     double2 k[4];
-    eo_hop<true, double, double2>(A, t, x, [&](int tn, int xn, int c) {
-        const size_t fs = (size_t)tn * A.Nx + xn;
-        const double inv = 1.0 / (d0 + A.g * ph[fs]);
-        const double2 v = e[c * V + fs];
-        return make_double2(v.x * inv, v.y * inv);
-    }, k);
+    eo_hop<true, double, double2>(A, t, x, [&](int tn, int xn, int c) { return uo[c * Vh + eo_j(A, tn, xn)]; }, k);
     const size_t fs = (size_t)t * A.Nx + x;
     double2 *bo = b + (size_t)sys * 4 * Vh;
 #pragma unroll
@@ -1275,11 +1277,11 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
     const int maxIter = c.p.cgMaxIter, maxRestarts = mixed ? 1 << 20 : 4;
     if (gen)
         hipLaunchKernelGGL(k_eo_eta<true>, gF, dim3(kNT), 0, c.stream, A, (const uint64_t *)c.seed_d, streamBase, step,
-                           c.eta, partE);
+                           phi, c.eta, B.U, partE);
     else
         hipLaunchKernelGGL(k_eo_eta<false>, gF, dim3(kNT), 0, c.stream, A, (const uint64_t *)c.seed_d, streamBase,
-                           step, c.eta, partE);
-    hipLaunchKernelGGL(k_eo_rhs, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.eta, B.b);
+                           step, phi, c.eta, B.U, partE);
+    hipLaunchKernelGGL(k_eo_rhs, gH, dim3(kNT), 0, c.stream, A, (const double2 *)c.eta, (const double2 *)B.U, B.b);
     // r = S^T b (and its FP32 copy), x = 0, p = 0
     hipLaunchKernelGGL(k_eo_hop_oe<false>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.b, B.U);
     hipLaunchKernelGGL((k_eo_hop_eo<false, 2>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.b,
