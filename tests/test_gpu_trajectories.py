@@ -1,0 +1,99 @@
+"""Whole Langevin trajectories at their full lengths, not single steps.
+
+With the same noise, Langevin dynamics contracts: two runs whose drifts differ at the level
+of the CG tolerance stay together.  The oracle run at tolerance 1e-10 and at 1e-12 differs
+by 2e-12 after 200 steps of the 32^2 workload and by 6e-13 after 2000.  So a GPU trajectory
+can be held to the oracle's over its full length, for every solver.
+
+- 32^2, the reference's own CPU-runnable case: m^2 = -0.2, lambda = 0.5, M = 0.3, g = 0.3,
+  s = 0.5, eps = 0.01 fixed, cold start, seed 1234, 2000 steps, CG to 1e-10.  Each solver
+  (plain FP64, mixed precision, even-odd FP64, even-odd pipelined mixed) is compared with
+  the oracle's plain FP64 trajectory every 500 steps.
+- 512^2 mixed-precision comparison: M = 0.05, g = 0.5, m^2 = -0.4, lambda = 1, s = 0.5,
+  eps = 0.005, hot start (init seed 7), 2000 steps.  FP32 inner iterations with FP64
+  reliable updates, plain and on the even-odd Schur complement, against the FP64 CG
+  trajectory (itself held to the oracle step by step at 256^2 and 1024^2 elsewhere)."""
+import os
+
+import numpy as np
+import pytest
+
+import oracle as O
+from test_gpu_parity import mk
+
+pytestmark = pytest.mark.gpu
+
+C1 = dict(Nt=32, Nx=32, m2=-0.2, lam=0.5, g=0.3, M=0.3, cutFraction=0.5, epsBar=0.01, adaptive=0, seed=1234,
+          cgTol=1e-10, cgTolMode=1, cgMaxIter=10000)
+C3 = dict(Nt=512, Nx=512, M=0.05, g=0.5, m2=-0.4, lam=1.0, cutFraction=0.5, epsBar=0.005, adaptive=0, seed=1234,
+          cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+STEPS, EVERY = 2000, 500
+METHODS = {"fp64": (0, 0), "mixed": (1, 0), "eo-fp64": (0, 1), "eo-mixed": (1, 1)}
+
+
+def l2rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _set_method(Y, p, method):
+    mixed, eo = METHODS[method]
+    p.cgPrecision = Y.YCN_CG_MIXED if mixed else p.cgPrecision
+    p.evenOdd = eo
+
+
+_oracle_c1 = {}
+
+
+def _oracle_snapshots():
+    """the oracle's 32^2 trajectory from a cold start, every EVERY steps (computed once)"""
+    if not _oracle_c1:
+        O.set_threads(min(8, os.cpu_count() or 1))
+        orc = O.params(**C1)
+        phi = np.zeros((C1["Nt"], C1["Nx"]))
+        for step in range(STEPS):
+            phi, _, st = O.langevin_step(orc, phi, step)
+            assert st["converged"] == 1
+            if (step + 1) % EVERY == 0:
+                _oracle_c1[step + 1] = phi.copy()
+    return _oracle_c1
+
+
+@pytest.mark.parametrize("method", list(METHODS))
+def test_32_full_trajectory(gpu, method):
+    Y = gpu
+    print(f"# 32^2 trajectory, {STEPS} steps, {method}", flush=True)
+    ref = _oracle_snapshots()
+    p, _ = mk(Y, **C1)
+    _set_method(Y, p, method)
+    with Y.Context(p) as c:
+        c.set_field(np.zeros((C1["Nt"], C1["Nx"])))
+        for k in range(0, STEPS, EVERY):
+            st = c.langevin_run(k, EVERY)
+            assert st["converged"] == 1, st
+            e = l2rel(c.get_field(), ref[k + EVERY])
+            print(f"#   step {k + EVERY}: |phi - phi_oracle| / |phi_oracle| = {e:.2e}, "
+                  f"{st['iters_mean']:.1f} iterations per solve", flush=True)
+            assert e < 1e-9, (k + EVERY, e)
+
+
+def test_512_mixed_precision_trajectories(gpu):
+    Y = gpu
+    N = C3["Nt"]
+    phi0 = O.hot_start(7, N, N)
+    fields, iters = {}, {}
+    for method in ("fp64", "mixed", "eo-mixed"):
+        print(f"# 512^2 trajectory, {STEPS} steps, {method}", flush=True)
+        p, _ = mk(Y, **C3)
+        _set_method(Y, p, method)
+        with Y.Context(p) as c:
+            c.set_field(phi0)
+            st = c.langevin_run(0, STEPS)
+            assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+            fields[method] = c.get_field()
+            iters[method] = st["iters_mean"]
+        print(f"#   {iters[method]:.1f} iterations per solve", flush=True)
+    assert np.all(np.isfinite(fields["fp64"]))
+    for method in ("mixed", "eo-mixed"):
+        e = l2rel(fields[method], fields["fp64"])
+        print(f"#   {method} vs fp64 after {STEPS} steps: {e:.2e}", flush=True)
+        assert e < 1e-9, method

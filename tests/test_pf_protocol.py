@@ -7,14 +7,15 @@ function of the (alpha, beta) a launch uses and of its iteration number, so any
 difference in the protocol shows up as different partials.  Both forms must hand every
 working launch the same alpha and beta, run the same launches, and leave the same state
 (iterations, rho, previous rho, pause) at every chunk end, over random sequences that
-include pauses at the stop, at the cap and on a non-positive <p, Ap>, on the first and
-on the last launch of a chunk."""
+include pauses at the stop, at the cap, on a non-positive <p, Ap> and for the drop since
+the last reliable update, on the first and on the last launch of a chunk."""
 import math
 import random
 
 import pytest
 
 TOL_REL_TRUE = 1
+RELIABLE_DROP = 1e-5                           # kReliableDrop
 
 
 class State:
@@ -23,6 +24,7 @@ class State:
         self.iter0 = 0
         self.rr = rr
         self.rr_prev = rr
+        self.rr_rel = rr                       # rrRel: rho at the last anchor
         self.alpha = 0.0
         self.beta = 0.0
         self.stop = stop
@@ -36,7 +38,8 @@ class State:
 def pf_decide(mu, nu, rho, it, S):
     """pf_decide: stop rule and the next alpha, beta"""
     stop = (it >= S.max_iter) or (math.sqrt(rho) <= S.stop and it % 2 == 0) or \
-        (not mu > 0.0 and it % 2 == 0) or not math.isfinite(mu) or not math.isfinite(rho)
+        (not mu > 0.0 and it % 2 == 0) or (rho < RELIABLE_DROP ** 2 * S.rr_rel and it % 2 == 0) or \
+        not math.isfinite(mu) or not math.isfinite(rho)
     alpha = rho / mu if mu > 0.0 else 0.0
     rho_next = alpha * alpha * nu - rho
     if not rho_next > 0.0:
@@ -49,7 +52,8 @@ def launch_output(alpha, beta, it, seed):
     """block-partial sums (mu, nu, rho) of the launch computing iteration `it`: a fixed
     function of what the launch used; rho decays, mu turns non-positive now and then"""
     rng = random.Random(hash((round(alpha, 12), round(beta, 12), it, seed)))
-    rho = 10.0 ** (-0.08 * it) * (0.5 + rng.random())
+    rate = 0.08 if seed % 2 else 0.5            # fast decay: pauses for the drop since the anchor
+    rho = 10.0 ** (-rate * it) * (0.5 + rng.random())
     mu = rho * (0.2 + rng.random()) * (-1.0 if rng.random() < 0.02 else 1.0)
     nu = mu * (1.0 + 3.0 * rng.random())
     return mu, nu, rho
@@ -108,6 +112,7 @@ def reliable_update(S, seed):
     S.alpha = 0.0
     S.beta = rn / S.rr_prev if S.rr_prev > 0.0 else 0.0
     S.rr = rn
+    S.rr_rel = rn
     S.iter0 = S.iter
     S.done = False
 
@@ -116,7 +121,7 @@ def reliable_update(S, seed):
 def test_prologue_finalize_equals_separate(seed):
     rng = random.Random(seed)
     max_iter = rng.choice([30, 57, 120, 400])
-    stop = 10.0 ** (-rng.uniform(1.0, 6.0))
+    stop = 10.0 ** (-rng.uniform(1.0, 6.0 if seed % 2 else 16.0))
     chunks = [rng.choice([8, 16, 24, 50]) for _ in range(12)]
     A, B = State(1.0, stop, max_iter), State(1.0, stop, max_iter)
     la, lb = [], []

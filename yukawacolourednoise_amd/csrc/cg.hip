@@ -124,6 +124,7 @@ k_fin_setup(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, 
     CgState &S = st[sys];
     S.rr = tot.x;
     S.rr_prev = tot.x;
+    S.rrRel = tot.x;
     S.bnorm2 = tot.x;
     S.tol = tol;
     S.mode = mode;
@@ -160,6 +161,7 @@ __global__ void __launch_bounds__(kNT) k_fin_alpha(const double2 *__restrict__ p
 
 // beta = |r'|^2 / |r|^2 and the reference stopping rule (CGsolver.cu:160,208-211).
 // MIXED: the FP32 loop pauses at the stop or the cap and waits for a reliable update.
+// It also pauses once the recursive |r| has fallen by kReliableDrop since the last anchor.
 // EVEN: a system pauses only after an even number of iterations (or at the cap): chunks
 // have even length and p ping-pongs between two buffers, so the last direction is then
 // in p[0], where the iteration that resumes it after a reliable update or a tightened
@@ -178,7 +180,8 @@ k_fin_beta(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, i
     S.beta = rrn / S.rr;
     S.rr = rrn;
     S.iter += 1;
-    const bool run = (S.iter < S.maxIter) && (sqrt(rrn) > S.stop || (EVEN && (S.iter & 1)));
+    const bool pause = !(sqrt(rrn) > S.stop) || (MIXED && rrn < kReliableDrop * kReliableDrop * S.rrRel);
+    const bool run = (S.iter < S.maxIter) && (!pause || (EVEN && (S.iter & 1)));
     if (!run) {
         S.done = 1;
         if (!MIXED) S.converged = sqrt(rrn) <= S.stop ? 1 : 0;
@@ -258,6 +261,7 @@ k_fin_mp_true(const double2 *__restrict__ part, int nb, CgState *__restrict__ st
         S.beta = rrt / S.rr_prev;
     }
     S.rr = rrt;
+    S.rrRel = rrt;
     if (S.done) { S.done = 0; atomicAdd(active, 1); }
 }
 

@@ -285,6 +285,7 @@ k_fin_eo_setup(const double2 *__restrict__ partE, int nbE, const double2 *__rest
     S.enorm2 = e.x;
     S.rr = cc.x;
     S.rr_prev = cc.x;
+    S.rrRel = cc.x;
     S.bnorm2 = cc.x;
     S.tol = tol;
     S.mode = mode;
@@ -377,7 +378,7 @@ struct PfStep {
 __device__ __forceinline__ PfStep pf_decide(double mu, double nu, double rho, int iter, const CgState &S) {
     PfStep d;
     d.stop = (iter >= S.maxIter) || (sqrt(rho) <= S.stop && !(iter & 1)) || (!(mu > 0.0) && !(iter & 1)) ||
-             !isfinite(mu) || !isfinite(rho);
+             (rho < kReliableDrop * kReliableDrop * S.rrRel && !(iter & 1)) || !isfinite(mu) || !isfinite(rho);
     d.alpha = (mu > 0.0) ? rho / mu : 0.0;
     double rhoNext = d.alpha * d.alpha * nu - rho;
     if (!(rhoNext > 0.0)) rhoNext = 0.0;
@@ -888,8 +889,9 @@ k_fin_eo_pf_true(const double2 *__restrict__ partRes, const double2 *__restrict_
     CgState &S = st[sys];
     S.true_rel = S.enorm2 > 0.0 ? sqrt(res.x / S.enorm2) : sqrt(res.x);
     S.restarts += 1;
-    // paused at the stop: the normal-equation threshold the true residual asks for
-    if (S.done && S.true_rel > 0.0) S.need = sqrt(rn.x) * S.tol / S.true_rel;
+    // paused at the stop (not for the drop since the last anchor): the normal-equation
+    // threshold the true residual asks for
+    if (S.done && sqrt(S.rr) <= S.stop && S.true_rel > 0.0) S.need = sqrt(rn.x) * S.tol / S.true_rel;
     const bool conv = (S.mode == YCN_TOL_REL_TRUE) ? (S.true_rel <= S.tol) : (sqrt(rn.x) <= S.stop);
     if (conv || S.iter >= S.maxIter) {
         S.converged = conv ? 1 : 0;
@@ -903,6 +905,7 @@ k_fin_eo_pf_true(const double2 *__restrict__ partRes, const double2 *__restrict_
     S.alpha = make_double2(0.0, 0.0);
     S.beta = (S.rr_prev > 0.0) ? rn.x / S.rr_prev : 0.0;
     S.rr = rn.x;
+    S.rrRel = rn.x;
     S.iter0 = S.iter;   // the next chunk's launches count from here
     if (S.mode == YCN_TOL_REL_TRUE && sqrt(rn.x) <= S.stop) S.stop = sqrt(rn.x) * 0.5 * S.tol / S.true_rel;
     if (S.done) {

@@ -49,7 +49,15 @@ struct CgState {
     int iter0;         // pipelined even-odd iteration: iterations done when the chunk began
     double stop0;      // even-odd solve: the unscaled starting threshold tol |r_0|
     double need;       // even-odd solve: threshold the true residual asked for at the last check at the stop
+    double rrRel;      // FP32 loops: recursive |r|^2 when the recursion was last anchored (start, reliable update)
 };
+
+// FP32 CG loops pause for an FP64 reliable update once the recursive |r| has fallen by
+// this factor since the recursion was last anchored to the true residual (Clark et al.,
+// CPC 181 (2010), sec. 5).  Further FP32 iterations would take the recursive residual
+// below the floor that FP32 rounding puts under the true one, and the direction kept
+// across the update, scaled by |r_true|^2 / |r_prev|^2, would then swamp the new residual.
+constexpr double kReliableDrop = 1e-5;
 
 // Per-replica Langevin state (device resident).
 struct StepState {
