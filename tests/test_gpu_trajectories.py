@@ -8,7 +8,9 @@ can be held to the oracle's over its full length, for every solver.
 - 32^2, the reference's own CPU-runnable case: m^2 = -0.2, lambda = 0.5, M = 0.3, g = 0.3,
   s = 0.5, eps = 0.01 fixed, cold start, seed 1234, 2000 steps, CG to 1e-10.  Each solver
   (plain FP64, mixed precision, even-odd FP64, even-odd pipelined mixed) is compared with
-  the oracle's plain FP64 trajectory every 500 steps.
+  the oracle's plain FP64 trajectory every 500 steps, and so are FP64 and even-odd mixed
+  with the reference's adaptive step size, and flexible CG with the learned preconditioner
+  after 500 steps of online training.
 - 512^2 mixed-precision comparison: M = 0.05, g = 0.5, m^2 = -0.4, lambda = 1, s = 0.5,
   eps = 0.005, hot start (init seed 7), 2000 steps.  FP32 inner iterations with FP64
   reliable updates, plain and on the even-odd Schur complement, against the FP64 CG
@@ -44,26 +46,40 @@ def _set_method(Y, p, method):
 _oracle_c1 = {}
 
 
-def _oracle_snapshots():
-    """the oracle's 32^2 trajectory from a cold start, every EVERY steps (computed once)"""
-    if not _oracle_c1:
+def _oracle_snapshots(adaptive=0):
+    """the oracle's 32^2 trajectory from a cold start, every EVERY steps (computed once
+    per step-size mode)"""
+    if adaptive not in _oracle_c1:
         O.set_threads(min(8, os.cpu_count() or 1))
-        orc = O.params(**C1)
+        orc = O.params(**{**C1, "adaptive": adaptive})
         phi = np.zeros((C1["Nt"], C1["Nx"]))
+        snaps = {}
         for step in range(STEPS):
             phi, _, st = O.langevin_step(orc, phi, step)
             assert st["converged"] == 1
             if (step + 1) % EVERY == 0:
-                _oracle_c1[step + 1] = phi.copy()
-    return _oracle_c1
+                snaps[step + 1] = phi.copy()
+        _oracle_c1[adaptive] = snaps
+    return _oracle_c1[adaptive]
 
 
 @pytest.mark.parametrize("method", list(METHODS))
 def test_32_full_trajectory(gpu, method):
-    Y = gpu
-    print(f"# 32^2 trajectory, {STEPS} steps, {method}", flush=True)
-    ref = _oracle_snapshots()
-    p, _ = mk(Y, **C1)
+    _trajectory_32(gpu, method, adaptive=0)
+
+
+@pytest.mark.parametrize("method", ["fp64", "eo-mixed"])
+def test_32_adaptive_step_trajectory(gpu, method):
+    """the reference's adaptive step (eps = epsBar 30 / max |drift|, clamped) over the
+    same 2000 steps: eps depends on the drift's maximum, so the solvers' drifts must agree
+    to the CG tolerance at every step for the trajectories to stay together"""
+    _trajectory_32(gpu, method, adaptive=1)
+
+
+def _trajectory_32(Y, method, adaptive):
+    print(f"# 32^2 trajectory, {STEPS} steps, {method}, adaptive={adaptive}", flush=True)
+    ref = _oracle_snapshots(adaptive)
+    p, _ = mk(Y, **{**C1, "adaptive": adaptive})
     _set_method(Y, p, method)
     with Y.Context(p) as c:
         c.set_field(np.zeros((C1["Nt"], C1["Nx"])))
@@ -73,6 +89,33 @@ def test_32_full_trajectory(gpu, method):
             e = l2rel(c.get_field(), ref[k + EVERY])
             print(f"#   step {k + EVERY}: |phi - phi_oracle| / |phi_oracle| = {e:.2e}, "
                   f"{st['iters_mean']:.1f} iterations per solve", flush=True)
+            assert e < 1e-9, (k + EVERY, e)
+
+
+def test_32_learned_preconditioner_trajectory(gpu):
+    """Flexible CG with the BF16 CNN preconditioner: trained online during the first 500
+    steps (whose solves are plain FP64 CG, so the chain is the oracle's), then 1500 steps
+    of flexible CG.  Convergence is judged on the FP64 true residual of the plain operator,
+    so the trajectory stays the oracle's whatever the network learned."""
+    Y = gpu
+    print(f"# 32^2 trajectory, {STEPS} steps, learned preconditioner", flush=True)
+    ref = _oracle_snapshots()
+    p, _ = mk(Y, **C1)
+    p.cgPrecision = Y.YCN_CG_MIXED
+    p.precond = Y.YCN_PRECOND_CNN
+    p.reliableEvery = 50
+    with Y.Context(p) as c:
+        c.set_field(np.zeros((C1["Nt"], C1["Nx"])))
+        info = c.precond_train(0, EVERY, pairs_per_solve=4, lr=1e-3)
+        e = l2rel(c.get_field(), ref[EVERY])
+        print(f"#   trained on {info['pairs']} pairs (loss {info['loss_first']:.3f} -> {info['loss_last']:.3f}); "
+              f"step {EVERY}: {e:.2e}", flush=True)
+        assert e < 1e-9
+        for k in range(EVERY, STEPS, EVERY):
+            st = c.langevin_run(k, EVERY)
+            assert st["converged"] == 1, st
+            e = l2rel(c.get_field(), ref[k + EVERY])
+            print(f"#   step {k + EVERY}: {e:.2e}, {st['iters_mean']:.1f} iterations per solve", flush=True)
             assert e < 1e-9, (k + EVERY, e)
 
 
