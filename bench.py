@@ -14,9 +14,13 @@ the thermalisation steps are also its online training (untimed, reported).  Inpu
 and state stay resident in HBM.
 
 Prints one JSON line.  The timed region runs uninstrumented, CG iteration chunks replayed
-as captured hipGraphs.  `roofline` is measured live right after it: in a profiled pass
-of `prof_steps` steps every launch of the dominant kernel is bracketed by HIP events on
-the library's own stream (launches that exit at once for converged systems excluded).
+as captured hipGraphs.  `roofline` is measured live right after it, in a profiled pass
+of `prof_steps` steps with HIP events on the library's own stream.  For the pipelined
+even-odd iteration with the prologue finalize, a CG chunk is nothing but back-to-back
+launches of that kernel: one event pair brackets each chunk, divided by the iterations
+done in it (launches after a pause count as time, not as launches).  Otherwise every
+launch of the dominant kernel is bracketed (launches that exit at once for converged
+systems excluded).
 `cpu_baseline` times the FP64 CPU oracle on a bounded sample of the same workload.
 """
 from __future__ import annotations
@@ -334,6 +338,9 @@ def main():
     k_ms, k_n, k_bytes = h["prof"]
     avg_ms = k_ms / max(1, k_n)
     achieved = (k_bytes / 1e9) / (avg_ms / 1e3) if k_n else 0.0
+    # the library brackets whole chunks when the even-odd iteration finalizes in the next
+    # launch's prologue (the tuning report names the form it chose)
+    chunked = head == "eo-mixed" and "prologue finalize" in h["tuning"]
     cg_desc = {"fp64": "plain FP64 CG", "mixed": "mixed-precision CG (FP32 inner, FP64 reliable update every 50)",
                "learned": "flexible CG with the online-trained BF16 CNN preconditioner (FP32 inner, FP64 reliable "
                           "update every 50)",
@@ -376,8 +383,11 @@ def main():
             "traffic": MEASURED_TRAFFIC.get((args.profile_kernel + ("-eo" if head.startswith("eo-") else ""),
                                              "fp64" if head in ("fp64", "eo-fp64") else "fp32", args.size)),
             "avg_launch_us": avg_ms * 1e3,
-            "measured": f"HIP events around every working launch in a profiled pass of {args.prof_steps} steps "
-                        "after the timed region (interval includes the launch's dispatch latency)",
+            "measured": (f"HIP events around each CG chunk of back-to-back launches in a profiled pass of "
+                         f"{args.prof_steps} steps after the timed region, divided by the iterations done in it"
+                         if chunked else
+                         f"HIP events around every working launch in a profiled pass of {args.prof_steps} steps "
+                         "after the timed region (interval includes the launch's dispatch latency)"),
             "launches": k_n,
             "algorithmic_bytes_per_launch": k_bytes,
         },

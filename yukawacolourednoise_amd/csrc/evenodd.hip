@@ -1312,6 +1312,11 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         const int nIt = every + (every & 1);
         const int hardCap = maxIter + nIt + 8;
         int itDone = 0;   // systems already converged at setup exit the chunk kernels at once
+        // profiling the iteration with the prologue finalize: a chunk is nothing but
+        // back-to-back iteration launches, so one event pair brackets it and its working
+        // launches are the iterations the systems did in it (the most of any system)
+        const bool chunkProf = c.profTarget == PK_NORMAL_OP && c.eoFinFused;
+        std::vector<int> itPrev(chunkProf ? nsys : 0, 0);
         while (active > 0 && launched < hardCap) {
             // chunks of the reliable-update interval until the previous solve's iteration
             // count is near, then just past it (a multiple of 8), so that few launches run
@@ -1319,8 +1324,10 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             int n = nIt;
             const int rem = c.eoPredIters - itDone + 4;
             if (c.eoPredIters > 0 && rem < nIt) n = std::max(8, (rem + 7) / 8 * 8);
+            const int slot = chunkProf ? prof_chunk_begin(c, PK_NORMAL_OP) : -1;
             rc = run_chunk(c, phi, nsys, YCN_DAGGER, n, 4, c.tileEo32,
                            [&](int i) { launch_eo_pf_iteration(c, nsys, i); });
+            prof_chunk_end(c, slot);
             if (rc) return rc;
             if (c.eoFinFused) launch_fin_pf_chunk(c, nsys, n);
             launched += n;
@@ -1334,6 +1341,14 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             YCN_HIP(hipGetLastError());
             rc = poll_iters(c, nsys, active, itDone);
             if (rc) return rc;
+            if (chunkProf) {
+                int work = 0;
+                for (int k = 0; k < nsys; ++k) {
+                    work = std::max(work, c.cg_h[k].iter - itPrev[k]);
+                    itPrev[k] = c.cg_h[k].iter;
+                }
+                if (slot >= 0) c.evWork[slot] = work;
+            }
         }
         SolveStats out;   // the last poll left the final states in the mirror
         rc = stats_from_mirror(c, nsys, &out);

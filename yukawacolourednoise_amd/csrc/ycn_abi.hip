@@ -20,15 +20,33 @@ namespace ycn {
 
 void prof_before(Ctx &c, int k, double bytes) {
     if (c.profTarget != k) return;
-    if (c.evCount >= (int)c.evStart.size()) return;
     c.profBytes = bytes;
+    if (c.profChunk || c.evCount >= (int)c.evStart.size()) return;
     (void)hipEventRecord(c.evStart[c.evCount], c.stream);
 }
 void prof_after(Ctx &c, int k) {
-    if (c.profTarget != k) return;
-    if (c.evCount >= (int)c.evStart.size()) return;
+    if (c.profTarget != k || c.profChunk || c.evCount >= (int)c.evStart.size()) return;
     (void)hipEventRecord(c.evStop[c.evCount], c.stream);
+    c.evWork[c.evCount] = -1;
     ++c.evCount;
+}
+
+// Bracket a whole chunk of back-to-back launches of kernel k with one event pair (no
+// events between its launches).  Returns the pair's slot, or -1 when k is not profiled
+// or the slots are used up; the caller sets evWork[slot] to the chunk's working launches
+// once its iteration count is known.
+int prof_chunk_begin(Ctx &c, int k) {
+    if (c.profTarget != k || c.evCount >= (int)c.evStart.size()) return -1;
+    (void)hipEventRecord(c.evStart[c.evCount], c.stream);
+    c.profChunk = 1;
+    return c.evCount;
+}
+void prof_chunk_end(Ctx &c, int slot) {
+    if (slot < 0) return;
+    (void)hipEventRecord(c.evStop[slot], c.stream);
+    c.profChunk = 0;
+    c.evWork[slot] = 0;
+    c.evCount = slot + 1;
 }
 
 }  // namespace ycn
@@ -187,6 +205,7 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
     }
     c.evStart.resize(8192);
     c.evStop.resize(8192);
+    c.evWork.assign(8192, -1);
     for (size_t i = 0; i < c.evStart.size(); ++i)
         if (hipEventCreate(&c.evStart[i]) != hipSuccess || hipEventCreate(&c.evStop[i]) != hipSuccess)
             return bail(YCN_ERR_HIP);
@@ -599,20 +618,28 @@ int ycn_profile_end(ycn_ctx *ctx, double *total_ms, int64_t *launches, double *b
     CHECK_CTX(ctx);
     Ctx &c = *ctx;
     YCN_HIP(hipStreamSynchronize(c.stream));
-    // Launches replayed after every system converged exit at once (a few us against tens
-    // for a working launch); those shorter than a quarter of the median are not counted.
-    std::vector<float> d(c.evCount);
-    for (int i = 0; i < c.evCount; ++i) YCN_HIP(hipEventElapsedTime(&d[i], c.evStart[i], c.evStop[i]));
-    std::vector<float> sorted(d);
-    std::sort(sorted.begin(), sorted.end());
-    const float cut = sorted.empty() ? 0.f : 0.25f * sorted[sorted.size() / 2];
+    // Single launches: those replayed after every system converged exit at once (a few us
+    // against tens for a working launch); those shorter than a quarter of the median of
+    // the single launches are not counted.  Chunks: the whole bracket over the launches
+    // that did an iteration (launches after a pause count as time, not as launches).
+    std::vector<float> d(c.evCount), single;
+    for (int i = 0; i < c.evCount; ++i) {
+        YCN_HIP(hipEventElapsedTime(&d[i], c.evStart[i], c.evStop[i]));
+        if (c.evWork[i] < 0) single.push_back(d[i]);
+    }
+    std::sort(single.begin(), single.end());
+    const float cut = single.empty() ? 0.f : 0.25f * single[single.size() / 2];
     double tot = 0;
     int64_t n = 0;
-    for (float ms : d)
-        if (ms >= cut) {
-            tot += ms;
+    for (int i = 0; i < c.evCount; ++i) {
+        if (c.evWork[i] < 0 && d[i] >= cut) {
+            tot += d[i];
             ++n;
+        } else if (c.evWork[i] > 0) {
+            tot += d[i];
+            n += c.evWork[i];
         }
+    }
     if (total_ms) *total_ms = tot;
     if (launches) *launches = n;
     if (bytes_per_launch) *bytes_per_launch = c.profBytes;

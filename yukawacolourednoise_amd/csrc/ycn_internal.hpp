@@ -147,7 +147,9 @@ struct Ctx {
     // profiling
     int profTarget = PK_NONE;
     std::vector<hipEvent_t> evStart, evStop;
+    std::vector<int> evWork;   // per event pair: -1 = one launch; n >= 0 = a chunk of n working launches
     int evCount = 0;
+    int profChunk = 0;         // 1 while a chunk is bracketed as a whole (per-launch records off)
     double profBytes = 0;
 };
 
@@ -164,6 +166,8 @@ struct Ctx {
 // profiling brackets around a launch of kernel `k`
 void prof_before(Ctx &c, int k, double bytes);
 void prof_after(Ctx &c, int k);
+int prof_chunk_begin(Ctx &c, int k);
+void prof_chunk_end(Ctx &c, int slot);
 
 // noise.hip
 int noise_init(Ctx &c);
