@@ -231,3 +231,41 @@ def test_eo_mixed_small_time_extents(gpu, Nt):
     dr, _ = O.fermion_drift(orc, phi, 2)
     assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
     assert rel(d, dr) < 1e-8
+
+
+@pytest.mark.parametrize("fin", ["fused", "separate"])
+def test_profile_counts_iterations(gpu, fin, monkeypatch):
+    """The library's kernel profiling (ycn_profile_begin / ycn_profile_end, the source of
+    bench.py's roofline figure).  With the finalize in the next launch's prologue a CG chunk
+    is bracketed as a whole and divided by the iterations done in it, so the counted
+    launches are exactly the profiled steps' iterations; with the separate finalize every
+    iteration launch is bracketed, and the launches that exit at once after a pause are
+    left out (at least the iterations are counted).  The bytes per launch are those of one
+    pipelined iteration and the profiled time stays within the wall time."""
+    import time
+    monkeypatch.setenv("YCN_EO_FIN", fin)
+    Y = gpu
+    N = 64
+    p, _ = mk(Y, Nt=N, Nx=N, M=0.05, g=0.5, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    p.evenOdd = 1
+    p.cgPrecision = Y.YCN_CG_MIXED
+    with Y.Context(p) as c:
+        rep = c.tuning_report()
+        assert ("prologue finalize" if fin == "fused" else "separate finalize") in rep.split("|")[-1], rep
+        c.hot_start(3)
+        c.langevin_run(0, 2)
+        c.sync()
+        c.profile_begin("cg_normal_op")
+        t0 = time.perf_counter()
+        st = c.langevin_run(2, 3)
+        c.sync()
+        wall = time.perf_counter() - t0
+        ms, n, b = c.profile_end()
+    print(f"# {fin}: {n} launches counted for {st['iters']} iterations, {1e3 * ms / max(n, 1):.2f} us each", flush=True)
+    assert st["converged"] == 1
+    if fin == "fused":
+        assert n == st["iters"]
+    else:
+        assert st["iters"] <= n <= st["iters"] + 3 * 64
+    assert b == N * N * (8 * 16 + 4)
+    assert 0.0 < ms < 1e3 * wall
