@@ -14,7 +14,9 @@ can be held to the oracle's over its full length, for every solver.
 - 512^2 mixed-precision comparison: M = 0.05, g = 0.5, m^2 = -0.4, lambda = 1, s = 0.5,
   eps = 0.005, hot start (init seed 7), 2000 steps.  FP32 inner iterations with FP64
   reliable updates, plain and on the even-odd Schur complement, against the FP64 CG
-  trajectory (itself held to the oracle step by step at 256^2 and 1024^2 elsewhere)."""
+  trajectory (itself held to the oracle step by step at 256^2 and 1024^2 elsewhere).
+- The 64-chain m^2 scan, 500 steps batched (512 systems per CG launch) against four of its
+  chains run alone."""
 import os
 
 import numpy as np
@@ -140,3 +142,41 @@ def test_512_mixed_precision_trajectories(gpu):
         e = l2rel(fields[method], fields["fp64"])
         print(f"#   {method} vs fp64 after {STEPS} steps: {e:.2e}", flush=True)
         assert e < 1e-9, method
+
+
+def test_chain_scan_long_run_batched_equals_alone(gpu):
+    """The 64-chain m^2 scan (64 chains of 64^2, m^2 = -1 .. 0.5, seeds 0..63, 8 noise
+    vectors per drift) over 500 steps with the even-odd mixed solver, batched (512 systems
+    per CG launch, per-system pauses, reliable updates and step state), against four of
+    its chains each run alone in a context of one replica: batching must not couple the
+    chains, so the fields agree to the CG tolerance after 500 steps."""
+    Y = gpu
+    R, N, n = 64, 64, 500
+    base = dict(Nt=N, Nx=N, lam=0.5, g=0.3, M=0.3, r=1.0, cutFraction=0.5, driftMode=0, epsBar=0.01, adaptive=0,
+                cgTol=1e-10, cgTolMode=1, cgMaxIter=20000, nNoise=8)
+    m2 = -1.0 + 1.5 * np.arange(R) / 63.0
+    seeds = np.arange(R, dtype=np.uint64)
+    phi0 = np.stack([O.hot_start(10 + k, N, N) for k in range(R)])
+
+    def ctx(nrep):
+        p = Y.default_params(nReplicas=nrep, **base)
+        p.evenOdd = 1
+        p.cgPrecision = Y.YCN_CG_MIXED
+        return Y.Context(p)
+
+    print(f"# chain scan, {R} chains batched, {n} steps", flush=True)
+    with ctx(R) as c:
+        c.set_replicas(m2, seeds)
+        c.set_field(phi0)
+        st = c.langevin_run(0, n)
+        assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+        g = c.get_field()
+    for k in (0, 21, 42, 63):
+        with ctx(1) as c:
+            c.set_replicas(m2[k:k + 1], seeds[k:k + 1])
+            c.set_field(phi0[k])
+            st = c.langevin_run(0, n)
+            assert st["converged"] == 1, st
+            e = l2rel(g[k], c.get_field())
+        print(f"#   chain {k} (m^2 = {m2[k]:.3f}): batched vs alone {e:.2e}", flush=True)
+        assert e < 1e-9, k
