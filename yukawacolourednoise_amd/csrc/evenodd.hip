@@ -1339,6 +1339,14 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             hipLaunchKernelGGL(k_fin_eo_pf_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partRu,
                                (const double2 *)partRu2, nbR, c.cg_d, c.active);
             YCN_HIP(hipGetLastError());
+            // poll only where the systems may have converged: well short of the previous
+            // solve's iteration count they cannot have, so the next chunk is enqueued at
+            // once (its kernels exit at once for systems that are final after all)
+            const int margin = std::max(8, c.eoPredIters / 8);
+            if (c.eoPredIters > 0 && itDone + n + margin < c.eoPredIters && !chunkProf) {
+                itDone += n;
+                continue;
+            }
             rc = poll_iters(c, nsys, active, itDone);
             if (rc) return rc;
             if (chunkProf) {
