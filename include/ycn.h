@@ -168,7 +168,9 @@ int ycn_langevin_run(ycn_ctx *ctx, uint64_t step0, int nsteps, ycn_cg_stats *sta
 /* Measurement block (main.cu:386-483): magnetisation, sum phi^2, eps, Langevin time
  * for replica `replica`; slices (Nt doubles, gpuTimeSlices) and corr (Nt doubles,
  * point-source propagator, gpuTimeSlices_spinors) if non-NULL; the trace needs a
- * fresh fermion drift with measurement noise (stream 0x4D00+j, counter = step). */
+ * fresh fermion drift with measurement noise (stream 0x4D00+j, counter = step).  With
+ * evenOdd the propagator D^-1 src is gamma_5 chi from the even-odd solve of
+ * D^dag chi = gamma_5 src (gamma_5 = sigma_2 per flavour, gamma_5 D gamma_5 = D^dag). */
 int ycn_measure(ycn_ctx *ctx, int replica, uint64_t step, ycn_observables *obs,
                 double *slices, double *corr);
 

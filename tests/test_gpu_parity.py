@@ -237,10 +237,18 @@ def test_multi_rhs_noise(gpu):
     assert rel(d, dr) < RTOL
 
 
-def test_measure(gpu):
+@pytest.mark.parametrize("method", ["plain", "eo-fp64", "eo-mixed"])
+def test_measure(gpu, method):
+    """Time slices, magnetisation, phi^2 and the point-source correlator (main.cu:441-469)
+    against the oracle.  With even-odd preconditioning the propagator D^-1 src comes from
+    the fermion-force solve of D^dag chi = gamma_5 src (gamma_5 D gamma_5 = D^dag)."""
     Y = gpu
-    Nt, Nx = 16, 16
+    Nt, Nx = 16, 32
     p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.5, g=0.3)
+    if method != "plain":
+        p.evenOdd = 1
+    if method == "eo-mixed":
+        p.cgPrecision = Y.YCN_CG_MIXED
     phi = O.hot_start(4, Nt, Nx)
     with Y.Context(p) as c:
         c.set_field(phi)

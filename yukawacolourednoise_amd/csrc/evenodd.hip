@@ -254,6 +254,60 @@ k_eo_drift(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__
     drift[(size_t)rep * V + s] = (A.nNoise == 1) ? fac * acc : fac * acc / (double)A.nNoise;
 }
 
+// gamma_5 = sigma_2 on each flavour's spin pair, (c0, c1) -> (-i c1, i c0); with it
+// gamma_5 D gamma_5 = D^dag for the Wilson-Yukawa operator (Dirac.cu:144-190)
+__device__ __forceinline__ void g5_pair(double2 a0, double2 a1, double2 &o0, double2 &o1) {
+    o0 = make_double2(a1.y, -a1.x);
+    o1 = make_double2(-a0.y, a0.x);
+}
+
+// v = gamma_5 v in place (system 0)
+__global__ void __launch_bounds__(kNT) k_g5(int V, double2 *__restrict__ v) {
+    const int s = blockIdx.x * kNT + threadIdx.x;
+    if (s >= V) return;
+#pragma unroll
+    for (int f = 0; f < 4; f += 2) {
+        double2 o0, o1;
+        g5_pair(v[f * V + s], v[(f + 1) * V + s], o0, o1);
+        v[f * V + s] = o0;
+        v[(f + 1) * V + s] = o1;
+    }
+}
+
+// The measurement's propagator from the even-odd solve of D^dag chi = eta with
+// eta = gamma_5 src (system 0): chi on both parities (the odd sites rebuilt as in
+// k_eo_drift) and out = gamma_5 chi = D^-1 src, full lattice, SoA.
+__global__ void __launch_bounds__(kNT)
+k_eo_propagator(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__ chi_e,
+                const double2 *__restrict__ eta, double2 *__restrict__ out) {
+    const int s = blockIdx.x * kNT + threadIdx.x;
+    if (s >= A.V) return;
+    const size_t V = A.V, Vh = A.Vh;
+    const int t = s / A.Nx, x = s - t * A.Nx;
+    double2 ch[4];
+    if ((t + x) & 1) {
+        double2 k[4];
+        eo_hop<true, double, double2>(A, t, x, [&](int tn, int xn, int c) { return chi_e[c * Vh + eo_j(A, tn, xn)]; },
+                                      k);
+        const double inv = 1.0 / (2.0 * A.r + A.M + A.g * phi[s]);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const double2 ev = eta[c * V + s];
+            ch[c] = make_double2((ev.x - k[c].x) * inv, (ev.y - k[c].y) * inv);
+        }
+    } else {
+#pragma unroll
+        for (int c = 0; c < 4; ++c) ch[c] = chi_e[c * Vh + eo_j(A, t, x)];
+    }
+#pragma unroll
+    for (int f = 0; f < 4; f += 2) {
+        double2 o0, o1;
+        g5_pair(ch[f], ch[f + 1], o0, o1);
+        out[f * V + s] = o0;
+        out[(f + 1) * V + s] = o1;
+    }
+}
+
 // ---------------------------------------------------------------------------
 // finalize kernels (one block per system, fixed-order sums)
 // ---------------------------------------------------------------------------
@@ -1397,6 +1451,28 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
     rc = read_stats(c, nsys, st);
     if (rc) return rc;
     eo_learn_stop_scale(c, nsys);
+    return YCN_OK;
+}
+
+// D^-1 src for the measurement's point source (src in c.eta, system 0, c.nNoise = 1):
+// gamma_5 D gamma_5 = D^dag, so D^-1 src = gamma_5 (D^dag)^-1 (gamma_5 src), and the
+// fermion-force solve of D^dag chi = eta gives it.  The solve's iteration count, starting
+// threshold and chunk length are not learned from this right-hand side: they size the
+// fermion-force solves.
+int eo_propagator(Ctx &c, const double *phi, double2 *out, SolveStats *st) {
+    const unsigned nbV = (unsigned)((c.V + kNT - 1) / kNT);
+    hipLaunchKernelGGL(k_g5, dim3(nbV), dim3(kNT), 0, c.stream, c.V, c.eta);
+    const int pred = c.eoPredIters, last = c.lastIters;
+    const double scale = c.eoStopScale;
+    int rc = eo_solve(c, phi, 1, false, 0, 0, st);
+    c.eoPredIters = pred;
+    c.lastIters = last;
+    c.eoStopScale = scale;
+    if (rc) return rc;
+    const EoArgs A = eo_args(c, 0, 0);
+    hipLaunchKernelGGL(k_eo_propagator, dim3(nbV), dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x,
+                       (const double2 *)c.eta, out);
+    YCN_HIP(hipGetLastError());
     return YCN_OK;
 }
 

@@ -452,7 +452,8 @@ int ycn_measure(ycn_ctx *ctx, int replica, uint64_t step, ycn_observables *obs, 
     o.clamp_events = S[replica].clamps;
     const double *phi_rep = c.phi[c.cur] + (size_t)replica * c.V;
     if (corr) {
-        // point source (main.cu:441-469): delta_{x,0} (1,1,0,0); CG on D D^dag, then D^dag
+        // point source (main.cu:441-469): delta_{x,0} (1,1,0,0); CG on D D^dag, then D^dag,
+        // i.e. D^-1 src (with even-odd preconditioning: the Schur-complement solve)
         std::vector<double> src(8 * (size_t)c.V, 0.0);
         src[0] = 1.0;
         src[2] = 1.0;
@@ -464,8 +465,12 @@ int ycn_measure(ycn_ctx *ctx, int replica, uint64_t step, ycn_observables *obs, 
         const int save = c.nNoise;
         c.nNoise = 1;
         SolveStats st;
-        rc = cg_solve(c, phi_rep, 1, YCN_NORMAL, false, 0, 0, &st);
-        if (!rc) rc = dirac_apply(c, phi_rep, c.x, c.t, 1, 1);
+        if (c.p.evenOdd && eo_available(c)) {
+            rc = eo_propagator(c, phi_rep, c.t, &st);
+        } else {
+            rc = cg_solve(c, phi_rep, 1, YCN_NORMAL, false, 0, 0, &st);
+            if (!rc) rc = dirac_apply(c, phi_rep, c.x, c.t, 1, 1);
+        }
         c.nNoise = save;
         if (rc) return rc;
         if ((rc = spinor_slices(c, c.t, cr))) return rc;

@@ -240,6 +240,7 @@ int eo_pf_alloc(Ctx &c);
 int eo_autotune(Ctx &c);
 int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase, uint64_t step, SolveStats *st);
 int eo_drift(Ctx &c, const double *phi, double *drift);
+int eo_propagator(Ctx &c, const double *phi, double2 *out, SolveStats *st);
 
 // precision conversion
 int spinor_to_f32(Ctx &c, const double2 *a, float2 *b, int nsys);
