@@ -73,7 +73,8 @@ typedef struct {
     int reliableEvery;     /* mixed precision: FP64 reliable update period (50)         */
     int precond;           /* YCN_PRECOND_*                                             */
     int device;            /* HIP device ordinal                                        */
-    int evenOdd;           /* 1: fermion force by the even-odd preconditioned solve of
+    int evenOdd;           /* 1: fermion force (and the measurement's point-source
+                              propagator) by the even-odd preconditioned solve of
                               D^dag chi = eta (Schur complement on the even sites; the
                               reference's even-odd blocks, Dirac.cu:256-379, are
                               commented out).  Same stopping rule on |eta - D^dag chi|,
