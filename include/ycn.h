@@ -178,7 +178,7 @@ int ycn_measure(ycn_ctx *ctx, int replica, uint64_t step, ycn_observables *obs,
 /* Wait for all work on the context stream. */
 int ycn_sync(ycn_ctx *ctx);
 
-/* ---- learned preconditioner (north-star addition; no reference counterpart) ----
+/* ---- learned preconditioner (no reference counterpart) ----
  * A 3-layer periodic 3x3 CNN, 8 -> 16 -> 16 -> 8 channels (channel 2c / 2c+1 = Re / Im
  * of spinor component c), no bias, tanh between layers, BF16 weights and activations
  * with FP32 accumulation on MFMA, as a learned correction to an exact skip path:

@@ -1,6 +1,6 @@
 """ctypes bindings for the CPU FP64 oracle (``oracle/liborc.so``).
 
-Test infrastructure: the checker for the HIP library.  Only ``tests/``,
+Test infrastructure: the reference the HIP library is tested against.  Only ``tests/``,
 ``__graft_entry__.smoke()`` and ``bench.py``'s ``cpu_baseline`` leg import this
 module; the product package never does.
 

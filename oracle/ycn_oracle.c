@@ -1,7 +1,7 @@
 /*
  * ycn_oracle.c -- CPU FP64 restatement of the reference hot path.
  *
- * TEST INFRASTRUCTURE ONLY (the checker): loaded by tests/, __graft_entry__.smoke()
+ * TEST INFRASTRUCTURE ONLY (the parity reference): loaded by tests/, __graft_entry__.smoke()
  * and bench.py's cpu_baseline leg, never by the product library.
  *
  * Built with gcc -O3 -fopenmp -ffp-contract=off (oracle/Makefile).  All reductions

@@ -2,7 +2,7 @@
  * ycn_oracle.h -- CPU FP64 restatement of the reference's Langevin update of the
  * 2D lattice Yukawa model with coloured noise.
  *
- * THIS IS TEST INFRASTRUCTURE.  It is the checker for the HIP library, never the
+ * THIS IS TEST INFRASTRUCTURE.  It is what the HIP library is tested against, never the
  * thing measured or shipped: only tests/, __graft_entry__.smoke() and bench.py's
  * cpu_baseline leg may load it.  The product path (libycn.so) never calls it.
  *

@@ -1,4 +1,4 @@
-"""Pins the CPU FP64 oracle (oracle/ycn_oracle.c) before it is trusted as the checker.
+"""Pins the CPU FP64 oracle (oracle/ycn_oracle.c) before the parity tests trust it.
 
 Each check is independent of the oracle's own code: published known-answer vectors,
 library routines (numpy.fft, numpy.linalg), closed forms, and an independent numpy

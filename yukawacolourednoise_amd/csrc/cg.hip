@@ -1132,7 +1132,7 @@ int spinor_slices(Ctx &c, const double2 *psi, double *out_dev) {
 }
 
 // ===========================================================================
-// Flexible CG with the learned preconditioner (north-star addition; no reference
+// Flexible CG with the learned preconditioner (no reference
 // counterpart): FP32 inner iterations as in the mixed solver, z = CNN(r_s) each
 // iteration, FP64 reliable updates every `reliableEvery` iterations that restart the
 // direction, and plain CG (identity preconditioner) once a reliable interval stalls.
