@@ -260,6 +260,27 @@ def test_measure(gpu, method):
     assert rel(cr, cref) < 1e-9
 
 
+@pytest.mark.parametrize("evenOdd", [0, 1])
+def test_measure_batched_replica(gpu, evenOdd):
+    """The correlator of replica 1 of a batched context with 2 noise vectors per drift
+    (the measurement solves one system on that replica's field) against the oracle on
+    that field alone."""
+    Y = gpu
+    Nt, Nx = 16, 32
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, M=0.5, g=0.3, nReplicas=2, nNoise=2)
+    p.evenOdd = evenOdd
+    p.cgPrecision = Y.YCN_CG_MIXED
+    phis = np.stack([O.hot_start(4, Nt, Nx), O.hot_start(9, Nt, Nx)])
+    with Y.Context(p) as c:
+        c.set_field(phis)
+        ob, sl, cr = c.measure(1, 3)
+        ob0, _, cr0 = c.measure(0, 3)
+    assert rel(sl, O.time_slices(phis[1])) < RTOL
+    orc1 = O.params(Nt=Nt, Nx=Nx, M=0.5, g=0.3)
+    assert rel(cr, O.correlator(orc1, phis[1])[0]) < 1e-9
+    assert rel(cr0, O.correlator(orc1, phis[0])[0]) < 1e-9
+
+
 @pytest.mark.parametrize("tile", [-1, 0, 1, 2, 3])
 @pytest.mark.parametrize("mtype", [0, 1])
 def test_cg_every_tile(gpu, tile, mtype, monkeypatch):
