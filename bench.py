@@ -224,11 +224,14 @@ C4 = dict(Nt=64, Nx=64, nReplicas=64, nNoise=8, lam=0.5, g=0.3, M=0.3, r=1.0, cu
           epsBar=0.01, adaptive=0, cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
 
 
-def bench_c4(Y, steps, warmup, therm, methods=("fp64", "mixed", "eo-fp64", "eo-mixed")):
+def bench_c4(Y, steps, warmup, therm, methods=("fp64", "mixed", "eo-fp64", "eo-mixed"), profile_kernel="cg_normal_op",
+             prof_steps=2):
     """The batched workload: 64 chains of 64^2 in one context, m^2_k = -1 + 1.5 k / 63,
     seeds 0..63, 8 noise vectors per drift estimate solved together (512 systems per
     CG launch), eps = 0.01 fixed.  Reports chain-steps/s per method from a common
-    thermalised start."""
+    thermalised start, and the roofline of the fastest method's normal-operator kernel
+    from a profiled pass of `prof_steps` steps after its timed region (HIP events on the
+    library's stream, as for the headline workload)."""
     m2 = [-1.0 + 1.5 * k / 63 for k in range(64)]
     seeds = list(range(64))
     out = {}
@@ -255,13 +258,22 @@ def bench_c4(Y, steps, warmup, therm, methods=("fp64", "mixed", "eo-fp64", "eo-m
         ctx.sync()
         dt = time.perf_counter() - t0
         tuning = ctx.tuning_report()
+        prof = (0.0, 0, 0.0)
+        if profile_kernel and prof_steps > 0:
+            ctx.profile_begin(profile_kernel)
+            ctx.langevin_run(step + steps, prof_steps)
+            prof = ctx.profile_end()
         ctx.close()
         out[method] = {"chain_steps_per_s": 64 * steps / dt, "ms_per_batch_step": 1e3 * dt / steps,
                        "cg_iters_per_solve_max": st["iters"] / max(1, steps),
                        "cg_iters_per_solve_mean": st["iters_mean"], "true_rel_resid_max": st["true_rel_resid"],
-                       "tuning": tuning}
+                       "tuning": tuning, "prof": prof}
         print(f"# c4 {method}: {out[method]['chain_steps_per_s']:.1f} chain-steps/s", file=sys.stderr, flush=True)
     head = max(out, key=lambda m: out[m]["chain_steps_per_s"])
+    k_ms, k_n, k_bytes = out[head]["prof"]
+    avg_ms = k_ms / max(1, k_n)
+    achieved = (k_bytes / 1e9) / (avg_ms / 1e3) if k_n else 0.0
+    chunked = head == "eo-mixed" and "prologue finalize" in out[head]["tuning"].split("|")[-1]
     res = {
         "metric": "chain-steps/s, 64 chains x 64^2, 8 noise vectors per drift (true resid <= 1e-10)",
         "value": out[head]["chain_steps_per_s"],
@@ -277,7 +289,25 @@ def bench_c4(Y, steps, warmup, therm, methods=("fp64", "mixed", "eo-fp64", "eo-m
         "data": f"synthetic (Philox hot start, seed 7, {therm} thermalisation steps)",
         "config": {"workload": "64 chains of 64x64 (m^2 scan -1 .. 0.5), 512 systems per CG launch, eps 0.01",
                    "cg": head, "parallelism": "single GPU, batched chains"},
-        "methods": out,
+        "methods": {m: {k: v for k, v in r.items() if k != "prof"} for m, r in out.items()},
+        "roofline": {
+            "kernel": profile_kernel + (" even-odd" if head.startswith("eo-") else "")
+            + (" (fp64)" if head in ("fp64", "eo-fp64") else " (fp32)") + ", 512 systems per launch",
+            "bound": "hbm",
+            "achieved": achieved,
+            "peak": HBM_PEAK_GBS,
+            "unit": "GB/s",
+            "frac": achieved / HBM_PEAK_GBS,
+            "traffic": None,
+            "avg_launch_us": avg_ms * 1e3,
+            "measured": (f"HIP events around each CG chunk of back-to-back launches in a profiled pass of "
+                         f"{prof_steps} steps after the timed region, divided by the iterations done in it"
+                         if chunked else
+                         f"HIP events around every working launch in a profiled pass of {prof_steps} steps "
+                         "after the timed region (interval includes the launch's dispatch latency)"),
+            "launches": k_n,
+            "algorithmic_bytes_per_launch": k_bytes,
+        },
     }
     print(json.dumps(res), flush=True)
 
@@ -315,7 +345,8 @@ def main():
         if args.method == "learned":
             sys.exit("the batched workload runs fp64, mixed, eo-fp64 and eo-mixed")
         bench_c4(Y, args.steps, args.warmup, args.therm,
-                 ("fp64", "mixed", "eo-fp64", "eo-mixed") if args.method == "all" else (args.method,))
+                 ("fp64", "mixed", "eo-fp64", "eo-mixed") if args.method == "all" else (args.method,),
+                 args.profile_kernel, min(args.prof_steps, 2))
         return
 
     methods = ["learned", "fp64", "mixed", "eo-fp64", "eo-mixed"] if args.method == "all" else [args.method]
@@ -339,8 +370,8 @@ def main():
     avg_ms = k_ms / max(1, k_n)
     achieved = (k_bytes / 1e9) / (avg_ms / 1e3) if k_n else 0.0
     # the library brackets whole chunks when the even-odd iteration finalizes in the next
-    # launch's prologue (the tuning report names the form it chose)
-    chunked = head == "eo-mixed" and "prologue finalize" in h["tuning"]
+    # launch's prologue (the tuning report's last entry names the form it chose)
+    chunked = head == "eo-mixed" and "prologue finalize" in h["tuning"].split("|")[-1]
     cg_desc = {"fp64": "plain FP64 CG", "mixed": "mixed-precision CG (FP32 inner, FP64 reliable update every 50)",
                "learned": "flexible CG with the online-trained BF16 CNN preconditioner (FP32 inner, FP64 reliable "
                           "update every 50)",
