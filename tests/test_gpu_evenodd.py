@@ -269,3 +269,23 @@ def test_profile_counts_iterations(gpu, fin, monkeypatch):
         assert st["iters"] <= n <= st["iters"] + 3 * 64
     assert b == N * N * (8 * 16 + 4)
     assert 0.0 < ms < 1e3 * wall
+
+
+def test_eo_mixed_iteration_cap_with_prediction(gpu):
+    """An even-odd mixed solve that cannot reach its tolerance stops at the iteration cap
+    and reports it (YCN_ERR_NOT_CONVERGED, converged = 0, iterations = cap), also when
+    the previous solve's count lets the host enqueue reliable updates without polling."""
+    Y = gpu
+    Nt, Nx = 32, 64
+    cap = 120
+    p, _ = mk(Y, Nt=Nt, Nx=Nx, M=0.05, g=0.5, cgTol=1e-30, cgTolMode=1, cgMaxIter=cap)
+    p.evenOdd = 1
+    p.cgPrecision = Y.YCN_CG_MIXED
+    phi = O.hot_start(11, Nt, Nx)
+    with Y.Context(p) as c:
+        c.set_field(phi)
+        for k in range(3):
+            d, st = c.fermion_force(5 + k)
+            print(f"# solve {k}: {st}", flush=True)
+            assert st["converged"] == 0 and st["iters"] == cap, st
+            assert np.all(np.isfinite(d)) and np.isfinite(st["true_rel_resid"])

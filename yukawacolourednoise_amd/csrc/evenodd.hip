@@ -1371,6 +1371,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         // launches are the iterations the systems did in it (the most of any system)
         const bool chunkProf = c.profTarget == PK_NORMAL_OP && c.eoFinFused;
         std::vector<int> itPrev(chunkProf ? nsys : 0, 0);
+        bool polled = true;
         while (active > 0 && launched < hardCap) {
             // chunks of the reliable-update interval until the previous solve's iteration
             // count is near, then just past it (a multiple of 8), so that few launches run
@@ -1399,10 +1400,12 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             const int margin = std::max(8, c.eoPredIters / 8);
             if (c.eoPredIters > 0 && itDone + n + margin < c.eoPredIters && !chunkProf) {
                 itDone += n;
+                polled = false;
                 continue;
             }
             rc = poll_iters(c, nsys, active, itDone);
             if (rc) return rc;
+            polled = true;
             if (chunkProf) {
                 int work = 0;
                 for (int k = 0; k < nsys; ++k) {
@@ -1411,6 +1414,10 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
                 }
                 if (slot >= 0) c.evWork[slot] = work;
             }
+        }
+        if (!polled) {   // the launch cap ended the loop after a skipped poll
+            rc = poll_iters(c, nsys, active, itDone);
+            if (rc) return rc;
         }
         SolveStats out;   // the last poll left the final states in the mirror
         rc = stats_from_mirror(c, nsys, &out);
