@@ -16,7 +16,8 @@ can be held to the oracle's over its full length, for every solver.
   reliable updates, plain and on the even-odd Schur complement, against the FP64 CG
   trajectory (itself held to the oracle step by step at 256^2 and 1024^2 elsewhere).
 - The 64-chain m^2 scan, 500 steps batched (512 systems per CG launch) against four of its
-  chains run alone."""
+  chains run alone.
+- 256^2 at s = 1 and 0.25, 1000 steps: mixed and even-odd mixed against FP64."""
 import os
 
 import numpy as np
@@ -180,3 +181,31 @@ def test_chain_scan_long_run_batched_equals_alone(gpu):
             e = l2rel(g[k], c.get_field())
         print(f"#   chain {k} (m^2 = {m2[k]:.3f}): batched vs alone {e:.2e}", flush=True)
         assert e < 1e-9, k
+
+
+@pytest.mark.parametrize("s", [1.0, 0.25])
+def test_256_trajectories(gpu, s):
+    """The 256^2 configuration (m^2 = -0.2, lambda = 0.5, M = 0.3, g = 0.3, hot start with
+    init seed 7, eps = 0.005) over 1000 steps at the unfiltered (s = 1) and the most
+    filtered (s = 0.25) coloured noise: mixed precision and even-odd mixed against the FP64
+    trajectory (itself held to the oracle over its first steps elsewhere)."""
+    Y = gpu
+    N, n = 256, 1000
+    base = dict(Nt=N, Nx=N, m2=-0.2, lam=0.5, g=0.3, M=0.3, cutFraction=s, epsBar=0.005, adaptive=0, seed=1234,
+                cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    phi0 = O.hot_start(7, N, N)
+    fields = {}
+    for method in ("fp64", "mixed", "eo-mixed"):
+        print(f"# 256^2 s={s} trajectory, {n} steps, {method}", flush=True)
+        p, _ = mk(Y, **base)
+        _set_method(Y, p, method)
+        with Y.Context(p) as c:
+            c.set_field(phi0)
+            st = c.langevin_run(0, n)
+            assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+            fields[method] = c.get_field()
+        print(f"#   {st['iters_mean']:.1f} iterations per solve", flush=True)
+    for method in ("mixed", "eo-mixed"):
+        e = l2rel(fields[method], fields["fp64"])
+        print(f"#   {method} vs fp64 after {n} steps: {e:.2e}", flush=True)
+        assert e < 1e-9, method
