@@ -209,3 +209,44 @@ def test_256_trajectories(gpu, s):
         e = l2rel(fields[method], fields["fp64"])
         print(f"#   {method} vs fp64 after {n} steps: {e:.2e}", flush=True)
         assert e < 1e-9, method
+
+
+@pytest.mark.skipif(not os.environ.get("YCN_LONG"), reason="5000 steps of plain FP64 CG at 1024^2 take about "
+                    "4 minutes; set YCN_LONG=1 to run")
+@pytest.mark.timeout(1200)
+def test_1024_headline_5000_steps(gpu):
+    """The 1024^2 headline workload over 5000 steps from a hot start (init seed 7): the
+    even-odd pipelined mixed solver against the reference's solver, plain FP64 CG, on the
+    same noise.  Progress every 500 steps."""
+    Y = gpu
+    N, n, every = 1024, 5000, 500
+    base = dict(Nt=N, Nx=N, M=0.05, g=0.5, m2=-0.4, lam=1.0, cutFraction=0.5, epsBar=0.005, adaptive=0, seed=1234,
+                cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+    phi0 = O.hot_start(7, N, N)
+    import time
+    fields, iters, secs = {}, {}, {}
+    for method in ("eo-mixed", "fp64"):
+        p, _ = mk(Y, **base)
+        _set_method(Y, p, method)
+        tot, wall = 0.0, 0.0
+        with Y.Context(p) as c:
+            c.set_field(phi0)
+            for k in range(0, n, every):
+                c.sync()
+                t0 = time.perf_counter()
+                st = c.langevin_run(k, every)
+                c.sync()
+                dt = time.perf_counter() - t0
+                wall += dt
+                assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+                tot += st["iters_mean"] * every
+                print(f"# {method}: {k + every} steps, {st['iters_mean']:.1f} iterations per solve, "
+                      f"{every / dt:.1f} steps/s", flush=True)
+            fields[method] = c.get_field()
+        iters[method], secs[method] = tot / n, wall
+    e = l2rel(fields["eo-mixed"], fields["fp64"])
+    print(f"# after {n} steps: |phi_eo-mixed - phi_fp64| / |phi_fp64| = {e:.2e}; iterations per solve "
+          f"{iters['eo-mixed']:.1f} (even-odd mixed) vs {iters['fp64']:.1f} (plain FP64); whole run "
+          f"{n / secs['eo-mixed']:.1f} vs {n / secs['fp64']:.1f} steps/s ({secs['fp64'] / secs['eo-mixed']:.2f}x)",
+          flush=True)
+    assert e < 1e-8
