@@ -323,20 +323,25 @@ void noise_free(Ctx &c) {
 }
 
 int noise_generate(Ctx &c, uint64_t step, const double *xi_dev, double *out_dev) {
+    return noise_generate_on(c, c.stream, step, xi_dev, out_dev);
+}
+
+// the three noise kernels on stream s (the context stream, or the step's noise stream)
+int noise_generate_on(Ctx &c, hipStream_t s, uint64_t step, const double *xi_dev, double *out_dev) {
     const int Nt = c.p.Nt, Nx = c.p.Nx, H = c.H;
     const int lx = lines_per_block(Nx), lt = lines_per_block(Nt);
     const int npairs = (Nt + 1) / 2;
     dim3 g1((npairs + lx - 1) / lx, c.nRep), g2((H + lt - 1) / lt, c.nRep);
     const size_t sh1 = (size_t)2 * lx * Nx * sizeof(double2), sh2 = (size_t)2 * lt * Nt * sizeof(double2);
-    hipLaunchKernelGGL(k_noise_rows_fwd, g1, dim3(kFftThreads), sh1, c.stream, xi_dev, (const uint64_t *)c.seed_d,
+    hipLaunchKernelGGL(k_noise_rows_fwd, g1, dim3(kFftThreads), sh1, s, xi_dev, (const uint64_t *)c.seed_d,
                        step, c.spec, Nt, Nx, H, lx, c.planX);
     YCN_HIP(hipGetLastError());
     prof_before(c, PK_NOISE_COLS, 32.0 * (double)c.nRep * Nt * H);
-    hipLaunchKernelGGL(k_noise_cols, g2, dim3(kFftThreads), sh2, c.stream, c.spec, (const int *)c.kmax, c.kmaxAll,
+    hipLaunchKernelGGL(k_noise_cols, g2, dim3(kFftThreads), sh2, s, c.spec, (const int *)c.kmax, c.kmaxAll,
                        1.0 / (double)c.V, Nt, H, lt, c.planT);
     prof_after(c, PK_NOISE_COLS);
     YCN_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_noise_rows_inv, g1, dim3(kFftThreads), sh1, c.stream, (const double2 *)c.spec, out_dev, Nt,
+    hipLaunchKernelGGL(k_noise_rows_inv, g1, dim3(kFftThreads), sh1, s, (const double2 *)c.spec, out_dev, Nt,
                        Nx, H, lx, c.planX);
     YCN_HIP(hipGetLastError());
     return YCN_OK;

@@ -107,6 +107,7 @@ void ycn_destroy(ycn_ctx *ctx) {
     if (!ctx) return;
     Ctx &c = *ctx;
     (void)hipSetDevice(c.p.device);
+    if (c.noiseStream) (void)hipStreamSynchronize(c.noiseStream);
     if (c.stream) (void)hipStreamSynchronize(c.stream);
     noise_free(c);
     cg_free(c);
@@ -121,6 +122,9 @@ void ycn_destroy(ycn_ctx *ctx) {
     if (c.step_d) (void)hipFree(c.step_d);
     for (auto e : c.evStart) (void)hipEventDestroy(e);
     for (auto e : c.evStop) (void)hipEventDestroy(e);
+    if (c.evMain) (void)hipEventDestroy(c.evMain);
+    if (c.evNoise) (void)hipEventDestroy(c.evNoise);
+    if (c.noiseStream) (void)hipStreamDestroy(c.noiseStream);
     if (c.stream) (void)hipStreamDestroy(c.stream);
     delete ctx;
 }
@@ -159,6 +163,10 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
     };
     if (hipSetDevice(p->device) != hipSuccess) return bail(YCN_ERR_HIP);
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return bail(YCN_ERR_HIP);
+    if (hipStreamCreateWithFlags(&c.noiseStream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreateWithFlags(&c.evMain, hipEventDisableTiming) != hipSuccess ||
+        hipEventCreateWithFlags(&c.evNoise, hipEventDisableTiming) != hipSuccess)
+        return bail(YCN_ERR_HIP);
     const size_t fb = sizeof(double) * (size_t)c.nRep * c.V;
     double **bufs[] = {&c.phi[0], &c.phi[1], &c.noise, &c.drift};
     for (double **b : bufs)
@@ -380,12 +388,27 @@ int ycn_fermion_force(ycn_ctx *ctx, uint64_t step, double *drift, ycn_cg_stats *
     return st.converged ? YCN_OK : YCN_ERR_NOT_CONVERGED;
 }
 
+// cn() does not depend on the force, so unless a kernel is being profiled it runs on
+// the noise stream under the fermion solve: it first waits for everything already on
+// the context stream (the previous update reads the buffer it overwrites), and the
+// context stream waits for it before the update, also when the solve failed.
 static int step_impl(Ctx &c, uint64_t step, SolveStats *st) {
-    int rc = noise_generate(c, step, nullptr, c.noise);     // cn()
+    const bool side = c.noiseStream && c.profTarget == PK_NONE;
+    int rc;
+    if (side) {
+        YCN_HIP(hipEventRecord(c.evMain, c.stream));
+        YCN_HIP(hipStreamWaitEvent(c.noiseStream, c.evMain, 0));
+        rc = noise_generate_on(c, c.noiseStream, step, nullptr, c.noise);   // cn()
+        if (rc) return rc;
+        YCN_HIP(hipEventRecord(c.evNoise, c.noiseStream));
+    } else {
+        rc = noise_generate(c, step, nullptr, c.noise);                      // cn()
+        if (rc) return rc;
+    }
+    rc = force_impl(c, step, 1u, st);                                         // fDrift.getForce(drift)
+    if (side) YCN_HIP(hipStreamWaitEvent(c.stream, c.evNoise, 0));
     if (rc) return rc;
-    rc = force_impl(c, step, 1u, st);                        // fDrift.getForce(drift)
-    if (rc) return rc;
-    return bosonic_update(c, c.p.adaptive ? 0 : 1);          // kli.Run(kAll)
+    return bosonic_update(c, c.p.adaptive ? 0 : 1);                           // kli.Run(kAll)
 }
 
 int ycn_langevin_step(ycn_ctx *ctx, uint64_t step, double *eps, ycn_cg_stats *stats) {

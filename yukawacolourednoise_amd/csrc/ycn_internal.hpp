@@ -74,6 +74,10 @@ struct Ctx {
     ycn_params p;
     int V = 0, H = 0, nRep = 0, nNoise = 0, nSys = 0;
     hipStream_t stream = nullptr;
+    // second stream (default priority) for the coloured noise of a Langevin step, which
+    // does not depend on the force; joined into `stream` before the update
+    hipStream_t noiseStream = nullptr;
+    hipEvent_t evMain = nullptr, evNoise = nullptr;
     std::string err;
 
     // host copies of per-replica parameters
@@ -174,6 +178,7 @@ int noise_init(Ctx &c);
 void noise_free(Ctx &c);
 // xi == nullptr: Philox field of (seeds, stream 0, step); else filter xi (device, nRep*V)
 int noise_generate(Ctx &c, uint64_t step, const double *xi_dev, double *out_dev);
+int noise_generate_on(Ctx &c, hipStream_t s, uint64_t step, const double *xi_dev, double *out_dev);
 
 // cg.hip
 struct SolveStats {
