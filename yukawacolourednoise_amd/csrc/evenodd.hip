@@ -155,14 +155,21 @@ k_eo_hop_oe(EoArgs A, const double *__restrict__ phi, const double2 *__restrict_
     for (int c = 0; c < 4; ++c) U[off + c * Vh + j] = make_double2(k[c].x * inv, k[c].y * inv);
 }
 
+// half-lattice vectors a setup launch also clears at its sites (any may be null)
+struct EoZero {
+    double2 *d[3];
+    float2 *f[3];
+};
+
 // v_e = A_e in_e - K_eo U_o (DAG: S in) or A_e in_e - H_eo U_o (S^T in).
-// MODE 0: out = v; MODE 1: out = b - v and partial |out|^2; MODE 2: out = v and partial
-// |v|^2.  out32 (optional) receives an FP32 copy of out.
+// MODE 0: out = v; MODE 1: out = b - v and partial |out|^2; MODE 2 (a solve's setup):
+// out = v and partial |v|^2, and the vectors of z cleared.  out32 (optional) receives
+// an FP32 copy of out.
 template <bool DAG, int MODE>
 __global__ void __launch_bounds__(kNT)
 k_eo_hop_eo(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__ in, const double2 *__restrict__ U,
             const double2 *__restrict__ b, double2 *__restrict__ out, float2 *__restrict__ out32,
-            double2 *__restrict__ part) {
+            double2 *__restrict__ part, EoZero z) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.y, j = blockIdx.x * kNT + threadIdx.x;
     const size_t Vh = A.Vh, off = (size_t)sys * 4 * Vh;
@@ -183,32 +190,19 @@ k_eo_hop_eo(EoArgs A, const double *__restrict__ phi, const double2 *__restrict_
             out[off + c * Vh + j] = v;
             if (out32) out32[off + c * Vh + j] = make_float2((float)v.x, (float)v.y);
             acc.x += v.x * v.x + v.y * v.y;
+            if (MODE == 2) {
+#pragma unroll
+                for (int q = 0; q < 3; ++q) {
+                    if (z.d[q]) z.d[q][off + c * Vh + j] = make_double2(0.0, 0.0);
+                    if (z.f[q]) z.f[q][off + c * Vh + j] = make_float2(0.f, 0.f);
+                }
+            }
         }
     }
     if (MODE != 0) {
         acc = block_sum2<kNT>(acc, lds);
         if (threadIdx.x == 0) part[(size_t)sys * gridDim.x + blockIdx.x] = acc;
     }
-}
-
-__global__ void __launch_bounds__(kNT) k_eo_zero(int64_t n, double2 *__restrict__ a, double2 *__restrict__ b,
-                                                 double2 *__restrict__ c) {
-    const int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x;
-    if (i >= n) return;
-    const double2 z = make_double2(0.0, 0.0);
-    a[i] = z;
-    if (b) b[i] = z;   // b, c may be null (the FP64 solver's directions, unused in mixed precision)
-    if (c) c[i] = z;
-}
-
-__global__ void __launch_bounds__(kNT) k_eo_zero32(int64_t n, float2 *__restrict__ a, float2 *__restrict__ b,
-                                                   float2 *__restrict__ c) {
-    const int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x;
-    if (i >= n) return;
-    const float2 z = make_float2(0.f, 0.f);
-    a[i] = z;
-    b[i] = z;
-    c[i] = z;
 }
 
 // drift_x = fac Re sum_k conj(chi_xk) eta_xk (FermionicDrift.cu:122-139) with chi_e from
@@ -1339,16 +1333,23 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         hipLaunchKernelGGL(k_eo_eta<false>, gF, dim3(kNT), 0, c.stream, A, (const uint64_t *)c.seed_d, streamBase,
                            step, phi, c.eta, B.U, partE);
     hipLaunchKernelGGL(k_eo_rhs, gH, dim3(kNT), 0, c.stream, A, (const double2 *)c.eta, (const double2 *)B.U, B.b);
-    // r = S^T b (and its FP32 copy), x = 0, p = 0
+    // r = S^T b (and its FP32 copy); the same launch clears x and the directions (FP64
+    // solver: p[0], p[1]; mixed: x_s, p[0], Ap)
     hipLaunchKernelGGL(k_eo_hop_oe<false>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.b, B.U);
-    hipLaunchKernelGGL((k_eo_hop_eo<false, 2>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.b,
-                       (const double2 *)B.U, (const double2 *)nullptr, c.r, mixed ? c.rs : (float2 *)nullptr, partH);
-    const int64_t nh = (int64_t)nsys * 4 * Vh;
-    hipLaunchKernelGGL(k_eo_zero, dim3((unsigned)((nh + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, nh, c.x,
-                       mixed ? (double2 *)nullptr : c.pv[0], mixed ? (double2 *)nullptr : c.pv[1]);
+    EoZero zero{};
+    zero.d[0] = c.x;
     if (mixed) {
-        hipLaunchKernelGGL(k_eo_zero32, dim3((unsigned)((nh + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, nh, c.xs,
-                           c.ps[0], c.Aps);
+        zero.f[0] = c.xs;
+        zero.f[1] = c.ps[0];
+        zero.f[2] = c.Aps;
+    } else {
+        zero.d[1] = c.pv[0];
+        zero.d[2] = c.pv[1];
+    }
+    hipLaunchKernelGGL((k_eo_hop_eo<false, 2>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)B.b,
+                       (const double2 *)B.U, (const double2 *)nullptr, c.r, mixed ? c.rs : (float2 *)nullptr, partH,
+                       zero);
+    if (mixed) {
         const int nrepUsed = (nsys + c.nNoise - 1) / c.nNoise;
         rc = to_f32_n(c, phi, c.phis, (int64_t)std::min(nrepUsed, c.nRep) * c.V);
         if (rc) return rc;
@@ -1446,7 +1447,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
         // true residual of the unpreconditioned system, |b_e - S x_e| / |eta|
         hipLaunchKernelGGL(k_eo_hop_oe<true>, gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x, B.U);
         hipLaunchKernelGGL((k_eo_hop_eo<true, 1>), gH, dim3(kNT), 0, c.stream, A, phi, (const double2 *)c.x,
-                           (const double2 *)B.U, (const double2 *)B.b, B.res, (float2 *)nullptr, partH);
+                           (const double2 *)B.U, (const double2 *)B.b, B.res, (float2 *)nullptr, partH, EoZero{});
         hipLaunchKernelGGL(k_fin_eo_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partH, nbH, c.cg_d,
                            c.active);
         YCN_HIP(hipGetLastError());
