@@ -163,10 +163,10 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
     };
     if (hipSetDevice(p->device) != hipSuccess) return bail(YCN_ERR_HIP);
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return bail(YCN_ERR_HIP);
-    if (hipStreamCreateWithFlags(&c.noiseStream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreateWithFlags(&c.evMain, hipEventDisableTiming) != hipSuccess ||
-        hipEventCreateWithFlags(&c.evNoise, hipEventDisableTiming) != hipSuccess)
-        return bail(YCN_ERR_HIP);
+    // the noise stream is created at the first step that uses it; YCN_NOISE_STREAM=0
+    // keeps the coloured noise on the context stream
+    const char *ns = std::getenv("YCN_NOISE_STREAM");
+    c.noiseSide = !(ns && std::atoi(ns) == 0) && c.p.precond == YCN_PRECOND_NONE;
     const size_t fb = sizeof(double) * (size_t)c.nRep * c.V;
     double **bufs[] = {&c.phi[0], &c.phi[1], &c.noise, &c.drift};
     for (double **b : bufs)
@@ -391,9 +391,21 @@ int ycn_fermion_force(ycn_ctx *ctx, uint64_t step, double *drift, ycn_cg_stats *
 // cn() does not depend on the force, so unless a kernel is being profiled it runs on
 // the noise stream under the fermion solve: it first waits for everything already on
 // the context stream (the previous update reads the buffer it overwrites), and the
-// context stream waits for it before the update, also when the solve failed.
+// context stream waits for it before the update, also when the solve failed.  Not with
+// the learned preconditioner: its flexible-CG steps measured 7-10 % slower once the
+// context owned a second stream, used or not (34.2 against 30.8-31.8 steps/s at
+// 1024^2), for a cause not yet found; such contexts never create it.
 static int step_impl(Ctx &c, uint64_t step, SolveStats *st) {
-    const bool side = c.noiseStream && c.profTarget == PK_NONE;
+    if (c.noiseSide && !c.noiseStream) {
+        if (hipSetDevice(c.p.device) != hipSuccess ||
+            hipStreamCreateWithFlags(&c.noiseStream, hipStreamNonBlocking) != hipSuccess ||
+            hipEventCreateWithFlags(&c.evMain, hipEventDisableTiming) != hipSuccess ||
+            hipEventCreateWithFlags(&c.evNoise, hipEventDisableTiming) != hipSuccess) {
+            c.err = "creating the noise stream failed";
+            return YCN_ERR_HIP;
+        }
+    }
+    const bool side = c.noiseSide && c.noiseStream && c.profTarget == PK_NONE;
     int rc;
     if (side) {
         YCN_HIP(hipEventRecord(c.evMain, c.stream));

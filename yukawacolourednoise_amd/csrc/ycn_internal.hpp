@@ -78,6 +78,7 @@ struct Ctx {
     // does not depend on the force; joined into `stream` before the update
     hipStream_t noiseStream = nullptr;
     hipEvent_t evMain = nullptr, evNoise = nullptr;
+    bool noiseSide = false;   // the steps of this context put the noise on noiseStream
     std::string err;
 
     // host copies of per-replica parameters
