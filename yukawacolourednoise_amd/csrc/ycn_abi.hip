@@ -401,8 +401,15 @@ static int step_impl(Ctx &c, uint64_t step, SolveStats *st) {
             hipStreamCreateWithFlags(&c.noiseStream, hipStreamNonBlocking) != hipSuccess ||
             hipEventCreateWithFlags(&c.evMain, hipEventDisableTiming) != hipSuccess ||
             hipEventCreateWithFlags(&c.evNoise, hipEventDisableTiming) != hipSuccess) {
-            c.err = "creating the noise stream failed";
-            return YCN_ERR_HIP;
+            // the side stream is only an overlap: without it the noise stays on the
+            // context stream
+            (void)hipGetLastError();
+            if (c.evMain) (void)hipEventDestroy(c.evMain);
+            if (c.evNoise) (void)hipEventDestroy(c.evNoise);
+            if (c.noiseStream) (void)hipStreamDestroy(c.noiseStream);
+            c.evMain = c.evNoise = nullptr;
+            c.noiseStream = nullptr;
+            c.noiseSide = false;
         }
     }
     const bool side = c.noiseSide && c.noiseStream && c.profTarget == PK_NONE;
