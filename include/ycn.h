@@ -64,10 +64,10 @@ typedef struct {
     double cutFraction;    /* physics.cutFraction (coloured-noise cutoff s)             */
     int driftMode;         /* fermions.driftMode: 0 normal, 1 rescaled by s^2           */
     double epsBar;         /* langevin.averageEpsilon                                   */
-    int adaptive;          /* 1: eps = epsBar*30/max|drift| (langevin_gpu_v2.cu:277)    */
+    int adaptive;          /* 1: eps = epsBar*30/max|drift| (langevin_gpu_v2.cu:112)    */
     uint64_t seed;         /* random.seed (Philox key)                                  */
     double cgTol;          /* default 1e-8 (params.h:14)                                */
-    int cgTolMode;         /* YCN_TOL_*; default absolute recursive (CGsolver.cu:160)   */
+    int cgTolMode;         /* YCN_TOL_*; default absolute recursive (CGsolver.cu:107)   */
     int cgMaxIter;         /* default 1000 (params.h:15)                                */
     int cgPrecision;       /* YCN_CG_*                                                  */
     int reliableEvery;     /* mixed precision: FP64 reliable update period (50)         */
@@ -148,12 +148,12 @@ int ycn_coloured_filter(ycn_ctx *ctx, const double *xi, double *out);
  * (YCN_DAGGER) with the context's field of replica 0 (or phi if non-NULL). */
 int ycn_dirac_apply(ycn_ctx *ctx, const double *phi, const double *in, double *out, int mtype);
 
-/* CGsolver::solve (code/src/CGsolver.cu:95-112): x = (D^dag D)^-1 b for YCN_DAGGER,
+/* CGsolver::solve (code/src/CGsolver.cu:42-59): x = (D^dag D)^-1 b for YCN_DAGGER,
  * (D D^dag)^-1 b for YCN_NORMAL, replica-0 field (or phi if non-NULL). */
 int ycn_cg_solve(ycn_ctx *ctx, const double *phi, const double *b, double *x, int mtype,
                  ycn_cg_stats *stats);
 
-/* FermionicDrift::getForce (code/src/FermionicDrift.cu:96-119): eta_j (stream 1+j,
+/* FermionicDrift::getForce (code/src/FermionicDrift.cu:57-80): eta_j (stream 1+j,
  * counter = step), CG on D^dag D, chi = D y, drift = -g Re sum conj(chi) eta (x s^2
  * in rescaled mode), averaged over nNoise.  drift (nReplicas*V) copied out if non-NULL. */
 int ycn_fermion_force(ycn_ctx *ctx, uint64_t step, double *drift, ycn_cg_stats *stats);

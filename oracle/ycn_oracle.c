@@ -45,7 +45,7 @@ int orc_set_threads(int n) {
 /* ========================================================================== */
 /* Philox4x32-10 (Salmon, Moraes, Dror, Shaw, SC'11 "Parallel random numbers:  */
 /* as easy as 1, 2, 3"), replacing cuRAND XORWOW (colouredNoise.cpp:31-36,     */
-/* FermionicDrift.cu:89-94).  Pinned by the Random123 known-answer vectors.    */
+/* FermionicDrift.cu:50-55).  Pinned by the Random123 known-answer vectors.    */
 /* ========================================================================== */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]) {
     uint32_t c0 = ctr[0], c1 = ctr[1], c2 = ctr[2], c3 = ctr[3];
@@ -298,7 +298,7 @@ void orc_dirac(const orc_params *p, const double *phi, const double *in_, double
     }
 }
 
-/* solve_kernel's operator (CGsolver.cu:162-182): mtype 1 (Dagger): D then D^dag,
+/* solve_kernel's operator (CGsolver.cu:113-129): mtype 1 (Dagger): D then D^dag,
  * i.e. D^dag D; mtype 0 (Normal): D^dag then D, i.e. D D^dag. */
 static void normal_op_tmp(const orc_params *p, const double *phi, const double *in, double *out, int mtype,
                           double *tmp) {
@@ -364,7 +364,7 @@ static cplx c_div(cplx a, cplx b) {
     return c;
 }
 
-/* CG on A = D^dag D (mtype 1) or D D^dag (mtype 0): solve_kernel (CGsolver.cu:125-223)
+/* CG on A = D^dag D (mtype 1) or D D^dag (mtype 0): solve_kernel (CGsolver.cu:72-170)
  * with x0 = 0, r = p = b, alpha = rr / (p.Ap) (complex), beta = |r'.r'| / rr. */
 int orc_cg(const orc_params *p, const double *phi, const double *b_, double *x_, int mtype,
            int maxIterOverride, orc_cg_stats *st) {
@@ -423,14 +423,14 @@ int orc_cg(const orc_params *p, const double *phi, const double *b_, double *x_,
 /* fermion force                                                               */
 /* ========================================================================== */
 void orc_fermion_noise(uint64_t seed, int j, uint64_t step, int64_t V, double *eta) {
-    /* fillNormalRND (FermionicDrift.cu:141-149): eta_i = (N(0,1), 0) for i < 4V */
+    /* fillNormalRND (FermionicDrift.cu:102-110): eta_i = (N(0,1), 0) for i < 4V */
     double *n = (double *)malloc(sizeof(double) * (size_t)(4 * V));
     orc_normals(seed, 1u + (uint32_t)j, step, 4 * V, n);
     for (int64_t i = 0; i < 4 * V; ++i) { eta[2 * i] = n[i]; eta[2 * i + 1] = 0.0; }
     free(n);
 }
 
-/* getForce (FermionicDrift.cu:96-119) + computeDrift (:122-139); with nNoise > 1 the
+/* getForce (FermionicDrift.cu:57-80) + computeDrift (:83-100); with nNoise > 1 the
  * drift is the mean of the nNoise estimators. */
 int orc_fermion_drift_eta(const orc_params *p, const double *phi, const double *eta_,
                           double *drift, orc_cg_stats *st) {

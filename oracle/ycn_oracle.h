@@ -33,12 +33,12 @@ typedef struct {
     double M;               /* bare fermion mass (input.toml fermions.fermion_mass)   */
     double r;               /* Wilson parameter (fermions.WilsonParam)                */
     double cutFraction;     /* coloured-noise cutoff fraction s (physics.cutFraction) */
-    int driftMode;          /* 0 normal, 1 rescaled by s^2 (FermionicDrift.cu:126-136)*/
-    double epsBar;          /* averageEpsilon (langevin_gpu_v2.cu:236-277)            */
+    int driftMode;          /* 0 normal, 1 rescaled by s^2 (FermionicDrift.cu:87-97)*/
+    double epsBar;          /* averageEpsilon (langevin_gpu_v2.cu:65-117)            */
     int adaptive;           /* 1: eps = epsBar*30/max|drift| (reference); 0: eps=epsBar*/
     uint64_t seed;          /* random.seed                                            */
     double cgTol;           /* CG tolerance (params.h:14, default 1e-8)               */
-    int cgTolMode;          /* 0: sqrt(r.r) <= tol absolute, recursive (CGsolver.cu:160)
+    int cgTolMode;          /* 0: sqrt(r.r) <= tol absolute, recursive (CGsolver.cu:107)
                                1: relative: recursive sqrt(r.r) <= tol*|b|, then the true
                                   residual |b-Ax|/|b| is recomputed and the solve
                                   restarted from x while it exceeds tol               */
@@ -57,7 +57,7 @@ typedef struct {
 } orc_cg_stats;
 
 /* ---- random numbers (replaces cuRAND XORWOW, colouredNoise.cpp:31-36,53-67 and
- *      FermionicDrift.cu:89-94,141-149) --------------------------------------- */
+ *      FermionicDrift.cu:50-55,102-110) --------------------------------------- */
 void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t out[4]);
 /* n standard normals of stream (seed, stream) at step `step`; element i comes from
  * Philox counter (i/2, 0, step_lo, step_hi) via Box-Muller. */
@@ -76,12 +76,12 @@ void orc_dirac(const orc_params *p, const double *phi, const double *in, double 
 /* A = D^dagger D for mtype 1 ("Dagger"), D D^dagger for mtype 0 ("Normal")     */
 void orc_normal_op(const orc_params *p, const double *phi, const double *in, double *out, int mtype);
 
-/* ---- CG (CGsolver.cu:125-230) ------------------------------------------------ */
+/* ---- CG (CGsolver.cu:72-177) ------------------------------------------------ */
 /* maxIterOverride > 0 forces a fixed iteration cap (used to compare at fixed k). */
 int orc_cg(const orc_params *p, const double *phi, const double *b, double *x, int mtype,
            int maxIterOverride, orc_cg_stats *st);
 
-/* ---- fermion force (FermionicDrift.cu:96-139) --------------------------------- */
+/* ---- fermion force (FermionicDrift.cu:57-100) --------------------------------- */
 int orc_fermion_drift(const orc_params *p, const double *phi, uint64_t step, double *drift,
                       orc_cg_stats *st);
 /* same, with the noise vectors passed in (nNoise * 4V complex) */

@@ -197,7 +197,7 @@ def test_dirac_free_spectrum_and_trace():
 
 
 # ---------------------------------------------------------------------------
-# CG (CGsolver.cu:125-223)
+# CG (CGsolver.cu:72-170)
 # ---------------------------------------------------------------------------
 @pytest.mark.parametrize("mtype", [0, 1])
 def test_cg_vs_dense(mtype):
@@ -299,7 +299,7 @@ def test_bosonic_plane_wave_and_update():
     assert np.abs(drift - ref).max() < 1e-14
     assert e == eps
     assert np.abs(phin - (phi - (eps * drift + math.sqrt(2 * eps) * noise))).max() < 1e-15
-    # adaptive step (langevin_gpu_v2.cu:236-277) and clamps (:248-256)
+    # adaptive step (langevin_gpu_v2.cu:65-117) and clamps (:248-256)
     p.adaptive = 1
     _, drift, e, mx = O.bosonic_update(p, phi, fd, noise)
     assert mx == np.abs(drift).max() and abs(e - eps * 30 / mx) < 1e-18

@@ -1,6 +1,6 @@
 // cg.hip -- Wilson-Dirac operator, CG on D^dag D / D D^dag and the stochastic
 // fermion force on gfx950 (replaces DiracOP, CGsolver and FermionicDrift,
-// code/src/Dirac.cu:23-190, CGsolver.cu:95-230, FermionicDrift.cu:96-149).
+// code/src/Dirac.cu:23-190, CGsolver.cu:42-177, FermionicDrift.cu:57-110).
 //
 // All kernels work on `nsys` independent systems at once (grid.y = system): a
 // system is one (replica, noise vector) pair, so replicas and several right-hand
@@ -13,7 +13,7 @@
 // needs neither atomics on data nor agent-scope release fences inside the
 // producers (on the multi-XCD gfx950 such a fence writes back the XCD's L2).
 //
-// One FP64 CG iteration (standard CG dataflow of CGsolver.cu:160-219):
+// One FP64 CG iteration (standard CG dataflow of CGsolver.cu:107-166):
 //   normal op : p' = r + beta p; Ap = D2 D1 p'; <p', Ap>   (fused LDS kernel, or the
 //               two-stencil pair cg_a + cg_b when no tile shape divides the lattice)
 //   fin_alpha : alpha = rr / <p', Ap>                        (complex, as the reference)
@@ -147,7 +147,7 @@ k_fin_setup(const double2 *__restrict__ part, int nb, CgState *__restrict__ st, 
     if (!run) atomicSub(active, 1);
 }
 
-// alpha = rr / <p', Ap> (complex division, CGsolver.cu:190)
+// alpha = rr / <p', Ap> (complex division, CGsolver.cu:137)
 __global__ void __launch_bounds__(kNT) k_fin_alpha(const double2 *__restrict__ part, int nb, CgState *__restrict__ st) {
     __shared__ double2 lds[kNT / kWave];
     const int sys = blockIdx.x;
@@ -159,7 +159,7 @@ __global__ void __launch_bounds__(kNT) k_fin_alpha(const double2 *__restrict__ p
     st[sys].alpha = make_double2(rr * dot.x / den, -rr * dot.y / den);
 }
 
-// beta = |r'|^2 / |r|^2 and the reference stopping rule (CGsolver.cu:160,208-211).
+// beta = |r'|^2 / |r|^2 and the reference stopping rule (CGsolver.cu:107,155-158).
 // MIXED: the FP32 loop pauses at the stop or the cap and waits for a reliable update.
 // It also pauses once the recursive |r| has fallen by kReliableDrop since the last anchor.
 // EVEN: a system pauses only after an even number of iterations (or at the cap): chunks
@@ -483,7 +483,7 @@ k_true_b(LatticeArgs L, const double *__restrict__ phi, const CgState *__restric
 }
 
 // drift_x = fac/nNoise * sum_j Re sum_k conj((D y_j)_xk) eta_j,xk
-// (FermionicDrift.cu:108 applyD Normal + computeDrift :122-139)
+// (FermionicDrift.cu:69 applyD Normal + computeDrift :83-100)
 __global__ void __launch_bounds__(kNT)
 k_drift(LatticeArgs L, const double *__restrict__ phi, const double2 *__restrict__ y, const double2 *__restrict__ eta,
         double *__restrict__ drift, double fac) {
@@ -855,21 +855,40 @@ int cg_autotune(Ctx &c) {
         (void)hipEventElapsedTime(&ms, e0, e1);
         return 1e3 * ms / reps;
     };
-    c.tuneReport = "fp64 normal-operator tiles (us/launch):";
-    double best = 1e30;
-    int bestTile = -1;
-    std::vector<int> cands = {-1};
+    std::vector<int> cands = {-1}, cands32;
     for (int i = 0; i < kNumTiles64; ++i)
         if (tile_fits(c, kTiles64[i])) cands.push_back(i);
-    for (int tile : cands) {
-        const double us =
-            time_it([&](int par) { (void)launch_normal_op(c, L, c.phi[c.cur], c.nSys, YCN_DAGGER, par, tile); });
+    if (c.rs)
+        for (int i = 0; i < kNumTiles32; ++i)
+            if (tile_fits(c, kTiles32[i])) cands32.push_back(i);
+    // measured times (us per launch) of cands, then cands32; reused from an earlier
+    // context of the same shape when there is one
+    std::vector<double> us;
+    const bool memo = tune_lookup(0, c, us) && us.size() == cands.size() + cands32.size();
+    if (!memo) {
+        us.clear();
+        for (int tile : cands)
+            us.push_back(time_it(
+                [&](int par) { (void)launch_normal_op(c, L, c.phi[c.cur], c.nSys, YCN_DAGGER, par, tile); }));
+        for (int i : cands32)
+            us.push_back(time_it([&](int par) {
+                launch_fused<float>(c, i, c.nSys, false, c.phis, c.cg_d, (const float2 *)c.rs, c.ps[par],
+                                    c.ps[par ^ 1], c.Aps, c.partials);
+            }));
+        tune_store(0, c, us);
+    }
+    c.tuneReport = memo ? "fp64 normal-operator tiles (us/launch, measured by an earlier context):"
+                        : "fp64 normal-operator tiles (us/launch):";
+    double best = 1e30;
+    int bestTile = -1;
+    for (size_t k = 0; k < cands.size(); ++k) {
+        const int tile = cands[k];
         if (tile < 0)
-            std::snprintf(line, sizeof line, " two-stencil %.1f;", us);
+            std::snprintf(line, sizeof line, " two-stencil %.1f;", us[k]);
         else
-            std::snprintf(line, sizeof line, " %dx%d %.1f;", kTiles64[tile].TT, kTiles64[tile].TX, us);
+            std::snprintf(line, sizeof line, " %dx%d %.1f;", kTiles64[tile].TT, kTiles64[tile].TX, us[k]);
         c.tuneReport += line;
-        if (us < best) { best = us; bestTile = tile; }
+        if (us[k] < best) { best = us[k]; bestTile = tile; }
     }
     if (const char *force = std::getenv("YCN_TILE")) {
         const int f = std::atoi(force);
@@ -881,15 +900,12 @@ int cg_autotune(Ctx &c) {
         c.tuneReport += " | fp32 tiles (us/launch):";
         best = 1e30;
         int best32 = -1;
-        for (int i = 0; i < kNumTiles32; ++i) {
-            if (!tile_fits(c, kTiles32[i])) continue;
-            const double us = time_it([&](int par) {
-                launch_fused<float>(c, i, c.nSys, false, c.phis, c.cg_d, (const float2 *)c.rs, c.ps[par],
-                                    c.ps[par ^ 1], c.Aps, c.partials);
-            });
-            std::snprintf(line, sizeof line, " %dx%d %.1f;", kTiles32[i].TT, kTiles32[i].TX, us);
+        for (size_t k = 0; k < cands32.size(); ++k) {
+            const int i = cands32[k];
+            const double t = us[cands.size() + k];
+            std::snprintf(line, sizeof line, " %dx%d %.1f;", kTiles32[i].TT, kTiles32[i].TX, t);
             c.tuneReport += line;
-            if (us < best) { best = us; best32 = i; }
+            if (t < best) { best = t; best32 = i; }
         }
         if (const char *f32 = std::getenv("YCN_TILE32")) {
             const int f = std::atoi(f32);
@@ -1092,7 +1108,7 @@ int mp_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
     rc = poll_active(c, active);
     if (rc) return rc;
     while (active > 0 && launched < hardCap) {
-        rc = run_chunk(c, phi, nsys, mtype, nIt, YCN_CG_MIXED, c.tile32,
+        rc = run_chunk(c, nullptr, nsys, mtype, nIt, YCN_CG_MIXED, c.tile32,
                        [&](int i) { launch_mp_iteration(c, nsys, mtype, i & 1); });
         if (rc) return rc;
         launched += nIt;
@@ -1193,7 +1209,7 @@ int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t
     rc = poll_active(c, active);
     if (rc) return rc;
     while (active > 0 && launched < hardCap) {
-        rc = run_chunk(c, phi, nsys, mtype, nIt, 2 /* flexible */, c.tile32,
+        rc = run_chunk(c, nullptr, nsys, mtype, nIt, 2 /* flexible */, c.tile32,
                        [&](int i) { launch_fcg_iteration(c, nsys, mtype, i & 1); });
         if (rc) return rc;
         launched += nIt;

@@ -12,6 +12,10 @@ constexpr int kGraphIters = 8;  // CG iterations per captured graph (even: keeps
 // (field, systems, operator, n, precision, tile) and replayed; every kernel reads its
 // scalars from device state and exits early for converged systems, so replays need no
 // host-side parameters.  No memset nodes are captured: all state changes are kernels.
+// `phi` is the field buffer the captured kernels read: one of the context's two
+// ping-pong buffers or its staging buffer for FP64 iterations, nullptr for the FP32
+// loops, which read the context's FP32 copy of the field; so each configuration has at
+// most two executables and no key outlives the buffer it names.
 template <typename Launch>
 inline int run_chunk(Ctx &c, const void *phi, int nsys, int mtype, int n, int precision, int tile, Launch launch) {
     if (!c.useGraphs || c.profTarget != PK_NONE) {

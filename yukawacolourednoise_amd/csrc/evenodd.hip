@@ -1,7 +1,7 @@
 // evenodd.hip -- even-odd preconditioned fermion force on gfx950.
 //
 // The drift only needs chi = (D^dag)^-1 eta (chi = D y with D^dag D y = eta,
-// FermionicDrift.cu:96-139).  With D^dag = A + K, A = diag(2r + M + g phi) and K the
+// FermionicDrift.cu:57-100).  With D^dag = A + K, A = diag(2r + M + g phi) and K the
 // hopping, which connects even and odd sites, the Schur complement on the even sites
 //   S = A_e - K_eo A_o^-1 K_oe,   b_e = eta_e - K_eo A_o^-1 eta_o,
 //   S chi_e = b_e,   chi_o = A_o^-1 (eta_o - K_oe chi_e)
@@ -205,7 +205,7 @@ k_eo_hop_eo(EoArgs A, const double *__restrict__ phi, const double2 *__restrict_
     }
 }
 
-// drift_x = fac Re sum_k conj(chi_xk) eta_xk (FermionicDrift.cu:122-139) with chi_e from
+// drift_x = fac Re sum_k conj(chi_xk) eta_xk (FermionicDrift.cu:83-100) with chi_e from
 // the solve and chi_o = A_o^-1 (eta_o - K_oe chi_e) formed on the fly; mean over noise
 // vectors.
 __global__ void __launch_bounds__(kNT)
@@ -1197,16 +1197,36 @@ int eo_autotune(Ctx &c) {
         return 1e3 * ms / reps;
     };
     char line[96];
-    c.tuneReport += " | even-odd fp64 tiles (us/launch):";
+    const bool pfOk = c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfSlot[0] && c.pfSlot[1];
+    // measured times: kNumEo64 FP64 tiles, then kNumEo32 x {separate, prologue} FP32
+    // pipelined iterations (1e30 where a tile does not fit); reused from an earlier
+    // context of the same shape when there is one
+    std::vector<double> us;
+    const bool memo = tune_lookup(1, c, us) && us.size() == (size_t)(kNumEo64 + 2 * kNumEo32);
+    if (!memo) {
+        us.assign(kNumEo64 + 2 * kNumEo32, 1e30);
+        for (int i = 0; i < kNumEo64; ++i)
+            if (eo_fits(c, kEoTiles64[i]))
+                us[i] = time_it([&](int k) {
+                    (void)launch_eo_fused(c, i, c.nSys, c.phi[c.cur], c.r, c.pv[k & 1], c.pv[(k & 1) ^ 1], c.Ap);
+                });
+        if (pfOk)
+            for (int i = 0; i < kNumEo32; ++i) {
+                if (!eo_fits(c, kEoTiles32[i])) continue;
+                us[kNumEo64 + 2 * i] =
+                    time_it([&](int k) { launch_fin_pf(c, c.nSys, launch_eo_pf(c, i, c.nSys, k, 0), 0); });
+                us[kNumEo64 + 2 * i + 1] = time_it([&](int k) { (void)launch_eo_pf(c, i, c.nSys, k, 2); });
+            }
+        tune_store(1, c, us);
+    }
+    c.tuneReport += memo ? " | even-odd fp64 tiles (us/launch, measured by an earlier context):"
+                         : " | even-odd fp64 tiles (us/launch):";
     double best = 1e30;
     for (int i = 0; i < kNumEo64; ++i) {
         if (!eo_fits(c, kEoTiles64[i])) continue;
-        const double us = time_it([&](int k) {
-            (void)launch_eo_fused(c, i, c.nSys, c.phi[c.cur], c.r, c.pv[k & 1], c.pv[(k & 1) ^ 1], c.Ap);
-        });
-        std::snprintf(line, sizeof line, " %dx%d %.1f;", kEoTiles64[i].TT, kEoTiles64[i].TX, us);
+        std::snprintf(line, sizeof line, " %dx%d %.1f;", kEoTiles64[i].TT, kEoTiles64[i].TX, us[i]);
         c.tuneReport += line;
-        if (us < best) { best = us; c.tileEo64 = i; }
+        if (us[i] < best) { best = us[i]; c.tileEo64 = i; }
     }
     // FP32 pipelined iteration: each tile with the separate finalize and with the
     // finalize in the next launch's prologue; YCN_EO_FIN=separate|fused restricts the
@@ -1215,14 +1235,15 @@ int eo_autotune(Ctx &c) {
     const bool allowSep = !fin || std::strcmp(fin, "fused") != 0;
     const bool allowFused = !fin || std::strcmp(fin, "separate") != 0;
     double us32[kNumEo32][2] = {};
-    if (c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfSlot[0] && c.pfSlot[1]) {
+    for (int i = 0; i < kNumEo32; ++i) {
+        us32[i][0] = us[kNumEo64 + 2 * i];
+        us32[i][1] = us[kNumEo64 + 2 * i + 1];
+    }
+    if (pfOk) {
         c.tuneReport += " | even-odd fp32 pipelined tiles (us/iteration, separate | prologue finalize):";
         best = 1e30;
         for (int i = 0; i < kNumEo32; ++i) {
-            us32[i][0] = us32[i][1] = 1e30;
             if (!eo_fits(c, kEoTiles32[i])) continue;
-            us32[i][0] = time_it([&](int k) { launch_fin_pf(c, c.nSys, launch_eo_pf(c, i, c.nSys, k, 0), 0); });
-            us32[i][1] = time_it([&](int k) { (void)launch_eo_pf(c, i, c.nSys, k, 2); });
             std::snprintf(line, sizeof line, " %dx%d%s %.1f|%.1f;", kEoTiles32[i].TT, kEoTiles32[i].TX,
                           kEoTiles32[i].persist ? "p" : "", us32[i][0], us32[i][1]);
             c.tuneReport += line;
@@ -1381,7 +1402,7 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
             const int rem = c.eoPredIters - itDone + 4;
             if (c.eoPredIters > 0 && rem < nIt) n = std::max(8, (rem + 7) / 8 * 8);
             const int slot = chunkProf ? prof_chunk_begin(c, PK_NORMAL_OP) : -1;
-            rc = run_chunk(c, phi, nsys, YCN_DAGGER, n, 4, c.tileEo32,
+            rc = run_chunk(c, nullptr, nsys, YCN_DAGGER, n, 4, c.tileEo32,
                            [&](int i) { launch_eo_pf_iteration(c, nsys, i); });
             prof_chunk_end(c, slot);
             if (rc) return rc;

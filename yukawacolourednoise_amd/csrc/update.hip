@@ -43,7 +43,7 @@ k_bosonic_max(const double *__restrict__ phi, double *__restrict__ drift, const 
 }
 
 // adaptive path, finalize (one block per replica): clamp max |drift| to [1e-5, 1e5]
-// (langevin_gpu_v2.cu:248-256), eps = epsBar * 30 / max (:277), Langevin time
+// (langevin_gpu_v2.cu:83-91), eps = epsBar * 30 / max (:112), Langevin time
 __global__ void __launch_bounds__(kUT)
 k_fin_eps(const double *__restrict__ part, int nb, double epsBar, int adaptive, StepState *__restrict__ st) {
     __shared__ double lds[kUT / kWave];

@@ -6,6 +6,7 @@
 #include <cmath>
 #include <cstdlib>
 #include <cstring>
+#include <mutex>
 #include <new>
 #include <string>
 #include <vector>
@@ -49,6 +50,50 @@ void prof_chunk_end(Ctx &c, int slot) {
     c.evCount = slot + 1;
 }
 
+// Process-wide memo of the autotuners' measured times: a context whose device,
+// lattice, system count, Wilson-r kernel variant and buffers match an earlier
+// context's reuses its timings (the choice, including any tile forced through the
+// environment, is still made per context).
+namespace {
+struct TuneKey {
+    int kind, device, Nt, Nx, nSys, r1, f32, pf;
+    bool operator==(const TuneKey &o) const {
+        return kind == o.kind && device == o.device && Nt == o.Nt && Nx == o.Nx && nSys == o.nSys && r1 == o.r1 &&
+               f32 == o.f32 && pf == o.pf;
+    }
+};
+std::mutex g_tuneMutex;
+std::vector<std::pair<TuneKey, std::vector<double>>> g_tune;
+TuneKey tune_key(int kind, const Ctx &c) {
+    return {kind, c.p.device, c.p.Nt, c.p.Nx, c.nSys, c.p.r == 1.0, c.rs != nullptr,
+            c.pfR1 != nullptr && c.pfA1 != nullptr && c.pfSlot[0] != nullptr && c.pfSlot[1] != nullptr};
+}
+}  // namespace
+
+bool tune_lookup(int kind, const Ctx &c, std::vector<double> &t) {
+    if (const char *m = std::getenv("YCN_TUNE_MEMO"))
+        if (std::atoi(m) == 0) return false;
+    const TuneKey k = tune_key(kind, c);
+    std::lock_guard<std::mutex> lock(g_tuneMutex);
+    for (const auto &e : g_tune)
+        if (e.first == k) {
+            t = e.second;
+            return true;
+        }
+    return false;
+}
+
+void tune_store(int kind, const Ctx &c, const std::vector<double> &t) {
+    const TuneKey k = tune_key(kind, c);
+    std::lock_guard<std::mutex> lock(g_tuneMutex);
+    for (auto &e : g_tune)
+        if (e.first == k) {
+            e.second = t;
+            return;
+        }
+    g_tune.push_back({k, t});
+}
+
 }  // namespace ycn
 
 #define CHECK_CTX(ctx)                                  \
@@ -88,7 +133,7 @@ void ycn_default_params(ycn_params *p) {
     p->cutFraction = 1.0;   // code/input.toml:7
     p->driftMode = 0;
     p->epsBar = 0.02;       // code/input.toml:10
-    p->adaptive = 1;        // langevin_gpu_v2.cu:277
+    p->adaptive = 1;        // langevin_gpu_v2.cu:112
     p->seed = 1234;         // code/input.toml:22
     p->cgTol = 1e-8;        // params.h:14
     p->cgTolMode = YCN_TOL_ABS_RECURSIVE;
@@ -115,7 +160,7 @@ void ycn_destroy(ycn_ctx *ctx) {
     cnn_free(c);
     if (c.pairR) (void)hipFree(c.pairR);
     if (c.pairX) (void)hipFree(c.pairX);
-    double **bufs[] = {&c.phi[0], &c.phi[1], &c.noise, &c.drift, &c.m2_d};
+    double **bufs[] = {&c.phi[0], &c.phi[1], &c.noise, &c.drift, &c.m2_d, &c.phiStage};
     for (double **b : bufs)
         if (*b) (void)hipFree(*b);
     if (c.seed_d) (void)hipFree(c.seed_d);
@@ -211,12 +256,6 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         c.err = "mixed-precision CG needs Nt % 4 == 0 and Nx % 16 == 0";
         return bail(YCN_ERR_UNSUPPORTED);
     }
-    c.evStart.resize(8192);
-    c.evStop.resize(8192);
-    c.evWork.assign(8192, -1);
-    for (size_t i = 0; i < c.evStart.size(); ++i)
-        if (hipEventCreate(&c.evStart[i]) != hipSuccess || hipEventCreate(&c.evStop[i]) != hipSuccess)
-            return bail(YCN_ERR_HIP);
     if (hipDeviceSynchronize() != hipSuccess) return bail(YCN_ERR_HIP);
     *out = ctx;
     return YCN_OK;
@@ -293,15 +332,17 @@ int ycn_coloured_filter(ycn_ctx *ctx, const double *xi, double *out) {
     return YCN_OK;
 }
 
-// phi for single-system entry points: replica 0 of the context or a host array
-static int stage_phi(Ctx &c, const double *phi_host, DevTmp &tmp, const double **phi_dev) {
+// phi for single-system entry points: replica 0 of the context, or a host array copied
+// into the context's staging buffer (one fixed address, so the captured CG chunks that
+// read it are keyed by a buffer that lives as long as the context)
+static int stage_phi(Ctx &c, const double *phi_host, const double **phi_dev) {
     if (!phi_host) {
         *phi_dev = c.phi[c.cur];
         return YCN_OK;
     }
-    YCN_HIP(hipMalloc(&tmp.p, sizeof(double) * c.V));
-    YCN_HIP(hipMemcpyAsync(tmp.p, phi_host, sizeof(double) * c.V, hipMemcpyHostToDevice, c.stream));
-    *phi_dev = (const double *)tmp.p;
+    if (!c.phiStage) YCN_HIP(hipMalloc(&c.phiStage, sizeof(double) * c.V));
+    YCN_HIP(hipMemcpyAsync(c.phiStage, phi_host, sizeof(double) * c.V, hipMemcpyHostToDevice, c.stream));
+    *phi_dev = c.phiStage;
     return YCN_OK;
 }
 
@@ -310,9 +351,9 @@ int ycn_dirac_apply(ycn_ctx *ctx, const double *phi, const double *in, double *o
     if (!in || !out || (mtype != YCN_NORMAL && mtype != YCN_DAGGER)) return YCN_ERR_INVALID;
     Ctx &c = *ctx;
     const size_t nb = sizeof(double2) * 4 * (size_t)c.V;
-    DevTmp ph, aos;
+    DevTmp aos;
     const double *phi_dev = nullptr;
-    int rc = stage_phi(c, phi, ph, &phi_dev);
+    int rc = stage_phi(c, phi, &phi_dev);
     if (rc) return rc;
     YCN_HIP(hipMalloc(&aos.p, nb));
     YCN_HIP(hipMemcpyAsync(aos.p, in, nb, hipMemcpyHostToDevice, c.stream));
@@ -344,9 +385,9 @@ int ycn_cg_solve(ycn_ctx *ctx, const double *phi, const double *b, double *x, in
     if (!b || !x || (mtype != YCN_NORMAL && mtype != YCN_DAGGER)) return YCN_ERR_INVALID;
     Ctx &c = *ctx;
     const size_t nb = sizeof(double2) * 4 * (size_t)c.V;
-    DevTmp ph, aos;
+    DevTmp aos;
     const double *phi_dev = nullptr;
-    int rc = stage_phi(c, phi, ph, &phi_dev);
+    int rc = stage_phi(c, phi, &phi_dev);
     if (rc) return rc;
     YCN_HIP(hipMalloc(&aos.p, nb));
     YCN_HIP(hipMemcpyAsync(aos.p, b, nb, hipMemcpyHostToDevice, c.stream));
@@ -599,9 +640,9 @@ int ycn_precond_loss_grad(ycn_ctx *ctx, const double *phi, const double *r, cons
         YCN_HIP(hipMalloc(&c.pairX, sizeof(double) * n));
         c.pairCap = 1;
     }
-    DevTmp ph, aos;
+    DevTmp aos;
     const double *phi_dev = nullptr;
-    if ((rc = stage_phi(c, phi, ph, &phi_dev))) return rc;
+    if ((rc = stage_phi(c, phi, &phi_dev))) return rc;
     YCN_HIP(hipMalloc(&aos.p, sizeof(double) * n));
     YCN_HIP(hipMemcpyAsync(aos.p, r, sizeof(double) * n, hipMemcpyHostToDevice, c.stream));
     if ((rc = aos_to_soa(c, (const double *)aos.p, c.pairR, 1))) return rc;
@@ -655,6 +696,21 @@ int ycn_profile_begin(ycn_ctx *ctx, const char *kernel) {
         if (std::strcmp(kernel, names[i]) == 0) k = i;
     if (k == PK_NONE) return fail(c, YCN_ERR_INVALID, std::string("unknown kernel ") + kernel);
     YCN_HIP(hipStreamSynchronize(c.stream));
+    if (c.evStart.empty()) {   // the timing events are made at the first profiled region
+        constexpr int kEvents = 8192;
+        c.evStart.reserve(kEvents);
+        c.evStop.reserve(kEvents);
+        for (int i = 0; i < kEvents; ++i) {
+            hipEvent_t a = nullptr, b = nullptr;
+            if (hipEventCreate(&a) != hipSuccess || hipEventCreate(&b) != hipSuccess) {
+                if (a) (void)hipEventDestroy(a);
+                return fail(c, YCN_ERR_HIP, "hipEventCreate failed");
+            }
+            c.evStart.push_back(a);
+            c.evStop.push_back(b);
+        }
+        c.evWork.assign(kEvents, -1);
+    }
     c.profTarget = k;
     c.evCount = 0;
     c.profBytes = 0;

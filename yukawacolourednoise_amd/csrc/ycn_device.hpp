@@ -176,7 +176,7 @@ __device__ __forceinline__ void wilson_flavour(const WilsonCoef &w, double d, V2
 
 // Gaussian noise vector of system sys at site s: component c is normal number 4s + c
 // of Philox stream (seed of the replica, streamBase + noise index), imaginary part 0
-// (replaces fillNormalRND, FermionicDrift.cu:141-149).
+// (replaces fillNormalRND, FermionicDrift.cu:102-110).
 __device__ __forceinline__ void eta_site(const uint64_t *seeds, uint32_t streamBase, uint64_t step, int sys,
                                          int nNoise, int s, double2 b[4]) {
     const int rep = sys / nNoise, j = sys - rep * nNoise;

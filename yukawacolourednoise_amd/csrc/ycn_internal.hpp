@@ -90,6 +90,7 @@ struct Ctx {
     double *phi[2] = {nullptr, nullptr};  // ping-pong for the fused fixed-eps update
     int cur = 0;
     double *noise = nullptr, *drift = nullptr;
+    double *phiStage = nullptr;           // a host field passed to a single-system entry point (V)
     double2 *spec = nullptr;              // half spectrum [rep][t][H]
     int *kmax = nullptr;                  // mask: keep kx <= kmax[kt]
     int kmaxAll = -1;
@@ -173,6 +174,10 @@ void prof_before(Ctx &c, int k, double bytes);
 void prof_after(Ctx &c, int k);
 int prof_chunk_begin(Ctx &c, int k);
 void prof_chunk_end(Ctx &c, int slot);
+
+// autotuner timing memo (ycn_abi.hip): kind 0 = normal-operator tiles, 1 = even-odd
+bool tune_lookup(int kind, const Ctx &c, std::vector<double> &t);
+void tune_store(int kind, const Ctx &c, const std::vector<double> &t);
 
 // noise.hip
 int noise_init(Ctx &c);

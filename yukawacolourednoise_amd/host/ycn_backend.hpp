@@ -54,7 +54,7 @@ private:
     const double *phi_ = nullptr;
 };
 
-// CGsolver::solve(in, out, D, MType) (CGsolver.cu:95-112)
+// CGsolver::solve(in, out, D, MType) (CGsolver.cu:42-59)
 class CGsolver {
 public:
     explicit CGsolver(Context &c) : c_(c) {}
@@ -64,7 +64,7 @@ private:
     Context &c_;
 };
 
-// FermionicDrift::getForce(out, ...) (FermionicDrift.cu:96-119)
+// FermionicDrift::getForce(out, ...) (FermionicDrift.cu:57-80)
 class FermionicDrift {
 public:
     explicit FermionicDrift(Context &c) : c_(c) {}
