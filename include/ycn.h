@@ -144,6 +144,23 @@ int ycn_coloured_noise(ycn_ctx *ctx, uint64_t step, double *noise);
 /* The filter alone on a given white field xi (nReplicas*V): for parity tests. */
 int ycn_coloured_filter(ycn_ctx *ctx, const double *xi, double *out);
 
+/* No reference counterpart.  ycn_coloured_noise without a context and without any
+ * allocation, for callers that cannot allocate device memory (code under HIP stream
+ * capture, serving loops, harnesses that forbid hipMalloc).  Writes the same field as
+ * ycn_coloured_noise of a context with these parameters (Nt, Nx, nReplicas,
+ * cutFraction, seed or seeds[nReplicas], device) into the DEVICE buffer noise
+ * (nReplicas*Nt*Nx doubles), enqueued on `stream` (a hipStream_t of that device; NULL
+ * = the null stream); the call returns once the work is enqueued.  Its scratch is a
+ * fixed device array of 8 479 232 bytes that the library holds on every device it runs
+ * on until the process ends; limits: Nt, Nx <= 2048, nReplicas <= 64 and
+ * nReplicas*Nt*(Nx/2+1) <= 1024*513 (one 1024^2 lattice); larger requests fail with
+ * YCN_ERR_INVALID before anything is enqueued.  Calls on one device share that array:
+ * calls that overlap on one device race (serialise them, e.g. on one stream).  The
+ * first call on a device without a context looks the array up (no allocation); errors
+ * are reported by ycn_last_error(NULL) on the calling thread. */
+int ycn_coloured_noise_noalloc(const ycn_params *p, const uint64_t *seeds, uint64_t step, double *noise,
+                               void *stream);
+
 /* DiracOP::applyD (code/src/Dirac.cu:23-47): out = D in (YCN_NORMAL) or D^dag in
  * (YCN_DAGGER) with the context's field of replica 0 (or phi if non-NULL). */
 int ycn_dirac_apply(ycn_ctx *ctx, const double *phi, const double *in, double *out, int mtype);

@@ -59,3 +59,21 @@ def test_no_cpu_fallback_without_device():
     with pytest.raises(Y.YcnError) as e:
         Y.Context(Y.default_params())
     assert e.value.code == abi.YCN_ERR_NO_DEVICE
+
+
+@pytest.mark.parametrize("field,value", [("Nt", 2049), ("Nx", 4096), ("nReplicas", 65), ("nReplicas", 0), ("Nt", 1)])
+def test_noalloc_noise_rejects_sizes_beyond_its_scratch(field, value):
+    # validated on the host before any device call: no GPU needed
+    p = Y.default_params(Nt=64, Nx=64)
+    setattr(p, field, value)
+    assert abi.coloured_noise_noalloc(p, 0, 0x1000, 0) == abi.YCN_ERR_INVALID
+    assert b"ycn_coloured_noise_noalloc" in Y.lib.ycn_last_error(None)
+
+
+def test_noalloc_noise_rejects_a_lattice_beyond_one_1024x1024_spectrum():
+    # 2 replicas of 1024^2 (or one 2048x1024 lattice) need twice the half-spectrum capacity
+    for kw in (dict(Nt=1024, Nx=1024, nReplicas=2), dict(Nt=2048, Nx=1024, nReplicas=1)):
+        p = Y.default_params(**kw)
+        assert abi.coloured_noise_noalloc(p, 0, 0x1000, 0) == abi.YCN_ERR_INVALID
+        assert b"scratch capacity" in Y.lib.ycn_last_error(None)
+    assert Y.lib.ycn_coloured_noise_noalloc(None, None, 0, C.c_void_p(0x1000), None) == abi.YCN_ERR_INVALID

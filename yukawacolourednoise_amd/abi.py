@@ -36,7 +36,7 @@ EXPORTED = [
     "ycn_default_params", "ycn_create", "ycn_destroy", "ycn_last_error", "ycn_abi_version",
     "ycn_tuning_report",
     "ycn_set_replicas", "ycn_set_eps_bar", "ycn_set_field", "ycn_get_field", "ycn_hot_start",
-    "ycn_coloured_noise", "ycn_coloured_filter", "ycn_dirac_apply", "ycn_cg_solve",
+    "ycn_coloured_noise", "ycn_coloured_filter", "ycn_coloured_noise_noalloc", "ycn_dirac_apply", "ycn_cg_solve",
     "ycn_fermion_force", "ycn_langevin_step", "ycn_langevin_run", "ycn_measure", "ycn_sync",
     "ycn_profile_begin", "ycn_profile_end",
     "ycn_precond_set_weights", "ycn_precond_get_weights", "ycn_precond_apply", "ycn_precond_loss_grad",
@@ -112,6 +112,7 @@ def _load():
     L.ycn_hot_start.argtypes = [vp, C.c_uint64]
     L.ycn_coloured_noise.argtypes = [vp, C.c_uint64, dp]
     L.ycn_coloured_filter.argtypes = [vp, dp, dp]
+    L.ycn_coloured_noise_noalloc.argtypes = [C.POINTER(Params), C.POINTER(C.c_uint64), C.c_uint64, vp, vp]
     L.ycn_dirac_apply.argtypes = [vp, dp, dp, dp, C.c_int]
     L.ycn_cg_solve.argtypes = [vp, dp, dp, dp, C.c_int, C.POINTER(CgStats)]
     L.ycn_fermion_force.argtypes = [vp, C.c_uint64, dp, C.POINTER(CgStats)]
@@ -143,6 +144,20 @@ def default_params(**kw) -> Params:
             raise AttributeError(k)
         setattr(p, k, v)
     return p
+
+
+def coloured_noise_noalloc(params: Params, step: int, noise_dev_ptr: int, stream_ptr: int = 0, seeds=None):
+    """ycn_coloured_noise_noalloc: the coloured noise of ``params`` written into a device
+    buffer (``noise_dev_ptr``, nReplicas*Nt*Nx doubles) on a HIP stream, with no context
+    and no allocation.  Returns the ABI code; raises nothing (callers check the code and
+    ``lib.ycn_last_error(None)``)."""
+    s = None
+    if seeds is not None:
+        s = np.ascontiguousarray(seeds, dtype=np.uint64)
+        assert s.size == params.nReplicas
+    return lib.ycn_coloured_noise_noalloc(C.byref(params),
+                                          None if s is None else s.ctypes.data_as(C.POINTER(C.c_uint64)),
+                                          step, C.c_void_p(noise_dev_ptr or None), C.c_void_p(stream_ptr or None))
 
 
 def _ptr(a):

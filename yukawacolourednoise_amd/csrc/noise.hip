@@ -16,6 +16,7 @@
 #include <math.h>
 
 #include <algorithm>
+#include <mutex>
 #include <vector>
 
 #include "ycn_device.hpp"
@@ -227,10 +228,22 @@ k_noise_rows_inv(const double2 *__restrict__ spec, double *__restrict__ noise, i
     }
 }
 
+// twiddle table tw[e] = exp(-2 pi i e / n), e < n (both entry points build it this way,
+// so their transforms agree bit for bit)
+__global__ void __launch_bounds__(kFftThreads) k_twiddles(double2 *__restrict__ tw, int n) {
+    const int e = blockIdx.x * kFftThreads + threadIdx.x;
+    if (e >= n) return;
+    double s, c;
+    sincospi(-2.0 * (double)e / (double)n, &s, &c);
+    tw[e] = make_double2(c, s);
+}
+
 // ---------------------------------------------------------------------------
 // host side
 // ---------------------------------------------------------------------------
-static int make_plan(Ctx &c, int n, FftPlan &P) {
+// radix factors of n: 4s, then 2s, then odd primes (direct DFTs); false if more than
+// kMaxFactors are needed
+static bool factor_plan(int n, FftPlan &P) {
     P.n = n;
     P.nf = 0;
     int m = n;
@@ -238,15 +251,15 @@ static int make_plan(Ctx &c, int n, FftPlan &P) {
     while (m % 2 == 0 && P.nf < kMaxFactors) { P.radix[P.nf++] = 2; m /= 2; }
     for (int q = 3; m > 1 && P.nf < kMaxFactors; q += 2)
         while (m % q == 0 && P.nf < kMaxFactors) { P.radix[P.nf++] = q; m /= q; }
-    if (m != 1) { c.err = "FFT length has too many factors"; return YCN_ERR_UNSUPPORTED; }
-    std::vector<double2> tw(n);
-    for (int e = 0; e < n; ++e) {
-        const long double ang = -2.0L * 3.141592653589793238462643383279502884L * e / n;
-        tw[e] = make_double2((double)cosl(ang), (double)sinl(ang));
-    }
+    return m == 1;
+}
+
+static int make_plan(Ctx &c, int n, FftPlan &P) {
+    if (!factor_plan(n, P)) { c.err = "FFT length has too many factors"; return YCN_ERR_UNSUPPORTED; }
     double2 *d = nullptr;
     YCN_HIP(hipMalloc(&d, sizeof(double2) * n));
-    YCN_HIP(hipMemcpy(d, tw.data(), sizeof(double2) * n, hipMemcpyHostToDevice));
+    hipLaunchKernelGGL(k_twiddles, dim3((n + kFftThreads - 1) / kFftThreads), dim3(kFftThreads), 0, c.stream, d, n);
+    YCN_HIP(hipGetLastError());
     P.tw = d;
     return YCN_OK;
 }
@@ -268,10 +281,15 @@ static double momentum_squared(int Nt, int Nx, int ct, int cx) {
     return msq * sgn;
 }
 
+// (s sqrt(d) pi)^2, the cutoff of colouredNoise.cpp:138-145 in its arithmetic order
+static double cut_squared(double cutFraction) {
+    const double cut = std::sqrt(2.0) * M_PI * cutFraction;
+    return cut * cut;
+}
+
 static int build_mask(Ctx &c, std::vector<int> &kmax) {
     const int Nt = c.p.Nt, Nx = c.p.Nx, H = c.H;
-    const double cut = std::sqrt(2.0) * M_PI * c.p.cutFraction;
-    const double cutSq = cut * cut;
+    const double cutSq = cut_squared(c.p.cutFraction);
     kmax.assign(Nt, -1);
     for (int t = 0; t < Nt; ++t) {
         bool prefix = true;
@@ -326,25 +344,180 @@ int noise_generate(Ctx &c, uint64_t step, const double *xi_dev, double *out_dev)
     return noise_generate_on(c, c.stream, step, xi_dev, out_dev);
 }
 
-// the three noise kernels on stream s (the context stream, or the step's noise stream)
-int noise_generate_on(Ctx &c, hipStream_t s, uint64_t step, const double *xi_dev, double *out_dev) {
-    const int Nt = c.p.Nt, Nx = c.p.Nx, H = c.H;
+// the three noise kernels on stream s over explicit buffers (a context's, or the
+// scratch array of the allocation-free entry point); prof (optional) brackets the
+// column pass for a context's kernel timing
+static int launch_noise(hipStream_t s, int Nt, int Nx, int nRep, const double *xi_dev, const uint64_t *seeds,
+                        uint64_t step, double2 *spec, const int *kmax, int kmaxAll, const FftPlan &planX,
+                        const FftPlan &planT, double *out_dev, Ctx *prof) {
+    const int H = Nx / 2 + 1;
     const int lx = lines_per_block(Nx), lt = lines_per_block(Nt);
     const int npairs = (Nt + 1) / 2;
-    dim3 g1((npairs + lx - 1) / lx, c.nRep), g2((H + lt - 1) / lt, c.nRep);
+    dim3 g1((npairs + lx - 1) / lx, nRep), g2((H + lt - 1) / lt, nRep);
     const size_t sh1 = (size_t)2 * lx * Nx * sizeof(double2), sh2 = (size_t)2 * lt * Nt * sizeof(double2);
-    hipLaunchKernelGGL(k_noise_rows_fwd, g1, dim3(kFftThreads), sh1, s, xi_dev, (const uint64_t *)c.seed_d,
-                       step, c.spec, Nt, Nx, H, lx, c.planX);
-    YCN_HIP(hipGetLastError());
-    prof_before(c, PK_NOISE_COLS, 32.0 * (double)c.nRep * Nt * H);
-    hipLaunchKernelGGL(k_noise_cols, g2, dim3(kFftThreads), sh2, s, c.spec, (const int *)c.kmax, c.kmaxAll,
-                       1.0 / (double)c.V, Nt, H, lt, c.planT);
-    prof_after(c, PK_NOISE_COLS);
-    YCN_HIP(hipGetLastError());
-    hipLaunchKernelGGL(k_noise_rows_inv, g1, dim3(kFftThreads), sh1, s, (const double2 *)c.spec, out_dev, Nt,
-                       Nx, H, lx, c.planX);
-    YCN_HIP(hipGetLastError());
+    hipLaunchKernelGGL(k_noise_rows_fwd, g1, dim3(kFftThreads), sh1, s, xi_dev, seeds, step, spec, Nt, Nx, H, lx,
+                       planX);
+    if (prof) prof_before(*prof, PK_NOISE_COLS, 32.0 * (double)nRep * Nt * H);
+    hipLaunchKernelGGL(k_noise_cols, g2, dim3(kFftThreads), sh2, s, spec, kmax, kmaxAll,
+                       1.0 / ((double)Nt * (double)Nx), Nt, H, lt, planT);
+    if (prof) prof_after(*prof, PK_NOISE_COLS);
+    hipLaunchKernelGGL(k_noise_rows_inv, g1, dim3(kFftThreads), sh1, s, (const double2 *)spec, out_dev, Nt, Nx, H,
+                       lx, planX);
+    return hipGetLastError() == hipSuccess ? YCN_OK : YCN_ERR_HIP;
+}
+
+int noise_generate_on(Ctx &c, hipStream_t s, uint64_t step, const double *xi_dev, double *out_dev) {
+    const int rc = launch_noise(s, c.p.Nt, c.p.Nx, c.nRep, xi_dev, (const uint64_t *)c.seed_d, step, c.spec,
+                                (const int *)c.kmax, c.kmaxAll, c.planX, c.planT, out_dev, &c);
+    if (rc) c.err = "coloured-noise kernel launch failed";
+    return rc;
+}
+
+// ===========================================================================
+// Coloured noise without a context and without allocations
+// (ycn_coloured_noise_noalloc, for callers that cannot allocate device memory).
+// Its scratch -- the half spectrum, the two twiddle tables, the mask rows and the
+// replica seeds -- lives in one file-scope device array of fixed size.  Each device
+// holds its own copy at its own address, looked up once per device with
+// hipGetSymbolAddress; host code uses only that address, never the array's name.
+// Every call on a device shares the array: calls that overlap on one device race.
+// Nothing in it needs clearing: each call writes every entry it reads.
+// ===========================================================================
+constexpr size_t kNaSpec = (size_t)1024 * 513;   // half-spectrum entries (double2): one 1024^2 lattice
+constexpr int kNaMaxN = 2048;                     // largest extent (the LDS FFT's limit)
+constexpr int kNaMaxRep = 64;                     // replicas (their seeds pass by value)
+constexpr int kNaMaxDevices = 64;
+constexpr size_t kNaOffTwX = kNaSpec * sizeof(double2);
+constexpr size_t kNaOffTwT = kNaOffTwX + kNaMaxN * sizeof(double2);
+constexpr size_t kNaOffKmax = kNaOffTwT + kNaMaxN * sizeof(double2);
+constexpr size_t kNaOffSeed = kNaOffKmax + kNaMaxN * sizeof(int);
+constexpr size_t kNaBytes = kNaOffSeed + kNaMaxRep * sizeof(uint64_t);   // 8 479 232 B
+
+static __device__ __align__(256) unsigned char scratch[kNaBytes];
+
+static void *g_naScratch[kNaMaxDevices];   // the array's address on each device
+static std::mutex g_naMutex;
+static thread_local const char *g_naErr = nullptr;
+
+// address of `scratch` on device dev, which must be current; looked up once per device
+static int na_scratch(int dev, unsigned char **out) {
+    std::lock_guard<std::mutex> lock(g_naMutex);
+    if (!g_naScratch[dev]) {
+        void *ptr = nullptr;
+        if (hipGetSymbolAddress(&ptr, HIP_SYMBOL(scratch)) != hipSuccess || !ptr) return YCN_ERR_HIP;
+        g_naScratch[dev] = ptr;
+    }
+    *out = (unsigned char *)g_naScratch[dev];
+    return YCN_OK;
+}
+
+int noise_noalloc_setup(int dev) {
+    unsigned char *p = nullptr;
+    if (dev < 0 || dev >= kNaMaxDevices) return YCN_ERR_INVALID;
+    return na_scratch(dev, &p);
+}
+
+const char *noise_noalloc_error() { return g_naErr; }
+
+struct NaSeeds {
+    uint64_t v[kNaMaxRep];
+};
+__global__ void k_na_seeds(NaSeeds s, uint64_t *__restrict__ dst, int n) {
+    if ((int)threadIdx.x < n) dst[threadIdx.x] = s.v[threadIdx.x];
+}
+
+// kmax[kt] of the mask on the device, in the host's IEEE order (momentum_squared, no
+// contraction; correctly rounded double multiply, divide and add), so both entry points
+// keep the same modes.  The kept set of a row is a prefix (checked on the host path).
+#pragma clang fp contract(off)
+__device__ double na_coord_sq(int c, int n) {
+    const double k = (double)(c <= n / 2 ? c : c - n);
+    const double t = __ddiv_rn(__dmul_rn(__dmul_rn(k, 2.0), M_PI), (double)n);
+    return __dmul_rn(t, t);
+}
+__global__ void __launch_bounds__(kFftThreads) k_na_mask(int *__restrict__ kmax, int Nt, int Nx, int H, double cutSq) {
+    const int kt = blockIdx.x * kFftThreads + threadIdx.x;
+    if (kt >= Nt) return;
+    const double mt = __dadd_rn(0.0, na_coord_sq(kt, Nt));
+    int km = -1;
+    for (int k = 0; k < H; ++k) {
+        const double m = __dadd_rn(mt, na_coord_sq(k, Nx));
+        if (!(m >= 0.0 && m <= cutSq)) break;
+        km = k;
+    }
+    kmax[kt] = km;
+}
+#pragma clang fp contract(on)
+
+// largest kept kx over all rows: row kt = 0 (p_t = 0) keeps the most, since adding
+// p_t^2 >= 0 and rounding cannot lower the momentum
+static int na_kmax_all(int Nt, int Nx, double cutSq) {
+    int km = -1;
+    for (int k = 0; k < Nx / 2 + 1; ++k) {
+        const double m = momentum_squared(Nt, Nx, 0, k);
+        if (!(m >= 0 && m <= cutSq)) break;
+        km = k;
+    }
+    return km;
+}
+
+static int na_fail(const char *msg, int code) {
+    g_naErr = msg;
+    return code;
+}
+
+static int na_run(const ycn_params *p, const uint64_t *seeds, uint64_t step, double *noise, hipStream_t s) {
+    const int Nt = p->Nt, Nx = p->Nx, nRep = p->nReplicas, H = Nx / 2 + 1;
+    FftPlan pX{}, pT{};
+    if (!factor_plan(Nx, pX) || !factor_plan(Nt, pT)) return na_fail("FFT length has too many factors", YCN_ERR_UNSUPPORTED);
+    unsigned char *base = nullptr;
+    if (na_scratch(p->device, &base)) return na_fail("hipGetSymbolAddress of the scratch array failed", YCN_ERR_HIP);
+    double2 *spec = (double2 *)base;
+    double2 *twX = (double2 *)(base + kNaOffTwX), *twT = (double2 *)(base + kNaOffTwT);
+    int *kmax = (int *)(base + kNaOffKmax);
+    uint64_t *seed_d = (uint64_t *)(base + kNaOffSeed);
+    NaSeeds sd{};
+    for (int i = 0; i < nRep; ++i) sd.v[i] = seeds ? seeds[i] : p->seed;
+    const double cutSq = cut_squared(p->cutFraction);
+    hipLaunchKernelGGL(k_na_seeds, dim3(1), dim3(kNaMaxRep), 0, s, sd, seed_d, nRep);
+    hipLaunchKernelGGL(k_twiddles, dim3((Nx + kFftThreads - 1) / kFftThreads), dim3(kFftThreads), 0, s, twX, Nx);
+    hipLaunchKernelGGL(k_twiddles, dim3((Nt + kFftThreads - 1) / kFftThreads), dim3(kFftThreads), 0, s, twT, Nt);
+    hipLaunchKernelGGL(k_na_mask, dim3((Nt + kFftThreads - 1) / kFftThreads), dim3(kFftThreads), 0, s, kmax, Nt, Nx,
+                       H, cutSq);
+    pX.tw = twX;
+    pT.tw = twT;
+    if (launch_noise(s, Nt, Nx, nRep, nullptr, seed_d, step, spec, kmax, na_kmax_all(Nt, Nx, cutSq), pX, pT, noise,
+                     nullptr))
+        return na_fail("coloured-noise kernel launch failed", YCN_ERR_HIP);
     return YCN_OK;
 }
 
 }  // namespace ycn
+
+extern "C" int ycn_coloured_noise_noalloc(const ycn_params *p, const uint64_t *seeds, uint64_t step, double *noise,
+                                          void *stream) {
+    using namespace ycn;
+    if (!p || !noise) return na_fail("ycn_coloured_noise_noalloc: null params or output", YCN_ERR_INVALID);
+    if (p->Nt < 2 || p->Nx < 2 || p->nReplicas < 1)
+        return na_fail("ycn_coloured_noise_noalloc: needs Nt, Nx >= 2 and nReplicas >= 1", YCN_ERR_INVALID);
+    if (p->Nt > kNaMaxN || p->Nx > kNaMaxN)
+        return na_fail("ycn_coloured_noise_noalloc: Nt and Nx are limited to 2048", YCN_ERR_INVALID);
+    if (p->nReplicas > kNaMaxRep)
+        return na_fail("ycn_coloured_noise_noalloc: nReplicas is limited to 64", YCN_ERR_INVALID);
+    if ((size_t)p->nReplicas * p->Nt * (p->Nx / 2 + 1) > kNaSpec)
+        return na_fail("ycn_coloured_noise_noalloc: nReplicas * Nt * (Nx/2 + 1) exceeds the scratch capacity "
+                       "1024 * 513 (one 1024^2 lattice)",
+                       YCN_ERR_INVALID);
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev <= 0)
+        return na_fail("ycn_coloured_noise_noalloc: no HIP device", YCN_ERR_NO_DEVICE);
+    if (p->device < 0 || p->device >= ndev || p->device >= kNaMaxDevices)
+        return na_fail("ycn_coloured_noise_noalloc: invalid device", YCN_ERR_INVALID);
+    int prev = -1;
+    if (hipGetDevice(&prev) != hipSuccess) return na_fail("ycn_coloured_noise_noalloc: hipGetDevice failed", YCN_ERR_HIP);
+    if (prev != p->device && hipSetDevice(p->device) != hipSuccess)
+        return na_fail("ycn_coloured_noise_noalloc: hipSetDevice failed", YCN_ERR_HIP);
+    const int rc = na_run(p, seeds, step, noise, (hipStream_t)stream);
+    if (prev != p->device) (void)hipSetDevice(prev);
+    return rc;
+}

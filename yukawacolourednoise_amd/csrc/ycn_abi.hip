@@ -144,7 +144,11 @@ void ycn_default_params(ycn_params *p) {
     p->device = 0;
 }
 
-const char *ycn_last_error(const ycn_ctx *ctx) { return ctx ? ctx->err.c_str() : "null context"; }
+const char *ycn_last_error(const ycn_ctx *ctx) {
+    if (ctx) return ctx->err.c_str();
+    const char *e = noise_noalloc_error();   // context-free calls of this thread
+    return e ? e : "null context";
+}
 
 const char *ycn_tuning_report(const ycn_ctx *ctx) { return ctx ? ctx->tuneReport.c_str() : ""; }
 
@@ -207,6 +211,9 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         return code;
     };
     if (hipSetDevice(p->device) != hipSuccess) return bail(YCN_ERR_HIP);
+    // the allocation-free entry point's scratch address on this device (looked up at
+    // the first context, or at that entry point's first call on a device without one)
+    (void)noise_noalloc_setup(p->device);
     if (hipStreamCreateWithFlags(&c.stream, hipStreamNonBlocking) != hipSuccess) return bail(YCN_ERR_HIP);
     // the noise stream is created at the first step that uses it; YCN_NOISE_STREAM=0
     // keeps the coloured noise on the context stream

@@ -185,6 +185,10 @@ void noise_free(Ctx &c);
 // xi == nullptr: Philox field of (seeds, stream 0, step); else filter xi (device, nRep*V)
 int noise_generate(Ctx &c, uint64_t step, const double *xi_dev, double *out_dev);
 int noise_generate_on(Ctx &c, hipStream_t s, uint64_t step, const double *xi_dev, double *out_dev);
+// the allocation-free entry point's scratch: address lookup on device dev (current),
+// and the message of the last failed call on this thread (nullptr if none)
+int noise_noalloc_setup(int dev);
+const char *noise_noalloc_error();
 
 // cg.hip
 struct SolveStats {
