@@ -197,28 +197,35 @@ int ycn_sync(ycn_ctx *ctx);
 
 /* ---- learned preconditioner (no reference counterpart) ----
  * A 3-layer periodic 3x3 CNN, 8 -> 16 -> 16 -> 8 channels (channel 2c / 2c+1 = Re / Im
- * of spinor component c), no bias, tanh between layers, BF16 weights and activations
- * with FP32 accumulation on MFMA, as a learned correction to an exact skip path:
- * z = c0 r + s * net(r / s), c0 = 1 / (2r + M)^2, s = RMS of r's 8V real entries.
+ * of spinor component c), no bias, BF16 weights and activations with FP32 accumulation
+ * on MFMA.  The data path is linear in r and each hidden activation is gated by the
+ * field: u1 = (W1*u0) . (1 + tanh(G1*q)), u2 = (W2*u1) . (1 + tanh(G2*q)), y = W3*u2,
+ * q = g phi; so the preconditioner is a linear map of r whose stencil follows the local
+ * Wilson diagonal.  It is a learned correction to an exact skip path:
+ * z = c0 r + s * y(r / s), c0 = 1 / (2r + M)^2, s = RMS of r's 8V real entries.
  * The default network has a zero output layer (z = c0 r, i.e. plain CG).
- * Weights (YCN_CNN_NWEIGHTS floats): W1[9][8][16], W2[9][16][16], W3[9][16][8] as
- * [tap][in][out], tap = (dt+1)*3 + (dx+1), out(site) = sum W * in(site + (dt,dx)).
+ * Weights (YCN_CNN_NWEIGHTS floats): W1[9][8][16], W2[9][16][16], W3[9][16][8],
+ * G1[9][16], G2[9][16] as [tap][in][out], tap = (dt+1)*3 + (dx+1),
+ * out(site) = sum W * in(site + (dt,dx)).
  * With precond = YCN_PRECOND_CNN the fermion-force solves run flexible CG with this
  * network (FP32 inner iterations, FP64 reliable updates, plain CG on stall); the FP64
  * true residual with the plain operator decides convergence.  Needs Nt, Nx % 16 == 0. */
-#define YCN_CNN_NWEIGHTS 4608
+#define YCN_CNN_NWEIGHTS 4896
 typedef struct {
     int pairs;             /* (residual, error) pairs trained on                        */
-    double loss_first;     /* energy-norm loss of the first pair (before its update)    */
+    double loss_first;     /* loss of the first pair (before its update)                */
     double loss_last;      /* loss of the last pair                                     */
     double eps_last;       /* Langevin step size of the last training step              */
 } ycn_train_stats;
 int ycn_precond_set_weights(ycn_ctx *ctx, const float *w);
 int ycn_precond_get_weights(ycn_ctx *ctx, float *w);
-/* z = CNN(r) for one AoS spinor (host arrays, 8V doubles each) */
+/* z = CNN(r) for one AoS spinor (host arrays, 8V doubles each); the gates read the
+ * context's replica-0 field */
 int ycn_precond_apply(ycn_ctx *ctx, const double *r, double *z);
-/* loss L = <e - z, A (e - z)> / <e, r> (A = D^dag D at phi, or the replica-0 field if
- * phi is NULL) for z = CNN(r), and dL/dW (YCN_CNN_NWEIGHTS floats); no update */
+/* loss L = |A (e - z)|^2 / |r|^2 (A = D^dag D at phi, or the replica-0 field if phi is
+ * NULL; for r = A e this is |r - A z|^2 / |r|^2, the residual the correction leaves)
+ * for z = CNN(r) with gates from the same field, and dL/dW (YCN_CNN_NWEIGHTS floats);
+ * no update */
 int ycn_precond_loss_grad(ycn_ctx *ctx, const double *phi, const double *r, const double *e, double *loss,
                           float *grad);
 /* Online training: nsteps Langevin steps from counter step0 (they advance the chain);

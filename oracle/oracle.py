@@ -80,7 +80,7 @@ def lib():
         L.orc_sum.argtypes = [C.c_int64, dp]
         L.orc_sum.restype = C.c_double
         fp = np.ctypeslib.ndpointer(dtype=np.float32, flags="C_CONTIGUOUS")
-        L.orc_cnn_forward.argtypes = [C.c_int, C.c_int, fp, C.c_int, C.c_double, dp, dp]
+        L.orc_cnn_forward.argtypes = [C.c_int, C.c_int, fp, C.c_int, C.c_double, C.c_double, dp, dp, dp]
         L.orc_cnn_loss_grad.argtypes = [pp, dp, fp, C.c_double, dp, dp, dp]
         L.orc_cnn_loss_grad.restype = C.c_double
         L.orc_bf16_round.argtypes = [C.c_double]
@@ -291,7 +291,7 @@ def eo_cg(p: OrcParams, phi, eta, max_iter_override=0):
     return _f64_to_spin(chi, V), st.as_dict()
 
 
-CNN_NW = 9 * 8 * 16 + 9 * 16 * 16 + 9 * 16 * 8
+CNN_NW = 9 * 8 * 16 + 9 * 16 * 16 + 9 * 16 * 8 + 2 * 9 * 16   # W1, W2, W3, gates G1, G2
 
 
 def cnn_c0(r=1.0, M=0.3):
@@ -299,12 +299,14 @@ def cnn_c0(r=1.0, M=0.3):
     return 1.0 / (2.0 * r + M) ** 2
 
 
-def cnn_forward(Nt, Nx, W, r, bf16=True, c0=0.0):
-    """z = c0 r + s net(r/s) for an AoS (V, 4) complex spinor; bf16 emulates the GPU's
-    rounding of weights and activations."""
+def cnn_forward(Nt, Nx, W, r, bf16=True, c0=0.0, phi=None, g=0.0):
+    """z = c0 r + s net(r/s) for an AoS (V, 4) complex spinor, the hidden layers gated by
+    q = g phi (phi None: q = 0, gates 1 + tanh(0) = 1); bf16 emulates the GPU's rounding
+    of weights and activations."""
     W = np.ascontiguousarray(W, dtype=np.float32)
+    ph = np.zeros(Nt * Nx) if phi is None else _f64(phi).reshape(-1)
     out = np.empty(8 * Nt * Nx, np.float64)
-    lib().orc_cnn_forward(Nt, Nx, W, int(bf16), c0, _spin_to_f64(r), out)
+    lib().orc_cnn_forward(Nt, Nx, W, int(bf16), c0, g, ph, _spin_to_f64(r), out)
     return _f64_to_spin(out, Nt * Nx)
 
 

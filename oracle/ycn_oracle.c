@@ -762,7 +762,8 @@ double orc_bf16_round(double v) {
     return (double)f;
 }
 
-static const int cnn_woff[3] = {0, 9 * 8 * 16, 9 * 8 * 16 + 9 * 16 * 16};
+static const int cnn_woff[5] = {0, 9 * 8 * 16, 9 * 8 * 16 + 9 * 16 * 16, 9 * 8 * 16 + 9 * 16 * 16 + 9 * 16 * 8,
+                                9 * 8 * 16 + 9 * 16 * 16 + 9 * 16 * 8 + 9 * 16};
 
 /* y[site][co] = sum_tap sum_ci W[tap][ci][co] u[site + off(tap)][ci], periodic */
 static void conv3x3(int Nt, int Nx, const double *W, int cin, int cout, const double *u, double *y) {
@@ -818,62 +819,108 @@ static void conv3x3_wgrad(int Nt, int Nx, int cin, int cout, const double *u, co
     }
 }
 
-/* forward pass keeping the activations; returns the input scale s */
-static double cnn_fwd(int Nt, int Nx, const float *Wf, int bf16, const double *r, double *u0, double *y1,
-                      double *u1, double *y2, double *u2, double *y3) {
+/* forward pass keeping the activations (y: convolution outputs, gm: gates, u: gated
+ * activations); returns the input scale s */
+typedef struct {
+    double *u0, *y1, *gm1, *u1, *y2, *gm2, *u2, *y3, *q;
+} cnn_acts;
+
+static cnn_acts cnn_acts_alloc(size_t V) {
+    cnn_acts a;
+    a.u0 = malloc(sizeof(double) * 8 * V);
+    a.y1 = malloc(sizeof(double) * 16 * V);
+    a.gm1 = malloc(sizeof(double) * 16 * V);
+    a.u1 = malloc(sizeof(double) * 16 * V);
+    a.y2 = malloc(sizeof(double) * 16 * V);
+    a.gm2 = malloc(sizeof(double) * 16 * V);
+    a.u2 = malloc(sizeof(double) * 16 * V);
+    a.y3 = malloc(sizeof(double) * 8 * V);
+    a.q = malloc(sizeof(double) * V);
+    return a;
+}
+
+static void cnn_acts_free(cnn_acts *a) {
+    free(a->u0); free(a->y1); free(a->gm1); free(a->u1); free(a->y2);
+    free(a->gm2); free(a->u2); free(a->y3); free(a->q);
+}
+
+static double cnn_fwd(int Nt, int Nx, const float *Wf, int bf16, double g, const double *phi, const double *r,
+                      cnn_acts *a) {
     const size_t V = (size_t)Nt * Nx;
     double W[ORC_CNN_NW];
-    for (int i = 0; i < ORC_CNN_NW; ++i) W[i] = bf16 ? orc_bf16_round(Wf[i]) : (double)Wf[i];
+    for (int i = 0; i < ORC_CNN_NW; ++i)
+        W[i] = (bf16 && i < cnn_woff[3]) ? orc_bf16_round(Wf[i]) : (double)Wf[i];
     double ss = 0.0;
     for (size_t i = 0; i < 8 * V; ++i) ss += r[i] * r[i];
     double s = sqrt(ss / (double)(8 * V));
     double inv = s > 0.0 ? 1.0 / s : 0.0;
-    for (size_t i = 0; i < 8 * V; ++i) u0[i] = bf16 ? orc_bf16_round(r[i] * inv) : r[i] * inv;
-    conv3x3(Nt, Nx, W + cnn_woff[0], 8, 16, u0, y1);
-    for (size_t i = 0; i < 16 * V; ++i) u1[i] = bf16 ? orc_bf16_round(tanh(y1[i])) : tanh(y1[i]);
-    conv3x3(Nt, Nx, W + cnn_woff[1], 16, 16, u1, y2);
-    for (size_t i = 0; i < 16 * V; ++i) u2[i] = bf16 ? orc_bf16_round(tanh(y2[i])) : tanh(y2[i]);
-    conv3x3(Nt, Nx, W + cnn_woff[2], 16, 8, u2, y3);
+    for (size_t i = 0; i < V; ++i) a->q[i] = g * phi[i];
+    for (size_t i = 0; i < 8 * V; ++i) a->u0[i] = bf16 ? orc_bf16_round(r[i] * inv) : r[i] * inv;
+    conv3x3(Nt, Nx, W + cnn_woff[0], 8, 16, a->u0, a->y1);
+    conv3x3(Nt, Nx, W + cnn_woff[3], 1, 16, a->q, a->gm1);
+    for (size_t i = 0; i < 16 * V; ++i) {
+        a->gm1[i] = 1.0 + tanh(a->gm1[i]);
+        a->u1[i] = bf16 ? orc_bf16_round(a->y1[i] * a->gm1[i]) : a->y1[i] * a->gm1[i];
+    }
+    conv3x3(Nt, Nx, W + cnn_woff[1], 16, 16, a->u1, a->y2);
+    conv3x3(Nt, Nx, W + cnn_woff[4], 1, 16, a->q, a->gm2);
+    for (size_t i = 0; i < 16 * V; ++i) {
+        a->gm2[i] = 1.0 + tanh(a->gm2[i]);
+        a->u2[i] = bf16 ? orc_bf16_round(a->y2[i] * a->gm2[i]) : a->y2[i] * a->gm2[i];
+    }
+    conv3x3(Nt, Nx, W + cnn_woff[2], 16, 8, a->u2, a->y3);
     return s;
 }
 
-void orc_cnn_forward(int Nt, int Nx, const float *W, int bf16, double c0, const double *r, double *z) {
+void orc_cnn_forward(int Nt, int Nx, const float *W, int bf16, double c0, double g, const double *phi,
+                     const double *r, double *z) {
     const size_t V = (size_t)Nt * Nx;
-    double *u0 = malloc(sizeof(double) * 8 * V), *y1 = malloc(sizeof(double) * 16 * V);
-    double *u1 = malloc(sizeof(double) * 16 * V), *y2 = malloc(sizeof(double) * 16 * V);
-    double *u2 = malloc(sizeof(double) * 16 * V), *y3 = malloc(sizeof(double) * 8 * V);
-    double s = cnn_fwd(Nt, Nx, W, bf16, r, u0, y1, u1, y2, u2, y3);
-    for (size_t i = 0; i < 8 * V; ++i) z[i] = c0 * r[i] + s * y3[i];
-    free(u0); free(y1); free(u1); free(y2); free(u2); free(y3);
+    cnn_acts a = cnn_acts_alloc(V);
+    double s = cnn_fwd(Nt, Nx, W, bf16, g, phi, r, &a);
+    for (size_t i = 0; i < 8 * V; ++i) z[i] = c0 * r[i] + s * a.y3[i];
+    cnn_acts_free(&a);
+}
+
+/* gate backward: from dL/du (gu) of u = y . gm with gm = 1 + tanh(t), dL/dy (into gu)
+ * and dL/dt (gt) */
+static void gate_bwd(size_t n, const double *y, const double *gm, double *gu, double *gt) {
+    for (size_t i = 0; i < n; ++i) {
+        const double th = gm[i] - 1.0;
+        gt[i] = gu[i] * y[i] * (1.0 - th * th);
+        gu[i] *= gm[i];
+    }
 }
 
 double orc_cnn_loss_grad(const orc_params *p, const double *phi, const float *W, double c0, const double *r,
                          const double *e, double *grad) {
     const int Nt = p->Nt, Nx = p->Nx;
     const size_t V = (size_t)Nt * Nx;
-    double *u0 = malloc(sizeof(double) * 8 * V), *y1 = malloc(sizeof(double) * 16 * V);
-    double *u1 = malloc(sizeof(double) * 16 * V), *y2 = malloc(sizeof(double) * 16 * V);
-    double *u2 = malloc(sizeof(double) * 16 * V), *y3 = malloc(sizeof(double) * 8 * V);
-    double *z = malloc(sizeof(double) * 8 * V), *Az = malloc(sizeof(double) * 8 * V);
-    double *g3 = malloc(sizeof(double) * 8 * V), *g2 = malloc(sizeof(double) * 16 * V);
-    double *g1 = malloc(sizeof(double) * 16 * V);
-    double s = cnn_fwd(Nt, Nx, W, 0, r, u0, y1, u1, y2, u2, y3);
-    /* d = e - z; L = <d, A d> / <e, r>; dL/dz = -2 A d / <e, r>; dL/dy3 = s dL/dz */
-    for (size_t i = 0; i < 8 * V; ++i) z[i] = e[i] - (c0 * r[i] + s * y3[i]);
-    orc_normal_op(p, phi, z, Az, 1);
-    double er = 0.0, L = 0.0;
-    for (size_t i = 0; i < 8 * V; ++i) { er += e[i] * r[i]; L += z[i] * Az[i]; }
-    L /= er;
+    cnn_acts a = cnn_acts_alloc(V);
+    double *d = malloc(sizeof(double) * 8 * V), *Ad = malloc(sizeof(double) * 8 * V);
+    double *AAd = malloc(sizeof(double) * 8 * V), *g3 = malloc(sizeof(double) * 8 * V);
+    double *g2 = malloc(sizeof(double) * 16 * V), *g1 = malloc(sizeof(double) * 16 * V);
+    double *gt = malloc(sizeof(double) * 16 * V);
+    double s = cnn_fwd(Nt, Nx, W, 0, p->g, phi, r, &a);
+    /* d = e - z; L = |A d|^2 / |r|^2; dL/dz = -2 A (A d) / |r|^2; dL/dy3 = s dL/dz */
+    for (size_t i = 0; i < 8 * V; ++i) d[i] = e[i] - (c0 * r[i] + s * a.y3[i]);
+    orc_normal_op(p, phi, d, Ad, 1);
+    orc_normal_op(p, phi, Ad, AAd, 1);
+    double rr = 0.0, L = 0.0;
+    for (size_t i = 0; i < 8 * V; ++i) { rr += r[i] * r[i]; L += Ad[i] * Ad[i]; }
+    L /= rr;
     double Wd[ORC_CNN_NW];
     for (int i = 0; i < ORC_CNN_NW; ++i) { Wd[i] = (double)W[i]; grad[i] = 0.0; }
-    for (size_t i = 0; i < 8 * V; ++i) g3[i] = -2.0 * s * Az[i] / er;
-    conv3x3_wgrad(Nt, Nx, 16, 8, u2, g3, grad + cnn_woff[2]);
-    conv3x3_t(Nt, Nx, Wd + cnn_woff[2], 16, 8, g3, g2);
-    for (size_t i = 0; i < 16 * V; ++i) g2[i] *= 1.0 - u2[i] * u2[i];
-    conv3x3_wgrad(Nt, Nx, 16, 16, u1, g2, grad + cnn_woff[1]);
-    conv3x3_t(Nt, Nx, Wd + cnn_woff[1], 16, 16, g2, g1);
-    for (size_t i = 0; i < 16 * V; ++i) g1[i] *= 1.0 - u1[i] * u1[i];
-    conv3x3_wgrad(Nt, Nx, 8, 16, u0, g1, grad + cnn_woff[0]);
-    free(u0); free(y1); free(u1); free(y2); free(u2); free(y3); free(z); free(Az); free(g3); free(g2); free(g1);
+    for (size_t i = 0; i < 8 * V; ++i) g3[i] = -2.0 * s * AAd[i] / rr;
+    conv3x3_wgrad(Nt, Nx, 16, 8, a.u2, g3, grad + cnn_woff[2]);
+    conv3x3_t(Nt, Nx, Wd + cnn_woff[2], 16, 8, g3, g2);     /* dL/du2 */
+    gate_bwd(16 * V, a.y2, a.gm2, g2, gt);                     /* dL/dy2, dL/dt2 */
+    conv3x3_wgrad(Nt, Nx, 1, 16, a.q, gt, grad + cnn_woff[4]);
+    conv3x3_wgrad(Nt, Nx, 16, 16, a.u1, g2, grad + cnn_woff[1]);
+    conv3x3_t(Nt, Nx, Wd + cnn_woff[1], 16, 16, g2, g1);    /* dL/du1 */
+    gate_bwd(16 * V, a.y1, a.gm1, g1, gt);                     /* dL/dy1, dL/dt1 */
+    conv3x3_wgrad(Nt, Nx, 1, 16, a.q, gt, grad + cnn_woff[3]);
+    conv3x3_wgrad(Nt, Nx, 8, 16, a.u0, g1, grad + cnn_woff[0]);
+    cnn_acts_free(&a);
+    free(d); free(Ad); free(AAd); free(g3); free(g2); free(g1); free(gt);
     return L;
 }

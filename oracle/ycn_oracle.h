@@ -120,20 +120,26 @@ int orc_eo_cg(const orc_params *p, const double *phi, const double *eta, double 
               orc_cg_stats *st);
 
 /* ---- learned preconditioner (no reference counterpart) ------------------------
- * 3 layers of 3x3 periodic convolutions, 8 -> 16 -> 16 -> 8 channels, no bias, tanh
- * between layers.  Channel 2c+0 / 2c+1 = Re / Im of spinor component c.  The network
- * is a learned correction to an exact scaled identity: z = c0 r + s net(r / s), with
- * s = RMS of the 8V real entries of r.
- * Weights: W1[9][8][16], W2[9][16][16], W3[9][16][8] ([tap][in][out], tap = (dt+1)*3 +
- * (dx+1), output(site) = sum W * input(site + (dt, dx))), flattened in that order.
- * bf16 != 0 rounds weights and activations to bfloat16 (round to nearest even) as
- * the GPU kernels do; accumulation is FP64 here. */
-#define ORC_CNN_NW (9 * 8 * 16 + 9 * 16 * 16 + 9 * 16 * 8)
-void orc_cnn_forward(int Nt, int Nx, const float *W, int bf16, double c0, const double *r, double *z);
-/* energy-norm loss of the correction z = CNN(r) for a solver pair (r, e):
- * L = <e - z, A (e - z)> / <e, r>  (A = D^dag D, real inner products over the 8V
- * entries; with r = A e this is the A-norm error relative to |e|_A^2), and dL/dW by
- * exact backprop in FP64 without bf16 rounding. */
+ * 3 layers of 3x3 periodic convolutions, 8 -> 16 -> 16 -> 8 channels, no bias.  The
+ * data path is linear in r; each hidden activation is gated by the field:
+ *   u1 = (W1 * u0) . (1 + tanh(G1 * q)),  u2 = (W2 * u1) . (1 + tanh(G2 * q)),
+ *   y = W3 * u2,  q = g phi (the Yukawa part of the Wilson diagonal 2r + M + g phi),
+ * so the preconditioner is a linear map of r whose stencil depends on the local field.
+ * Channel 2c+0 / 2c+1 = Re / Im of spinor component c.  The network is a learned
+ * correction to an exact scaled identity: z = c0 r + s y(r / s), with s = RMS of the
+ * 8V real entries of r.
+ * Weights: W1[9][8][16], W2[9][16][16], W3[9][16][8], G1[9][16], G2[9][16] ([tap][in]
+ * [out], the gates' single input being q; tap = (dt+1)*3 + (dx+1), output(site) =
+ * sum W * input(site + (dt, dx))), flattened in that order.
+ * bf16 != 0 rounds the convolution weights and the activations u0, u1, u2 to bfloat16
+ * (round to nearest even) as the GPU kernels do; accumulation is FP64 here. */
+#define ORC_CNN_NW (9 * 8 * 16 + 9 * 16 * 16 + 9 * 16 * 8 + 2 * 9 * 16)
+void orc_cnn_forward(int Nt, int Nx, const float *W, int bf16, double c0, double g, const double *phi,
+                     const double *r, double *z);
+/* residual-space loss of the correction z = CNN(r) for a solver pair (r, e):
+ * L = |A (e - z)|^2 / |r|^2  (A = D^dag D; with r = A e this is |r - A z|^2 / |r|^2, the
+ * residual left by the preconditioned correction relative to the residual), and dL/dW
+ * by exact backprop in FP64 without bf16 rounding. */
 double orc_cnn_loss_grad(const orc_params *p, const double *phi, const float *W, double c0, const double *r,
                          const double *e, double *grad);
 double orc_bf16_round(double v);

@@ -7,7 +7,8 @@ accumulates in FP32; the oracle emulates the bf16 rounding and accumulates in FP
 an activation can land on the other side of a bf16 rounding boundary.  Forward outputs
 agree to 1e-2 of the largest output (relative L2 error < 3e-3); the gradient, whose
 oracle has no bf16 rounding at all, agrees in direction (cosine > 0.99) and norm (10%).
-Solves reach the same FP64 true residual as plain CG (<= 1e-10)."""
+The gates run in FP32 on the GPU and in FP64 in the oracle.  Solves reach the same FP64
+true residual as plain CG (<= 1e-10)."""
 import numpy as np
 import pytest
 
@@ -37,17 +38,22 @@ def test_cnn_forward_vs_oracle(gpu, Nt, Nx):
     rng = np.random.default_rng(Nt + Nx)
     W = rand_weights(rng)
     r = rand_spinor(rng, Nt * Nx, 1e-3)
+    phi = O.hot_start(3, Nt, Nx)
     with ctx(Y, Nt, Nx) as c:
+        c.set_field(phi)
         c.precond_set_weights(W)
         assert np.array_equal(c.precond_get_weights(), W)
         z = c.precond_apply(r)
         z2 = c.precond_apply(7.5 * r)
-    ref = O.cnn_forward(Nt, Nx, W, r, bf16=True, c0=O.cnn_c0(1.0, 0.05))
+    ref = O.cnn_forward(Nt, Nx, W, r, bf16=True, c0=O.cnn_c0(1.0, 0.05), phi=phi, g=0.5)
     err = np.abs(z - ref).max() / np.abs(ref).max()
     l2 = np.linalg.norm(z - ref) / np.linalg.norm(ref)
     assert err < 1e-2 and l2 < 3e-3, (err, l2)
-    # scale equivariance z(a r) = a z(r)
+    # linear in r (the gates depend on the field only): z(a r) = a z(r)
     assert np.linalg.norm(z2 - 7.5 * z) / np.linalg.norm(7.5 * z) < 1e-3
+    # the gates are live: the same weights without the field give another output
+    ref0 = O.cnn_forward(Nt, Nx, W, r, bf16=True, c0=O.cnn_c0(1.0, 0.05))
+    assert np.linalg.norm(z - ref0) > 10 * np.linalg.norm(z - ref)
 
 
 def test_cnn_gradient_vs_oracle(gpu):

@@ -1,0 +1,74 @@
+"""The learned-preconditioner workload at its full size: 1024^2 near the chiral point
+(M = 0.05, g = 0.5, m^2 = -0.4, lambda = 1, s = 0.5, eps = 0.005), hot start.
+
+For each of four hot-start seeds the network is trained online on about 2000
+(residual, error) pairs: 125 thermalisation steps whose fermion solves are the
+reference's plain FP64 CG (CGsolver.cu:72-170), 16 pairs per solve.  From the field those
+steps leave, 20 Langevin steps are then run twice on the same noise: with flexible CG and
+the trained network (FP32 inner iterations, FP64 reliable updates, plain CG on stall), and
+with plain FP64 CG.  Every solve must reach the FP64 true residual 1e-10, the two
+trajectories must agree to 1e-9, and the learned solves must take at most the stated
+fraction of the plain solves' iterations (measured on one MI355X: 152.0 / 302.4 = 0.503,
+148.7 / 293.0 = 0.508, 148.5 / 303.4 = 0.489 and 146.4 / 294.1 = 0.498 for seeds 7-10;
+the bound 0.56 leaves room for rounding differences in the training between boxes)."""
+import time
+
+import numpy as np
+import pytest
+
+import oracle as O
+from test_gpu_parity import mk
+
+pytestmark = pytest.mark.gpu
+
+C5 = dict(Nt=1024, Nx=1024, M=0.05, g=0.5, m2=-0.4, lam=1.0, cutFraction=0.5, epsBar=0.005, adaptive=0, seed=1234,
+          cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
+TRAIN_STEPS, PAIRS_PER_SOLVE, LR = 125, 16, 1e-3
+MEASURE = 20
+# learned / plain FP64 iterations per solve, upper bound per hot-start seed
+RATIO_BOUND = {7: 0.56, 8: 0.56, 9: 0.56, 10: 0.56}
+
+
+def l2rel(a, b):
+    return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
+
+
+def _run(c, step0):
+    c.sync()
+    t0 = time.perf_counter()
+    st = c.langevin_run(step0, MEASURE)
+    c.sync()
+    return st, MEASURE / (time.perf_counter() - t0)
+
+
+@pytest.mark.timeout(900)
+@pytest.mark.parametrize("seed", sorted(RATIO_BOUND))
+def test_1024_learned_vs_plain_fp64(gpu, seed):
+    Y = gpu
+    print(f"# 1024^2 learned preconditioner, hot-start seed {seed}", flush=True)
+    p, _ = mk(Y, **C5)
+    p.cgPrecision = Y.YCN_CG_MIXED
+    p.precond = Y.YCN_PRECOND_CNN
+    with Y.Context(p) as c:
+        c.hot_start(seed)
+        info = c.precond_train(0, TRAIN_STEPS, pairs_per_solve=PAIRS_PER_SOLVE, lr=LR)
+        print(f"#   trained on {info['pairs']} pairs, loss {info['loss_first']:.3f} -> {info['loss_last']:.3f}",
+              flush=True)
+        assert info["pairs"] >= 1900
+        phi0 = c.get_field()
+        stL, rateL = _run(c, TRAIN_STEPS)
+        phiL = c.get_field()
+    p, _ = mk(Y, **C5)
+    with Y.Context(p) as c:
+        c.set_field(phi0)
+        stP, rateP = _run(c, TRAIN_STEPS)
+        phiP = c.get_field()
+    ratio = stL["iters_mean"] / stP["iters_mean"]
+    e = l2rel(phiL, phiP)
+    print(f"#   learned {stL['iters_mean']:.1f} vs plain FP64 {stP['iters_mean']:.1f} iterations per solve "
+          f"(ratio {ratio:.3f}); {rateL:.1f} vs {rateP:.1f} steps/s; true residual {stL['true_rel_resid']:.2e} / "
+          f"{stP['true_rel_resid']:.2e}; fields {e:.1e} apart", flush=True)
+    for st in (stL, stP):
+        assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+    assert e <= 1e-9
+    assert ratio <= RATIO_BOUND[seed], ratio

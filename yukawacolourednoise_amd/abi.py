@@ -42,7 +42,7 @@ EXPORTED = [
     "ycn_precond_set_weights", "ycn_precond_get_weights", "ycn_precond_apply", "ycn_precond_loss_grad",
     "ycn_precond_train",
 ]
-YCN_CNN_NWEIGHTS = 4608
+YCN_CNN_NWEIGHTS = 4896
 
 
 class Params(C.Structure):

@@ -1,23 +1,30 @@
 // cnn.hip -- the learned preconditioner on gfx950: a 3-layer periodic 3x3 CNN
-// (8 -> 16 -> 16 -> 8 channels, no bias, tanh between layers) with BF16 weights and
-// activations and FP32 accumulation on MFMA, trained online with Adam on (residual,
-// error) pairs the CG solver produces.  No reference counterpart.
+// (8 -> 16 -> 16 -> 8 channels, no bias) with BF16 weights and activations and FP32
+// accumulation on MFMA, trained online with Adam on (residual, error) pairs the CG
+// solver produces.  No reference counterpart.
 //
-// Channel 2c / 2c+1 = Re / Im of spinor component c.  The network is a learned
-// correction to an exact FP32 scaled identity: z = c0 r + s * net(r / s), with
-// c0 = 1 / (2r + M)^2 and s = RMS of the 8V real entries of r, so the preconditioner
-// is scale-equivariant, a zero network is plain CG, and the bf16 rounding only touches
-// the learned part.
+// The data path is linear in r; each hidden activation is gated by the field:
+//   u1 = (W1 * u0) . (1 + tanh(G1 * q)),  u2 = (W2 * u1) . (1 + tanh(G2 * q)),
+//   y = W3 * u2,  q = g phi,
+// so the preconditioner is a linear map of r (flexible CG stays well behaved) whose
+// stencil follows the local Wilson diagonal 2r + M + g phi.  Channel 2c / 2c+1 = Re / Im
+// of spinor component c.  The network is a learned correction to an exact FP32 scaled
+// identity: z = c0 r + s * y(r / s), with c0 = 1 / (2r + M)^2 and s = RMS of the 8V real
+// entries of r, so a zero network is plain CG and the bf16 rounding only touches the
+// learned part.
 // Weight layout (FP32 master copy): W1[9][8][16], W2[9][16][16], W3[9][16][8],
-// [tap][in][out], tap = (dt+1)*3 + (dx+1), out(site) = sum W * in(site + (dt, dx)).
+// G1[9][16], G2[9][16], [tap][in][out], tap = (dt+1)*3 + (dx+1),
+// out(site) = sum W * in(site + (dt, dx)).  The gates run in FP32 on the VALU.
 //
 // Inference (inside flexible CG): k_cnn_fused keeps all three layers of a 16x16 tile
 // (plus its 3-site halo) in LDS as BF16 and fuses the two dot products the flexible
-// CG recurrence needs.  Compulsory HBM traffic per site: read r_s, Ap (32 + 32 B),
-// write z (32 B).
+// CG recurrence needs.  Compulsory HBM traffic per site: read r_s, Ap (32 + 32 B) and
+// the FP32 field (4 B), write z (32 B).
 // Training: layer-wise MFMA kernels over channels-last FP32 activations in HBM,
 // forward, data gradient (a forward convolution with mirrored, transposed weights),
 // weight gradient (an MFMA GEMM whose reduction runs over lattice sites), then Adam.
+// The loss is the residual the correction leaves, |A (e - z)|^2 / |r|^2 (= |r - A z|^2 /
+// |r|^2 for r = A e): it weights every mode the way the next CG residual does.
 #include <algorithm>
 #include <cmath>
 #include <vector>
@@ -76,12 +83,15 @@ __global__ void k_make_frags(const float *__restrict__ W, bf16x8 *__restrict__ f
 // ---------------------------------------------------------------------------
 // fused inference: z = s * net(r_s / s) on a 16x16 tile, plus partial <z, r_s>, <z, Ap>
 // ---------------------------------------------------------------------------
-template <int CIN, int COUT, int WOUT, int WIN, int KS, bool TANH, typename Store>
+template <int CIN, int COUT, int WOUT, int WIN, int KS, typename Store>
 __device__ __forceinline__ void conv_lds(const __bf16 *__restrict__ in, const bf16x8 *__restrict__ frag, Store store) {
     constexpr int N = WOUT * WOUT;
     constexpr int G = (N + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int kg = lane >> 4, col = lane & 15;
+    bf16x8 B[KS];   // this lane's weight fragments, held for all of the wave's site groups
+#pragma unroll
+    for (int ks = 0; ks < KS; ++ks) B[ks] = frag[ks * 64 + lane];
     for (int g = wave; g < G; g += kCT / 64) {
         const int m = g * 16 + col;
         const bool valid = m < N;
@@ -97,25 +107,57 @@ __device__ __forceinline__ void conv_lds(const __bf16 *__restrict__ in, const bf
                 const int site = (a + 1 + dt) * WIN + (b + 1 + dx);
                 A = *reinterpret_cast<const bf16x8 *>(in + site * CIN + ci0);
             }
-            acc = mfma_bf16(A, frag[ks * 64 + lane], acc);
+            acc = mfma_bf16(A, B[ks], acc);
         }
 #pragma unroll
         for (int i = 0; i < 4; ++i) {
             const int mm = g * 16 + kg * 4 + i;
-            if (mm < N && col < COUT) store(mm, col, TANH ? fast_tanh(acc[i]) : acc[i]);
+            if (mm < N && col < COUT) store(mm, col, acc[i]);
         }
     }
 }
 
+// gates of one hidden layer on a WxW array of the tile: GT[m][n] = 1 + tanh(sum_tap
+// G[tap][n] q(site + off(tap))) for array site m = (a, b), which sits at (a + o, b + o)
+// of the q array Q (width WQ).  One thread per site: its nine q taps in registers, the
+// weights G[tap][16] in LDS, read as broadcast float4s.
+template <int W, int WQ, int o>
+__device__ __forceinline__ void gate_pass(const float *Q, const float4 *G, _Float16 *GT) {
+    for (int m = threadIdx.x; m < W * W; m += kCT) {
+        const int a = m / W, b = m - a * W;
+        float qn[9];
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) qn[tap] = Q[(a + o + tap / 3 - 1) * WQ + (b + o + tap % 3 - 1)];
+        float t[16];
+#pragma unroll
+        for (int n = 0; n < 16; ++n) t[n] = 0.f;
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap)
+#pragma unroll
+            for (int n4 = 0; n4 < 4; ++n4) {
+                const float4 w = G[tap * 4 + n4];
+                t[4 * n4] += w.x * qn[tap];
+                t[4 * n4 + 1] += w.y * qn[tap];
+                t[4 * n4 + 2] += w.z * qn[tap];
+                t[4 * n4 + 3] += w.w * qn[tap];
+            }
+#pragma unroll
+        for (int n = 0; n < 16; ++n) GT[m * 16 + n] = (_Float16)gate_fn(t[n]);
+    }
+}
+
 __global__ void __launch_bounds__(kCT)
-k_cnn_fused(int Nt, int Nx, int tilesX, float c0, const CgState *__restrict__ st, const float2 *__restrict__ rs,
-            const float2 *__restrict__ Ap, float2 *__restrict__ z, const bf16x8 *__restrict__ gf1,
+k_cnn_fused(int Nt, int Nx, int tilesX, int nNoise, float c0, float g, const CgState *__restrict__ st,
+            const float2 *__restrict__ rs, const float2 *__restrict__ Ap, const float *__restrict__ phis,
+            const float *__restrict__ gw, float2 *__restrict__ z, const bf16x8 *__restrict__ gf1,
             const bf16x8 *__restrict__ gf2, const bf16x8 *__restrict__ gf3, double2 *__restrict__ part) {
     constexpr int W0 = 22, W1 = 20, W2 = 18, W3 = 16;
     __shared__ __attribute__((aligned(16))) __bf16 U0[W0 * W0 * 8];
     __shared__ __attribute__((aligned(16))) __bf16 U1[W1 * W1 * 16];
     __shared__ __attribute__((aligned(16))) __bf16 U2[W2 * W2 * 16];
-    __shared__ bf16x8 F1[kFrag1], F2[kFrag2], F3[kFrag3];
+    __shared__ float Q[W0 * W0];            // q = g phi on the tile and its 3-site halo
+    __shared__ _Float16 GT[W1 * W1 * 16];   // gates of layer 1, then of layer 2
+    __shared__ float4 GW[2 * kCnnNWG / 4];  // gate weights G1, G2
     __shared__ double2 red[kCT / kWave];
     const int sys = blockIdx.y;
     if (st[sys].done) return;
@@ -139,6 +181,7 @@ k_cnn_fused(int Nt, int Nx, int tilesX, float c0, const CgState *__restrict__ st
             acc.y += (double)(rv.x * av.x + rv.y * av.y);
         }
     } else {
+        const float *ph = phis + (size_t)(sys / nNoise) * V;
         for (int w = threadIdx.x; w < W0 * W0; w += kCT) {
             const int a = w / W0, b = w - a * W0;
             const size_t gs = (size_t)wrapn(t0 - 3 + a, Nt) * Nx + wrapn(x0 - 3 + b, Nx);
@@ -148,16 +191,25 @@ k_cnn_fused(int Nt, int Nx, int tilesX, float c0, const CgState *__restrict__ st
                 U0[w * 8 + 2 * c] = (__bf16)(v.x * inv);
                 U0[w * 8 + 2 * c + 1] = (__bf16)(v.y * inv);
             }
+            Q[w] = g * ph[gs];
         }
-        for (int i = threadIdx.x; i < kFrag1; i += kCT) F1[i] = gf1[i];
-        for (int i = threadIdx.x; i < kFrag2; i += kCT) F2[i] = gf2[i];
-        for (int i = threadIdx.x; i < kFrag3; i += kCT) F3[i] = gf3[i];
+        for (int i = threadIdx.x; i < 2 * kCnnNWG; i += kCT) reinterpret_cast<float *>(GW)[i] = gw[i];
         __syncthreads();
-        conv_lds<8, 16, W1, W0, ksteps(8), true>(U0, F1, [&](int m, int n, float v) { U1[m * 16 + n] = (__bf16)v; });
+        // layer-1 site (a, b) of the 20x20 array is (a + 1, b + 1) of the q array
+        gate_pass<W1, W0, 1>(Q, GW, GT);
         __syncthreads();
-        conv_lds<16, 16, W2, W1, ksteps(16), true>(U1, F2, [&](int m, int n, float v) { U2[m * 16 + n] = (__bf16)v; });
+        conv_lds<8, 16, W1, W0, ksteps(8)>(U0, gf1, [&](int m, int n, float v) {
+            U1[m * 16 + n] = (__bf16)(v * (float)GT[m * 16 + n]);
+        });
         __syncthreads();
-        conv_lds<16, 8, W3, W2, ksteps(16), false>(U2, F3, [&](int m, int n, float v) {
+        // layer-2 site (a, b) of the 18x18 array is (a + 2, b + 2) of the q array
+        gate_pass<W2, W0, 2>(Q, GW + kCnnNWG / 4, GT);
+        __syncthreads();
+        conv_lds<16, 16, W2, W1, ksteps(16)>(U1, gf2, [&](int m, int n, float v) {
+            U2[m * 16 + n] = (__bf16)(v * (float)GT[m * 16 + n]);
+        });
+        __syncthreads();
+        conv_lds<16, 8, W3, W2, ksteps(16)>(U2, gf3, [&](int m, int n, float v) {
             const int a = m / W3, b = m - a * W3;
             const size_t gs = (size_t)(t0 + a) * Nx + x0 + b;
             const size_t idx = off + (size_t)(n >> 1) * V + gs;
@@ -176,8 +228,8 @@ k_cnn_fused(int Nt, int Nx, int tilesX, float c0, const CgState *__restrict__ st
 // ---------------------------------------------------------------------------
 // training kernels (layer-wise, channels-last FP32 activations [site][C])
 // ---------------------------------------------------------------------------
-// out[site][co] = act(sum_k in[site + off(tap)][ci] W[k][co]); one wave = 16 sites
-template <int CIN, int COUT, bool TANH>
+// out[site][co] = sum_k in[site + off(tap)][ci] W[k][co]; one wave = 16 sites
+template <int CIN, int COUT>
 __global__ void __launch_bounds__(kCT)
 k_conv_fwd(int Nt, int Nx, const float *__restrict__ in, const bf16x8 *__restrict__ frag, float *__restrict__ out) {
     constexpr int KS = ksteps(CIN);
@@ -207,7 +259,7 @@ k_conv_fwd(int Nt, int Nx, const float *__restrict__ in, const bf16x8 *__restric
 #pragma unroll
     for (int i = 0; i < 4; ++i) {
         const int so = g * 16 + kg * 4 + i;
-        if (col < COUT) out[(size_t)so * COUT + col] = TANH ? fast_tanh(acc[i]) : acc[i];
+        if (col < COUT) out[(size_t)so * COUT + col] = acc[i];
     }
 }
 
@@ -303,10 +355,55 @@ k_cnn_g3(const float2 *__restrict__ Ad, int V, float fac, float *__restrict__ g3
     }
 }
 
-// g *= 1 - u^2 (tanh derivative)
-__global__ void k_tanh_grad(float *__restrict__ g, const float *__restrict__ u, int64_t n) {
+// q[site] = g phi[site] (the gates' input, channels-last with one channel)
+__global__ void __launch_bounds__(kCT) k_gate_input(const float *__restrict__ phis, int V, float g, float *__restrict__ q) {
+    const int i = blockIdx.x * kCT + threadIdx.x;
+    if (i < V) q[i] = g * phis[i];
+}
+
+// gm[site][n] = 1 + tanh(sum_tap G[tap][n] q[site + off(tap)]), u = y . gm (16 channels)
+__global__ void __launch_bounds__(kCT)
+k_gate_fwd(int Nt, int Nx, const float *__restrict__ y, const float *__restrict__ q, const float *__restrict__ G,
+           float *__restrict__ gm, float *__restrict__ u) {
+    const int s = blockIdx.x * kCT + threadIdx.x;
+    if (s >= Nt * Nx) return;
+    const int t = s / Nx, x = s - t * Nx;
+    float qn[9];
+#pragma unroll
+    for (int tap = 0; tap < 9; ++tap) qn[tap] = q[wrapn(t + tap / 3 - 1, Nt) * Nx + wrapn(x + tap % 3 - 1, Nx)];
+#pragma unroll
+    for (int n = 0; n < 16; ++n) {
+        float a = 0.f;
+#pragma unroll
+        for (int tap = 0; tap < 9; ++tap) a += G[tap * 16 + n] * qn[tap];
+        const float gv = gate_fn(a);
+        gm[(size_t)s * 16 + n] = gv;
+        u[(size_t)s * 16 + n] = y[(size_t)s * 16 + n] * gv;
+    }
+}
+
+// backward through u = y . gm, gm = 1 + tanh(t): from dL/du (in g) to dL/dy (in g) and
+// dL/dt (gt)
+__global__ void k_gate_bwd(float *__restrict__ g, const float *__restrict__ y, const float *__restrict__ gm,
+                           float *__restrict__ gt, int64_t n) {
     const int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (i < n) g[i] *= 1.f - u[i] * u[i];
+    if (i >= n) return;
+    const float th = gm[i] - 1.f;
+    gt[i] = g[i] * y[i] * (1.f - th * th);
+    g[i] *= gm[i];
+}
+
+// <a, b> partials of FP32 SoA spinors, accumulated in FP64
+__global__ void __launch_bounds__(kCT)
+k_dot_f2(const float2 *__restrict__ a, const float2 *__restrict__ b, int n, double2 *__restrict__ part) {
+    __shared__ double2 red[kCT / kWave];
+    double2 acc = make_double2(0.0, 0.0);
+    for (int i = blockIdx.x * kCT + threadIdx.x; i < n; i += gridDim.x * kCT) {
+        const float2 u = a[i], v = b[i];
+        acc.x += (double)u.x * v.x + (double)u.y * v.y;
+    }
+    acc = block_sum2<kCT>(acc, red);
+    if (threadIdx.x == 0) part[blockIdx.x] = acc;
 }
 
 // <a, b> and <a, a> partials of FP64 SoA spinors (for s and <e, r>)
@@ -344,8 +441,8 @@ static bool cnn_fits(const Ctx &c) { return c.p.Nt % 16 == 0 && c.p.Nx % 16 == 0
 // Skip-path weight c0 = 1 / (2r + M)^2 (the scale of the diagonal of D^dag D).
 double cnn_c0(const Ctx &c) { return 1.0 / ((2.0 * c.p.r + c.p.M) * (2.0 * c.p.r + c.p.M)); }
 
-// Default weights: identity-like centre taps in the hidden layers (tanh in its linear
-// range) and a zero output layer, so the untrained preconditioner is z = c0 r (plain CG).
+// Default weights: identity-like centre taps in the hidden layers, zero gates (gates of
+// 1) and a zero output layer, so the untrained preconditioner is z = c0 r (plain CG).
 static std::vector<float> default_weights(const Ctx &) {
     std::vector<float> W(kCnnNW, 0.f);
     for (int ci = 0; ci < 8; ++ci) W[(4 * 8 + ci) * 16 + ci] = 0.1f;              // W1 centre
@@ -405,21 +502,23 @@ int cnn_get_weights(Ctx &c, float *w) {
 
 int cnn_apply(Ctx &c, int nsys, const CgState *st, const float2 *rs, const float2 *Ap, float2 *z, double2 *part) {
     const int tilesX = c.p.Nx / 16, tiles = (c.p.Nt / 16) * tilesX;
-    prof_before(c, PK_CNN, (double)nsys * c.V * 96.0);
-    hipLaunchKernelGGL(k_cnn_fused, dim3(tiles, nsys), dim3(kCT), 0, c.stream, c.p.Nt, c.p.Nx, tilesX,
-                       (float)cnn_c0(c), st, rs, Ap, z,
-                       (const bf16x8 *)c.cnnF1, (const bf16x8 *)c.cnnF2, (const bf16x8 *)c.cnnF3, part);
+    prof_before(c, PK_CNN, (double)nsys * c.V * 100.0);
+    hipLaunchKernelGGL(k_cnn_fused, dim3(tiles, nsys), dim3(kCT), 0, c.stream, c.p.Nt, c.p.Nx, tilesX, c.nNoise,
+                       (float)cnn_c0(c), (float)c.p.g, st, rs, Ap, (const float *)c.phis,
+                       (const float *)(c.cnnW + kCnnOffG1), z, (const bf16x8 *)c.cnnF1, (const bf16x8 *)c.cnnF2,
+                       (const bf16x8 *)c.cnnF3, part);
     prof_after(c, PK_CNN);
     YCN_HIP(hipGetLastError());
     return tiles;
 }
 
-// activations for training at lattice volume V: x0[8V] u1[16V] u2[16V] y3[8V] g3[8V]
-// ga[16V] gb[16V] wpart[blocks*144*16] red[256]
+// activations for training at lattice volume V: x0[8V] y1[16V] gm1[16V] u1[16V]
+// y2[16V] gm2[16V] u2[16V] y3[8V] g3[8V] ga[16V] gb[16V] gt[16V] q[V]
+// wpart[blocks*144*16] red[256]
 static int ensure_act(Ctx &c) {
     const size_t V = c.V;
     const size_t nwb = (V + kWgSites - 1) / kWgSites;
-    const size_t need = V * (8 + 16 + 16 + 8 + 8 + 16 + 16) + nwb * 144 * 16 + 512;
+    const size_t need = V * (8 + 16 * 6 + 8 + 8 + 16 * 3 + 1) + nwb * 144 * 16 + 512;
     if (c.cnnActV == need) return YCN_OK;
     if (c.cnnAct) (void)hipFree(c.cnnAct);
     YCN_HIP(hipMalloc(&c.cnnAct, sizeof(float) * need));
@@ -428,41 +527,48 @@ static int ensure_act(Ctx &c) {
 }
 
 // One training step on the pair (r, e) (FP64 SoA, system layout of one system) at the
-// field phis (FP32): forward, loss L = <d, A d> / <e, r> with d = e - z, backward,
-// Adam.  Uses FP32 normal operator scratch (rs/ps/Aps of system 0) and the CgState
-// slot cg_d[0].  Returns the loss (before the update) in *loss.
+// field phis (FP32): forward, loss L = |A d|^2 / |r|^2 with d = e - z (the residual the
+// correction leaves), backward, Adam.  Uses FP32 normal-operator scratch (rs, ps, Aps,
+// zs of system 0) and the CgState slot cg_d[0].  Returns the loss (before the update)
+// in *loss.
 int cnn_train_step(Ctx &c, const double2 *r, const double2 *e, const float *phis, float lr, double *loss,
                    bool update) {
     int rc = ensure_act(c);
     if (rc) return rc;
     const int Nt = c.p.Nt, Nx = c.p.Nx, V = c.V;
-    float *x0 = c.cnnAct, *u1 = x0 + (size_t)8 * V, *u2 = u1 + (size_t)16 * V, *y3 = u2 + (size_t)16 * V;
-    float *g3 = y3 + (size_t)8 * V, *ga = g3 + (size_t)8 * V, *gb = ga + (size_t)16 * V;
-    float *wpart = gb + (size_t)16 * V;
+    float *x0 = c.cnnAct, *y1 = x0 + (size_t)8 * V, *gm1 = y1 + (size_t)16 * V, *u1 = gm1 + (size_t)16 * V;
+    float *y2 = u1 + (size_t)16 * V, *gm2 = y2 + (size_t)16 * V, *u2 = gm2 + (size_t)16 * V;
+    float *y3 = u2 + (size_t)16 * V, *g3 = y3 + (size_t)8 * V, *ga = g3 + (size_t)8 * V, *gb = ga + (size_t)16 * V;
+    float *gt = gb + (size_t)16 * V, *q = gt + (size_t)16 * V;
+    float *wpart = q + (size_t)V;
     const int nwb = (V + kWgSites - 1) / kWgSites;
     double2 *red = reinterpret_cast<double2 *>(wpart + (size_t)nwb * 144 * 16);
     float *grad = c.cnnW + 3 * kCnnNW;
-    // s = RMS(r), <e, r>
+    const float *G1 = c.cnnW + kCnnOffG1, *G2 = c.cnnW + kCnnOffG2;
+    // s = RMS(r), |r|^2
     const int nred = 64;
-    hipLaunchKernelGGL(k_dot2, dim3(nred), dim3(kCT), 0, c.stream, e, r, 4 * V, red);
-    hipLaunchKernelGGL(k_dot2, dim3(nred), dim3(kCT), 0, c.stream, r, r, 4 * V, red + nred);
-    std::vector<double2> hr(2 * nred);
-    YCN_HIP(hipMemcpyAsync(hr.data(), red, sizeof(double2) * 2 * nred, hipMemcpyDeviceToHost, c.stream));
+    hipLaunchKernelGGL(k_dot2, dim3(nred), dim3(kCT), 0, c.stream, r, r, 4 * V, red);
+    std::vector<double2> hr(nred);
+    YCN_HIP(hipMemcpyAsync(hr.data(), red, sizeof(double2) * nred, hipMemcpyDeviceToHost, c.stream));
     YCN_HIP(hipStreamSynchronize(c.stream));
-    double er = 0, rr = 0;
-    for (int i = 0; i < nred; ++i) { er += hr[i].x; rr += hr[nred + i].y; }
-    if (!(rr > 0) || !(er > 0)) { *loss = 0; return YCN_OK; }
+    double rr = 0;
+    for (int i = 0; i < nred; ++i) rr += hr[i].y;
+    if (!(rr > 0)) { *loss = 0; return YCN_OK; }
     const double s = std::sqrt(rr / (8.0 * V));
     const dim3 gs((V + kCT - 1) / kCT), gc((V / 16 + 3) / 4);
+    const dim3 g16((16 * V + 255) / 256);
     // forward
     hipLaunchKernelGGL(k_cnn_prep, gs, dim3(kCT), 0, c.stream, r, V, 1.0 / s, x0);
-    hipLaunchKernelGGL((k_conv_fwd<8, 16, true>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)x0,
-                       (const bf16x8 *)c.cnnF1, u1);
-    hipLaunchKernelGGL((k_conv_fwd<16, 16, true>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u1,
-                       (const bf16x8 *)c.cnnF2, u2);
-    hipLaunchKernelGGL((k_conv_fwd<16, 8, false>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u2,
+    hipLaunchKernelGGL(k_gate_input, gs, dim3(kCT), 0, c.stream, phis, V, (float)c.p.g, q);
+    hipLaunchKernelGGL((k_conv_fwd<8, 16>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)x0,
+                       (const bf16x8 *)c.cnnF1, y1);
+    hipLaunchKernelGGL(k_gate_fwd, gs, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)y1, (const float *)q, G1, gm1, u1);
+    hipLaunchKernelGGL((k_conv_fwd<16, 16>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u1,
+                       (const bf16x8 *)c.cnnF2, y2);
+    hipLaunchKernelGGL(k_gate_fwd, gs, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)y2, (const float *)q, G2, gm2, u2);
+    hipLaunchKernelGGL((k_conv_fwd<16, 8>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u2,
                        (const bf16x8 *)c.cnnF3, y3);
-    // d = e - s y3; A d with the FP32 fused normal operator (beta = 0: p' = d)
+    // d = e - z; A d, then A (A d) with the FP32 fused normal operator (beta = 0: p' = input)
     hipLaunchKernelGGL(k_cnn_resid, gs, dim3(kCT), 0, c.stream, e, r, (const float *)y3, V, s, cnn_c0(c), c.rs);
     CgState S0{};
     S0.beta = 0.0;
@@ -470,27 +576,38 @@ int cnn_train_step(Ctx &c, const double2 *r, const double2 *e, const float *phis
     YCN_HIP(hipMemcpyAsync(c.cg_d, &S0, sizeof(CgState), hipMemcpyHostToDevice, c.stream));
     launch_fused<float>(c, c.tile32, 1, false, phis, c.cg_d, (const float2 *)c.rs, c.ps[0], c.ps[1], c.Aps,
                         c.partials);
-    const int nbF = fused_args(c, kTiles32[c.tile32].TT, kTiles32[c.tile32].TX).tilesPerSys;
-    std::vector<double2> hp(nbF);
-    YCN_HIP(hipMemcpyAsync(hp.data(), c.partials, sizeof(double2) * nbF, hipMemcpyDeviceToHost, c.stream));
-    // backward
-    hipLaunchKernelGGL(k_cnn_g3, gs, dim3(kCT), 0, c.stream, (const float2 *)c.Aps, V, (float)(-2.0 * s / er), g3);
+    launch_fused<float>(c, c.tile32, 1, false, phis, c.cg_d, (const float2 *)c.Aps, c.ps[0], c.ps[1], c.zs,
+                        c.partials);
+    hipLaunchKernelGGL(k_dot_f2, dim3(nred), dim3(kCT), 0, c.stream, (const float2 *)c.Aps, (const float2 *)c.Aps,
+                       4 * V, red);
+    std::vector<double2> hp(nred);
+    YCN_HIP(hipMemcpyAsync(hp.data(), red, sizeof(double2) * nred, hipMemcpyDeviceToHost, c.stream));
+    // backward: dL/dy3 = s dL/dz = -2 s A (A d) / |r|^2
+    hipLaunchKernelGGL(k_cnn_g3, gs, dim3(kCT), 0, c.stream, (const float2 *)c.zs, V, (float)(-2.0 * s / rr), g3);
     hipLaunchKernelGGL((k_conv_wgrad<16, 8>), dim3(nwb), dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u2,
                        (const float *)g3, wpart);
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((144 * 8 + 255) / 256), dim3(256), 0, c.stream, (const float *)wpart, nwb,
                        144, 8, grad + kCnnNW1 + kCnnNW2);
-    hipLaunchKernelGGL((k_conv_fwd<8, 16, false>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)g3,
-                       (const bf16x8 *)c.cnnD3, ga);
-    hipLaunchKernelGGL(k_tanh_grad, dim3((16 * V + 255) / 256), dim3(256), 0, c.stream, ga, (const float *)u2,
-                       (int64_t)16 * V);
+    hipLaunchKernelGGL((k_conv_fwd<8, 16>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)g3,
+                       (const bf16x8 *)c.cnnD3, ga);                                          // dL/du2
+    hipLaunchKernelGGL(k_gate_bwd, g16, dim3(256), 0, c.stream, ga, (const float *)y2, (const float *)gm2, gt,
+                       (int64_t)16 * V);                                                        // dL/dy2, dL/dt2
+    hipLaunchKernelGGL((k_conv_wgrad<1, 16>), dim3(nwb), dim3(kCT), 0, c.stream, Nt, Nx, (const float *)q,
+                       (const float *)gt, wpart);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((9 * 16 + 255) / 256), dim3(256), 0, c.stream, (const float *)wpart, nwb,
+                       9, 16, grad + kCnnOffG2);
     hipLaunchKernelGGL((k_conv_wgrad<16, 16>), dim3(nwb), dim3(kCT), 0, c.stream, Nt, Nx, (const float *)u1,
                        (const float *)ga, wpart);
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((144 * 16 + 255) / 256), dim3(256), 0, c.stream, (const float *)wpart, nwb,
                        144, 16, grad + kCnnNW1);
-    hipLaunchKernelGGL((k_conv_fwd<16, 16, false>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)ga,
-                       (const bf16x8 *)c.cnnD2, gb);
-    hipLaunchKernelGGL(k_tanh_grad, dim3((16 * V + 255) / 256), dim3(256), 0, c.stream, gb, (const float *)u1,
-                       (int64_t)16 * V);
+    hipLaunchKernelGGL((k_conv_fwd<16, 16>), gc, dim3(kCT), 0, c.stream, Nt, Nx, (const float *)ga,
+                       (const bf16x8 *)c.cnnD2, gb);                                          // dL/du1
+    hipLaunchKernelGGL(k_gate_bwd, g16, dim3(256), 0, c.stream, gb, (const float *)y1, (const float *)gm1, gt,
+                       (int64_t)16 * V);                                                        // dL/dy1, dL/dt1
+    hipLaunchKernelGGL((k_conv_wgrad<1, 16>), dim3(nwb), dim3(kCT), 0, c.stream, Nt, Nx, (const float *)q,
+                       (const float *)gt, wpart);
+    hipLaunchKernelGGL(k_wgrad_reduce, dim3((9 * 16 + 255) / 256), dim3(256), 0, c.stream, (const float *)wpart, nwb,
+                       9, 16, grad + kCnnOffG1);
     hipLaunchKernelGGL((k_conv_wgrad<8, 16>), dim3(nwb), dim3(kCT), 0, c.stream, Nt, Nx, (const float *)x0,
                        (const float *)gb, wpart);
     hipLaunchKernelGGL(k_wgrad_reduce, dim3((72 * 16 + 255) / 256), dim3(256), 0, c.stream, (const float *)wpart, nwb,
@@ -506,9 +623,9 @@ int cnn_train_step(Ctx &c, const double2 *r, const double2 *e, const float *phis
         if (rc) return rc;
     }
     YCN_HIP(hipStreamSynchronize(c.stream));
-    double dad = 0;
-    for (const double2 &v : hp) dad += v.x;
-    *loss = dad / er;
+    double add = 0;
+    for (const double2 &v : hp) add += v.x;
+    *loss = add / rr;
     return YCN_OK;
 }
 

@@ -21,7 +21,9 @@ typedef __bf16 bf16x8 __attribute__((ext_vector_type(8)));
 typedef float f32x4 __attribute__((ext_vector_type(4)));
 
 constexpr int kCnnNW1 = 9 * 8 * 16, kCnnNW2 = 9 * 16 * 16, kCnnNW3 = 9 * 16 * 8;
-constexpr int kCnnNW = kCnnNW1 + kCnnNW2 + kCnnNW3;   // 4608 weights
+constexpr int kCnnNWG = 9 * 16;                        // one gate layer: G[tap][16], input q = g phi
+constexpr int kCnnOffG1 = kCnnNW1 + kCnnNW2 + kCnnNW3, kCnnOffG2 = kCnnOffG1 + kCnnNWG;
+constexpr int kCnnNW = kCnnOffG2 + kCnnNWG;            // 4896 weights
 constexpr int ksteps(int cin) { return (9 * cin + 31) / 32; }
 // B-fragment sizes (bf16x8 per lane per K-step)
 constexpr int kFrag1 = ksteps(8) * 64, kFrag2 = ksteps(16) * 64, kFrag3 = ksteps(16) * 64;
@@ -37,6 +39,8 @@ __device__ __forceinline__ bf16x8 bf16x8_zero() {
     return v;
 }
 
-__device__ __forceinline__ float fast_tanh(float x) { return tanhf(x); }
+// gate value 1 + tanh(t) = 2 / (1 + e^{-2t}) in [0, 2] (hardware exp and reciprocal,
+// about 1 ulp each)
+__device__ __forceinline__ float gate_fn(float t) { return 2.f * __builtin_amdgcn_rcpf(1.f + __expf(-2.f * t)); }
 
 }  // namespace ycn
