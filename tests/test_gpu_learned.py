@@ -16,7 +16,6 @@ import time
 import numpy as np
 import pytest
 
-import oracle as O
 from test_gpu_parity import mk
 
 pytestmark = pytest.mark.gpu

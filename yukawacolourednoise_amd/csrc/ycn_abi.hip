@@ -245,8 +245,14 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         return bail(YCN_ERR_UNSUPPORTED);
     }
     // YCN_GRAPHS=0 launches CG chunks kernel by kernel instead of replaying captured
-    // graphs (for kernel-trace profilers); results are identical
-    if (const char *gr = std::getenv("YCN_GRAPHS")) c.useGraphs = std::atoi(gr) != 0;
+    // graphs; results are identical.  Without YCN_GRAPHS, chunks also launch eagerly when
+    // a profiler tool library is loaded (rocprofv3 announces it in ROCP_TOOL_LIBRARIES):
+    // on ROCm 7.2 a kernel trace of a process that captures these graphs ends in a
+    // host-side segmentation fault inside the runtime below the first capture.
+    if (const char *gr = std::getenv("YCN_GRAPHS"))
+        c.useGraphs = std::atoi(gr) != 0;
+    else if (std::getenv("ROCP_TOOL_LIBRARIES"))
+        c.useGraphs = false;
     if ((rc = cg_autotune(c))) return bail(rc);
     if (c.p.evenOdd) {
         if (c.p.precond == YCN_PRECOND_CNN) {
