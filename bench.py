@@ -21,7 +21,9 @@ launches of that kernel: one event pair brackets each chunk, divided by the iter
 done in it (launches after a pause count as time, not as launches).  Otherwise every
 launch of the dominant kernel is bracketed (launches that exit at once for converged
 systems excluded).
-`cpu_baseline` times the FP64 CPU oracle on a bounded sample of the same workload.
+`cpu_baseline` times full FP64 CPU-oracle steps of the same workload from the same start
+field on the host cores (about 20 s).  `--cpu-plan` runs the oracle alone on the lattice
+sizes of the workload table and exits.
 """
 from __future__ import annotations
 
@@ -50,66 +52,74 @@ WORKLOAD = dict(Nt=1024, Nx=1024, m2=-0.4, lam=1.0, g=0.5, M=0.05, r=1.0, cutFra
                 cgMaxIter=20000)
 
 
-def cpu_baseline(gpu_iters_per_solve: float, budget_s: float = 20.0):
-    """FP64 oracle on the host cores: a bounded sample of the 1024^2 step.
+def cpu_baseline(phi0, step0: int, budget_s: float = 20.0):
+    """FP64 oracle on the host cores: full Langevin steps of the 1024^2 workload.
 
-    The sample: one coloured-noise field, one bosonic update, and a fixed number of
-    CG iterations on D^dag D (each = two oracle Dirac applications + BLAS-1), timed
-    separately; a step is then priced at the GPU's measured iterations per solve
-    plus the true-residual check."""
+    The oracle (noise, plain FP64 CG on D^dag D to the same true residual 1e-10, drift,
+    update) steps from the GPU methods' shared start field phi0 at the same step counter,
+    so it solves the systems the timed GPU steps solve; steps run until `budget_s` is
+    used (at least one).  Nothing is priced or extrapolated."""
     sys.path.insert(0, os.path.join(ROOT, "oracle"))
     import numpy as np
     import oracle as O
 
     cores = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
     p = O.params(**WORKLOAD)
-    Nt, Nx = p.Nt, p.Nx
-    phi = O.hot_start(7, Nt, Nx)
-    t0 = time.perf_counter()
-    noise = O.coloured_noise(p, 0)
-    t_noise = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    O.bosonic_update(p, phi, np.zeros((Nt, Nx)), noise)
-    t_upd = time.perf_counter() - t0
-    # CG iterations: fixed-count solves until the budget is used
-    eta = O.fermion_noise(p.seed, 0, 0, Nt * Nx)
-    q = O.params(**{**WORKLOAD, "cgTol": 0.0, "cgTolMode": 0, "cgMaxIter": 4})
-    iters, t_cg = 0, 0.0
-    while t_cg < budget_s and iters < 400:
+    phi = np.ascontiguousarray(phi0, dtype=np.float64)
+    n, wall, iters = 0, 0.0, 0
+    while n == 0 or wall < budget_s:
         t0 = time.perf_counter()
-        _, st = O.cg(q, phi, eta, 1)
-        t_cg += time.perf_counter() - t0
+        phi, _, st = O.langevin_step(p, phi, step0 + n)
+        wall += time.perf_counter() - t0
         iters += st["iters"]
-    # each fixed-count solve also pays one true-residual check (~ one iteration)
-    solves = iters / 4
-    t_iter = t_cg / (iters + solves)
-    t_step = t_noise + t_upd + (gpu_iters_per_solve + 1.0) * t_iter
-    # the same sample on one thread (two 4-iteration solves)
-    O.set_threads(1)
-    t0 = time.perf_counter()
-    O.coloured_noise(p, 0)
-    O.bosonic_update(p, phi, np.zeros((Nt, Nx)), noise)
-    t_fixed1 = time.perf_counter() - t0
-    t0 = time.perf_counter()
-    for _ in range(2):
-        O.cg(q, phi, eta, 1)
-    t_iter1 = (time.perf_counter() - t0) / (8 + 2)
-    O.set_threads(cores)
-    t_step1 = t_fixed1 + (gpu_iters_per_solve + 1.0) * t_iter1
+        n += 1
     return {
-        "value": 1.0 / t_step,
+        "value": n / wall,
         "unit": "steps/s",
         "cores": cores,
         "kind": "port",
-        "sample": (f"oracle FP64 at 1024^2 on {cores} host threads: 1 coloured-noise field "
-                   f"({t_noise:.2f} s), 1 bosonic update ({t_upd:.3f} s), {iters} CG iterations "
-                   f"({t_iter * 1e3:.1f} ms/iter incl. its share of true-residual checks); step priced "
-                   f"at the GPU's {gpu_iters_per_solve:.0f} iterations/solve + 1"),
-        "single_thread_value": 1.0 / t_step1,
-        "single_thread_sample": (f"the same on 1 thread: noise + update {t_fixed1:.2f} s, 8 CG iterations "
-                                 f"({t_iter1 * 1e3:.0f} ms/iter)"),
+        "sample": (f"oracle FP64 (plain CG to true residual 1e-10) on {cores} host threads: {n} full {p.Nt}x{p.Nx} "
+                   f"Langevin steps from the GPU methods' start field, {wall:.1f} s, {iters / n:.0f} CG "
+                   f"iterations per solve"),
         "cpu": cpu_model(),
     }
+
+
+# The CPU plan of the workload table: the 32^2 case in full, the larger lattices for
+# 20 steps each (hot start, seed 7), plain FP64 CG to true residual 1e-10.
+CPU_PLAN = {
+    "32x32 (m2=-0.2, lambda=0.5, M=0.3, g=0.3, eps=0.01, cold start, 2000 steps)":
+        (dict(Nt=32, Nx=32, m2=-0.2, lam=0.5, g=0.3, M=0.3, epsBar=0.01), 2000, None),
+    "256x256 (same couplings, eps=0.005, s=0.5, 20 steps)":
+        (dict(Nt=256, Nx=256, m2=-0.2, lam=0.5, g=0.3, M=0.3, epsBar=0.005), 20, 7),
+    "512x512 (M=0.05, g=0.5, m2=-0.4, lambda=1, eps=0.005, 20 steps)":
+        (dict(Nt=512, Nx=512, m2=-0.4, lam=1.0, g=0.5, M=0.05, epsBar=0.005), 20, 7),
+    "1024x1024 (the headline couplings, 20 steps)":
+        (dict(Nt=1024, Nx=1024, m2=-0.4, lam=1.0, g=0.5, M=0.05, epsBar=0.005), 20, 7),
+}
+
+
+def cpu_plan():
+    """Runs CPU_PLAN with the oracle on the host cores; one JSON line."""
+    sys.path.insert(0, os.path.join(ROOT, "oracle"))
+    import numpy as np
+    import oracle as O
+
+    cores = O.set_threads(int(os.environ.get("OMP_NUM_THREADS", os.cpu_count() or 1)))
+    out = {"cpu_plan": {}, "cores": cores, "cpu": cpu_model()}
+    for name, (kw, steps, hot) in CPU_PLAN.items():
+        p = O.params(**{**WORKLOAD, **kw, "r": 1.0, "cutFraction": 0.5})
+        phi = O.hot_start(hot, p.Nt, p.Nx) if hot is not None else np.zeros((p.Nt, p.Nx))
+        iters, worst, t0 = 0, 0.0, time.perf_counter()
+        for step in range(steps):
+            phi, _, st = O.langevin_step(p, phi, step)
+            iters += st["iters"]
+            worst = max(worst, st["true_rel_resid"])
+        wall = time.perf_counter() - t0
+        out["cpu_plan"][name] = {"steps_per_s": steps / wall, "steps": steps, "seconds": wall,
+                                 "cg_iters_per_solve": iters / steps, "true_rel_resid_max": worst}
+        print(f"# {name}: {steps / wall:.3g} steps/s, {iters / steps:.0f} CG iterations per solve", flush=True)
+    print(json.dumps(out))
 
 
 def cpu_model():
@@ -326,6 +336,8 @@ def main():
     ap.add_argument("--prof-steps", type=int, default=3,
                     help="steps of the profiled pass after the timed region (roofline)")
     ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--cpu-plan", action="store_true",
+                    help="run the oracle's CPU plan (32^2 in full, 256^2 / 512^2 / 1024^2 for 20 steps) and exit")
     ap.add_argument("--workload", choices=["headline", "c4"], default="headline",
                     help="c4: the batched 64-chain 64^2 workload instead of the 1024^2 headline")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -337,6 +349,9 @@ def main():
         sys.exit(2)
     if args.copy_calibration:
         copy_calibration()
+        return
+    if args.cpu_plan:
+        cpu_plan()
         return
 
     import yukawacolourednoise_amd as Y
@@ -430,7 +445,7 @@ def main():
         out["cg_iters_learned_vs_plain"] = {"learned": res["learned"]["cg_iters_per_solve"],
                                             "plain": base["cg_iters_per_solve"] if base else None}
     if not args.no_cpu_baseline:
-        out["cpu_baseline"] = cpu_baseline(res.get("fp64", h)["cg_iters_per_solve"])
+        out["cpu_baseline"] = cpu_baseline(phi0, args.therm)
     print(json.dumps(out), flush=True)
 
 

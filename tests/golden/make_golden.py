@@ -48,8 +48,24 @@ def dirac_fixture(name, Nt, Nx):
              Ddagpsi=O.dirac(p, phi, psi, True), M=0.05, g=0.5, r=1.0)
 
 
+def c1_trajectory_fixture(name, steps=2000, every=500):
+    """the 32^2 CPU-runnable workload in full: cold start, 2000 steps, snapshots every 500"""
+    kw = dict(COMMON, Nt=32, Nx=32, adaptive=0, epsBar=0.01)
+    p = O.params(**kw)
+    phi = np.zeros((32, 32))
+    snaps, iters = [], []
+    for step in range(steps):
+        phi, _, st = O.langevin_step(p, phi, step)
+        iters.append(st["iters"])
+        if (step + 1) % every == 0:
+            snaps.append(phi.copy())
+    np.savez(os.path.join(HERE, name), snapshots=np.array(snaps), every=every, steps=steps,
+             cg_iters=np.array(iters, np.int32), **{k: v for k, v in kw.items()})
+
+
 def main():
     O.set_threads(1)
+    c1_trajectory_fixture("c1_32x32_2000_steps.npz")
     step_fixture("step_16x16_fixed.npz", 16, 16, 0, 0.01, 7, 0)
     # adaptive eps depends on max|drift|: fixed CG iteration count so both sides run the
     # same recurrence and eps is reproducible to rounding
