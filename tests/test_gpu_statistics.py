@@ -1,14 +1,27 @@
-"""Long-run statistical agreement of the GPU Langevin chain with the CPU oracle.
+"""Long-run statistical agreement of the GPU Langevin chains with the CPU oracle.
 
 Independent chains (different seeds) of the same model must give the same equilibrium
-expectation values within statistical errors.  The GPU runs 32 replicas in one batched
-context; the oracle runs 6 chains.  Each chain's time series is analysed with Wolff's
-Gamma method (yukawacolourednoise_amd.analysis.ComputeStatistics), the chain means are
-averaged, and GPU and oracle must agree within 5 combined standard errors for <phi^2>,
-<|sigma|> and V <sigma^2>.  The same comparison against an oracle run at a shifted m^2
-must fail, so the test can tell the two apart."""
-import sys
-
+expectation values within statistical errors.  On a 16x16 lattice (m^2 = -0.2,
+lambda = 0.5, g = 0.3, M = 0.3, s = 0.5, eps = 0.05) the GPU runs 32 replicas in one
+batched context for each of three solvers:
+- plain FP64 CG, the reference's solver;
+- the even-odd preconditioned pipelined mixed-precision solver;
+- flexible CG with the learned preconditioner (trained online on one replica, its
+  weights then used by all 32).
+Each solver's replicas have their own seeds and hot start, so the three comparisons are
+independent.  The oracle runs 6 chains with plain FP64 CG.  Each chain's time series is analysed with
+Wolff's Gamma method (yukawacolourednoise_amd.analysis.ComputeStatistics), the chain
+means are averaged, and every GPU solver must agree with the oracle within 5 combined
+standard errors on:
+- <phi^2>, <|sigma|> and V <sigma^2> (sigma = mean phi);
+- the chiral condensate, Tr = sum_x drift_x / (g V) of a fermion drift with fresh noise
+  (the `tr` column of traces.csv, code/src/main.cu:472-483);
+- the connected time-slice correlator G(t) at t = 0, 1, 2, 4
+  (analysis/correlations.py:220-230);
+- the fermion correlator's effective mass log(C(1) / C(2)) of the point-source
+  propagator (code/src/main.cu:441-469), with jackknife errors over measurements.
+The same comparison against an oracle run at m^2 shifted by +0.3 must fail on <phi^2>,
+so the test can tell the two apart."""
 import numpy as np
 import pytest
 
@@ -17,75 +30,156 @@ import yukawacolourednoise_amd.analysis as A
 
 pytestmark = pytest.mark.gpu
 
-NT = NX = 8
+NT = NX = 16
 V = NT * NX
 PHYS = dict(m2=-0.2, lam=0.5, g=0.3, M=0.3, cutFraction=0.5, epsBar=0.05, adaptive=0, cgTol=1e-10,
             cgTolMode=1, cgMaxIter=10000)
 THERM, STEPS, EVERY = 500, 3000, 2
+COND_EVERY, CORR_EVERY = 4, 150      # condensate and fermion-correlator measurement periods
+R = 32                               # GPU replicas per solver
+TS = (0, 1, 2, 4)                    # distances of the connected correlator
+MEAS_STEP = 10 ** 7                  # step counters of the measurement noise (unused by the chains)
 
 
-def _observables(phi):
+def _bosonic(phi):
     """phi[..., Nt, Nx] -> (<phi^2>, |sigma|, V sigma^2) per configuration."""
     f = phi.reshape(phi.shape[:-2] + (V,))
     s = f.mean(axis=-1)
     return np.stack([np.mean(f * f, axis=-1), np.abs(s), V * s * s], axis=-1)
 
 
-def _combine(series):
-    """series[chain, sample, obs] -> (mean, error) per observable from per-chain Gamma-method
-    errors of independent chains."""
-    K, _, nobs = series.shape
-    out = []
-    for o in range(nobs):
-        st = [A.ComputeStatistics(series[k, :, o]) for k in range(K)]
-        m = np.mean([s.average for s in st])
-        e = np.sqrt(np.sum([s.sigmaF ** 2 for s in st])) / K
-        out.append((m, e))
+def _gamma(series):
+    """series[chain, sample] -> (mean, error) from per-chain Gamma-method errors of
+    independent chains."""
+    st = [A.ComputeStatistics(s) for s in series]
+    return np.mean([s.average for s in st]), np.sqrt(np.sum([s.sigmaF ** 2 for s in st])) / len(st)
+
+
+def _connected(slices):
+    """slices[chain, sample, t] -> [chain, sample, len(TS)]: per-configuration products
+    mean_t0 dS(t0) dS(t0 - t) with dS = S - (the chain's mean S), whose average is the
+    connected correlator of analysis/correlations.py:220-230."""
+    d = slices - slices.mean(axis=1, keepdims=True)
+    return np.stack([np.mean(d * np.roll(d, t, axis=2), axis=2) for t in TS], axis=2)
+
+
+def _meff(corr):
+    """corr[n, t] pooled measurements -> jackknife (mean, error) of log(C(1) / C(2))."""
+    c = np.asarray(corr)
+    n = len(c)
+    tot = c.sum(axis=0)
+    sub = np.array([np.log((tot[1] - c[i, 1]) / (tot[2] - c[i, 2])) for i in range(n)])
+    full = np.log(tot[1] / tot[2])
+    return full, np.sqrt((n - 1) * np.mean((sub - sub.mean()) ** 2))
+
+
+def _summarise(bos, cond, slc, corr):
+    """per-observable (mean, error) from the chains' series"""
+    out = {}
+    for i, name in enumerate(("<phi^2>", "<|sigma|>", "V<sigma^2>")):
+        out[name] = _gamma(bos[:, :, i])
+    out["condensate"] = _gamma(cond)
+    con = _connected(slc)
+    for j, t in enumerate(TS):
+        out[f"G({t})"] = _gamma(con[:, :, j])
+    out["m_eff(1)"] = _meff(corr.reshape(-1, NT))
     return out
 
 
-def _oracle_series(m2, seeds):
+def _oracle_chains(m2, seeds, fermion=True):
     O.set_threads(1)
-    out = []
+    bos, cond, slc, corr = [], [], [], []
     for sd in seeds:
         p = O.params(Nt=NT, Nx=NX, **{**PHYS, "m2": m2}, seed=int(sd))
         phi = O.hot_start(int(sd) + 7, NT, NX)
-        obs = []
+        b, cd, sl, cr = [], [], [], []
         for step in range(THERM + STEPS):
             phi, _, st = O.langevin_step(p, phi, step)
             assert st["converged"] == 1
-            if step >= THERM and (step - THERM) % EVERY == 0:
-                obs.append(_observables(phi))
-        out.append(np.array(obs))
-    return np.array(out)
+            k = step + 1 - THERM
+            if k <= 0:
+                continue
+            if k % EVERY == 0:
+                b.append(_bosonic(phi))
+                sl.append(O.time_slices(phi))
+            if fermion and k % COND_EVERY == 0:
+                eta = O.fermion_noise(int(sd) + 5000, 0, MEAS_STEP + step, V)
+                drift, _ = O.fermion_drift_eta(p, phi, eta)
+                cd.append(drift.sum() / (PHYS["g"] * V))
+            if fermion and k % CORR_EVERY == 0:
+                cr.append(O.correlator(p, phi)[0])
+        bos.append(b)
+        slc.append(sl)
+        if fermion:
+            cond.append(cd)
+            corr.append(cr)
+    return np.array(bos), np.array(cond), np.array(slc), np.array(corr)
 
 
+SEED0 = {"fp64": 100, "eo-mixed": 200, "learned": 300}   # independent replica seeds per solver
+
+
+def _gpu_chains(Y, method, weights=None):
+    p = Y.default_params(Nt=NT, Nx=NX, nReplicas=R, **PHYS)
+    if method in ("eo-mixed", "learned"):
+        p.cgPrecision = Y.YCN_CG_MIXED
+    if method == "eo-mixed":
+        p.evenOdd = 1
+    if method == "learned":
+        p.precond = Y.YCN_PRECOND_CNN
+    bos, cond, slc, corr = [], [], [], []
+    with Y.Context(p) as c:
+        c.set_replicas(np.full(R, PHYS["m2"]), np.arange(SEED0[method], SEED0[method] + R, dtype=np.uint64))
+        if weights is not None:
+            c.precond_set_weights(weights)
+        c.hot_start(SEED0[method])
+        st = c.langevin_run(0, THERM)
+        assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+        for k in range(EVERY, STEPS + 1, EVERY):
+            st = c.langevin_run(THERM + k - EVERY, EVERY)
+            assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+            phi = c.get_field()
+            bos.append(_bosonic(phi))
+            slc.append(phi.sum(axis=2))
+            if k % COND_EVERY == 0:
+                # a fermion drift with noise no step of the chains uses: the condensate
+                drift, _ = c.fermion_force(MEAS_STEP + THERM + k)
+                cond.append(drift.reshape(R, V).sum(axis=1) / (PHYS["g"] * V))
+            if k % CORR_EVERY == 0:
+                corr.append([c.measure(rep, THERM + k, slices=False, corr=True)[2] for rep in range(R)])
+    tr = lambda a: np.swapaxes(np.array(a), 0, 1)   # [sample, replica, ...] -> [replica, sample, ...]
+    return tr(bos), tr(cond), tr(slc), tr(corr)
+
+
+def _train_weights(Y):
+    """the learned preconditioner trained online on one 16x16 replica (400 pairs)"""
+    p = Y.default_params(Nt=NT, Nx=NX, **PHYS)
+    p.cgPrecision = Y.YCN_CG_MIXED
+    p.precond = Y.YCN_PRECOND_CNN
+    with Y.Context(p) as c:
+        c.hot_start(3)
+        info = c.precond_train(0, 100, pairs_per_solve=4, lr=1e-3)
+        print(f"# learned preconditioner: {info['pairs']} pairs, loss {info['loss_first']:.3f} -> "
+              f"{info['loss_last']:.3f}", flush=True)
+        return c.precond_get_weights()
+
+
+@pytest.mark.timeout(900)
 def test_long_run_observables_agree_with_oracle(gpu):
     Y = gpu
-    R = 32
-    p = Y.default_params(Nt=NT, Nx=NX, nReplicas=R, **PHYS)
-    print(f"# GPU: {R} replicas of {NT}x{NX}, {THERM}+{STEPS} steps", flush=True)
-    with Y.Context(p) as c:
-        c.set_replicas(np.full(R, PHYS["m2"]), np.arange(100, 100 + R, dtype=np.uint64))
-        c.hot_start(1)
-        c.langevin_run(0, THERM)
-        obs = []
-        for k in range(0, STEPS, EVERY):
-            obs.append(_observables(c.get_field()))
-            st = c.langevin_run(THERM + k, EVERY)
-            assert st["converged"] == 1
-    gpu_series = np.transpose(np.array(obs), (1, 0, 2))          # [replica, sample, obs]
-    print("# oracle chains", flush=True)
-    orc = _oracle_series(PHYS["m2"], range(900, 906))
-    shifted = _oracle_series(PHYS["m2"] + 0.3, range(950, 956))
-    g, o, s = _combine(gpu_series), _combine(orc), _combine(shifted)
-    names = ["<phi^2>", "<|sigma|>", "V<sigma^2>"]
-    for n, (mg, eg), (mo, eo), (ms, es) in zip(names, g, o, s):
-        z = abs(mg - mo) / np.hypot(eg, eo)
-        zs = abs(mg - ms) / np.hypot(eg, es)
-        print(f"# {n}: gpu {mg:.5f} +- {eg:.5f}  oracle {mo:.5f} +- {eo:.5f}  (z = {z:.2f}); "
-              f"shifted m^2 {ms:.5f} +- {es:.5f} (z = {zs:.1f})", flush=True)
-        assert z < 5.0, n
-    # the shifted-m^2 run must be told apart on the most precise observable
-    zs_phi2 = abs(g[0][0] - s[0][0]) / np.hypot(g[0][1], s[0][1])
-    assert zs_phi2 > 5.0, zs_phi2
+    print(f"# oracle: 6 chains of {NT}x{NX}, {THERM}+{STEPS} steps, and 6 at m^2 + 0.3", flush=True)
+    ref = _summarise(*_oracle_chains(PHYS["m2"], range(900, 906)))
+    shifted = _gamma(_oracle_chains(PHYS["m2"] + 0.3, range(950, 956), fermion=False)[0][:, :, 0])
+    weights = _train_weights(Y)
+    for method in ("fp64", "eo-mixed", "learned"):
+        print(f"# GPU {method}: {R} replicas, {THERM}+{STEPS} steps", flush=True)
+        got = _summarise(*_gpu_chains(Y, method, weights if method == "learned" else None))
+        for name, (mo, eo) in ref.items():
+            mg, eg = got[name]
+            z = abs(mg - mo) / np.hypot(eg, eo)
+            print(f"#   {name}: gpu {mg:.5g} +- {eg:.2g}  oracle {mo:.5g} +- {eo:.2g}  (z = {z:.2f})", flush=True)
+            assert z < 5.0, (method, name, z)
+        # the shifted-m^2 run must be told apart on the most precise observable
+        zs = abs(got["<phi^2>"][0] - shifted[0]) / np.hypot(got["<phi^2>"][1], shifted[1])
+        print(f"#   <phi^2> at m^2 + 0.3: {shifted[0]:.5f} +- {shifted[1]:.5f} (z = {zs:.1f})", flush=True)
+        assert zs > 5.0, (method, zs)

@@ -17,7 +17,9 @@ can be held to the oracle's over its full length, for every solver.
   trajectory (itself held to the oracle step by step at 256^2 and 1024^2 elsewhere).
 - The 64-chain m^2 scan, 500 steps batched (512 systems per CG launch) against four of its
   chains run alone.
-- 256^2 at s = 1 and 0.25, 1000 steps: mixed and even-odd mixed against FP64."""
+- 256^2 at s = 1 and 0.25, 1000 steps: mixed and even-odd mixed against FP64.
+- The 1024^2 headline over 5000 steps: even-odd mixed and the learned preconditioner
+  against plain FP64."""
 import os
 
 import numpy as np
@@ -211,42 +213,55 @@ def test_256_trajectories(gpu, s):
         assert e < 1e-9, method
 
 
-@pytest.mark.skipif(not os.environ.get("YCN_LONG"), reason="5000 steps of plain FP64 CG at 1024^2 take about "
-                    "4 minutes; set YCN_LONG=1 to run")
 @pytest.mark.timeout(1200)
 def test_1024_headline_5000_steps(gpu):
     """The 1024^2 headline workload over 5000 steps from a hot start (init seed 7): the
-    even-odd pipelined mixed solver against the reference's solver, plain FP64 CG, on the
-    same noise.  Progress every 500 steps."""
+    even-odd pipelined mixed solver and the learned preconditioner against the
+    reference's solver, plain FP64 CG, on the same noise.  The learned chain's first 125
+    steps are its online training (about 2000 pairs), whose solves are plain FP64 CG;
+    flexible CG with the trained network runs the remaining 4875.  Progress every 500
+    steps."""
     Y = gpu
-    N, n, every = 1024, 5000, 500
+    N, n, every, train = 1024, 5000, 500, 125
     base = dict(Nt=N, Nx=N, M=0.05, g=0.5, m2=-0.4, lam=1.0, cutFraction=0.5, epsBar=0.005, adaptive=0, seed=1234,
                 cgTol=1e-10, cgTolMode=1, cgMaxIter=20000)
     phi0 = O.hot_start(7, N, N)
     import time
     fields, iters, secs = {}, {}, {}
-    for method in ("eo-mixed", "fp64"):
+    for method in ("eo-mixed", "learned", "fp64"):
         p, _ = mk(Y, **base)
-        _set_method(Y, p, method)
-        tot, wall = 0.0, 0.0
+        if method == "learned":
+            p.cgPrecision = Y.YCN_CG_MIXED
+            p.precond = Y.YCN_PRECOND_CNN
+        else:
+            _set_method(Y, p, method)
+        tot, wall, k = 0.0, 0.0, 0
         with Y.Context(p) as c:
             c.set_field(phi0)
-            for k in range(0, n, every):
+            if method == "learned":
+                info = c.precond_train(0, train, pairs_per_solve=16, lr=1e-3)
+                print(f"# learned: trained on {info['pairs']} pairs in the first {train} steps", flush=True)
+                k = train
+            while k < n:
+                m = min(every - k % every, n - k)
                 c.sync()
                 t0 = time.perf_counter()
-                st = c.langevin_run(k, every)
+                st = c.langevin_run(k, m)
                 c.sync()
                 dt = time.perf_counter() - t0
                 wall += dt
                 assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
-                tot += st["iters_mean"] * every
-                print(f"# {method}: {k + every} steps, {st['iters_mean']:.1f} iterations per solve, "
-                      f"{every / dt:.1f} steps/s", flush=True)
+                tot += st["iters_mean"] * m
+                k += m
+                print(f"# {method}: {k} steps, {st['iters_mean']:.1f} iterations per solve, "
+                      f"{m / dt:.1f} steps/s", flush=True)
             fields[method] = c.get_field()
-        iters[method], secs[method] = tot / n, wall
-    e = l2rel(fields["eo-mixed"], fields["fp64"])
-    print(f"# after {n} steps: |phi_eo-mixed - phi_fp64| / |phi_fp64| = {e:.2e}; iterations per solve "
-          f"{iters['eo-mixed']:.1f} (even-odd mixed) vs {iters['fp64']:.1f} (plain FP64); whole run "
-          f"{n / secs['eo-mixed']:.1f} vs {n / secs['fp64']:.1f} steps/s ({secs['fp64'] / secs['eo-mixed']:.2f}x)",
-          flush=True)
-    assert e < 1e-8
+        steps = n - (train if method == "learned" else 0)
+        iters[method], secs[method] = tot / steps, wall / steps
+    for method in ("eo-mixed", "learned"):
+        e = l2rel(fields[method], fields["fp64"])
+        print(f"# after {n} steps: |phi_{method} - phi_fp64| / |phi_fp64| = {e:.2e}; iterations per solve "
+              f"{iters[method]:.1f} vs {iters['fp64']:.1f} (plain FP64); "
+              f"{1 / secs[method]:.1f} vs {1 / secs['fp64']:.1f} steps/s ({secs['fp64'] / secs[method]:.2f}x)",
+              flush=True)
+        assert e < 1e-8, method
