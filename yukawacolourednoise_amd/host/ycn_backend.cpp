@@ -56,9 +56,9 @@ void DiracOP::applyD(const double *in, double *out, MatrixType t) {
     check(c_.get(), ycn_dirac_apply(c_.get(), phi_, in, out, (int)t), "DiracOP::applyD");
 }
 
-ycn_cg_stats CGsolver::solve(const double *in, double *out, DiracOP &, MatrixType t, const double *phi) {
+ycn_cg_stats CGsolver::solve(const double *in, double *out, DiracOP &D, MatrixType t, const double *phi) {
     ycn_cg_stats st{};
-    check(c_.get(), ycn_cg_solve(c_.get(), phi, in, out, (int)t, &st), "CGsolver::solve", true);
+    check(c_.get(), ycn_cg_solve(c_.get(), phi ? phi : D.scalar(), in, out, (int)t, &st), "CGsolver::solve", true);
     return st;
 }
 

@@ -47,6 +47,7 @@ class DiracOP {
 public:
     explicit DiracOP(Context &c) : c_(c) {}
     void setScalar(const double *phi) { phi_ = phi; }   // host field; nullptr = context field
+    const double *scalar() const { return phi_; }
     void applyD(const double *in, double *out, MatrixType t);
 
 private:
@@ -54,7 +55,8 @@ private:
     const double *phi_ = nullptr;
 };
 
-// CGsolver::solve(in, out, D, MType) (CGsolver.cu:42-59)
+// CGsolver::solve(in, out, D, MType) (CGsolver.cu:42-59): the operator is D's, at D's
+// scalar field (setScalar) unless phi is given
 class CGsolver {
 public:
     explicit CGsolver(Context &c) : c_(c) {}
