@@ -178,10 +178,24 @@ int ycn_fermion_force(ycn_ctx *ctx, uint64_t step, double *drift, ycn_cg_stats *
 /* One full Langevin step (main.cu:305-315 = cn(); getForce(); kli.Run(kAll)):
  * eps (nReplicas) copied out if non-NULL. */
 int ycn_langevin_step(ycn_ctx *ctx, uint64_t step, double *eps, ycn_cg_stats *stats);
-/* nsteps steps starting at counter step0 without host read-back other than CG
- * convergence polls; the step sizes stay on the device.  stats (optional) receives
- * the totals: iters = sum over steps of the per-step max, iters_mean = mean per solve. */
+/* nsteps steps starting at counter step0 (the loop of main.cu:300-502 between two
+ * exports, which synchronises and copies eps to the host every step: main.cu:305-315,
+ * FermionicDrift.cu:61-78).  The step sizes and the Langevin time stay on the device.
+ * With even-odd preconditioning and mixed precision every step after the context's
+ * first solve is one replay of a captured graph (noise, solve setup, a fixed chunk
+ * schedule sized from the previous solve, drift, update): the host waits once per
+ * segment of up to 16 steps and never inside a step; a step whose solve outlasts its
+ * schedule is finished by the host at the end of the segment.  Other configurations
+ * poll CG convergence inside each step.  Results are the same either way.
+ * stats (optional) receives the totals: iters = sum over steps of the per-step max,
+ * iters_mean = mean per solve. */
 int ycn_langevin_run(ycn_ctx *ctx, uint64_t step0, int nsteps, ycn_cg_stats *stats);
+
+/* Host waits on the context's work since its creation (the CG solvers' convergence
+ * polls, the reads of the captured steps' control block) and the captured steps the
+ * host had to finish; either pointer may be NULL.  No counterpart in the reference:
+ * it is how a caller sees how often the host synchronises inside its Langevin runs. */
+int ycn_host_syncs(ycn_ctx *ctx, int64_t *syncs, int64_t *halted_steps);
 
 /* Measurement block (main.cu:386-483): magnetisation, sum phi^2, eps, Langevin time
  * for replica `replica`; slices (Nt doubles, gpuTimeSlices) and corr (Nt doubles,

@@ -37,7 +37,7 @@ EXPORTED = [
     "ycn_tuning_report",
     "ycn_set_replicas", "ycn_set_eps_bar", "ycn_set_field", "ycn_get_field", "ycn_hot_start",
     "ycn_coloured_noise", "ycn_coloured_filter", "ycn_coloured_noise_noalloc", "ycn_dirac_apply", "ycn_cg_solve",
-    "ycn_fermion_force", "ycn_langevin_step", "ycn_langevin_run", "ycn_measure", "ycn_sync",
+    "ycn_fermion_force", "ycn_langevin_step", "ycn_langevin_run", "ycn_host_syncs", "ycn_measure", "ycn_sync",
     "ycn_profile_begin", "ycn_profile_end",
     "ycn_precond_set_weights", "ycn_precond_get_weights", "ycn_precond_apply", "ycn_precond_loss_grad",
     "ycn_precond_train",
@@ -118,6 +118,7 @@ def _load():
     L.ycn_fermion_force.argtypes = [vp, C.c_uint64, dp, C.POINTER(CgStats)]
     L.ycn_langevin_step.argtypes = [vp, C.c_uint64, dp, C.POINTER(CgStats)]
     L.ycn_langevin_run.argtypes = [vp, C.c_uint64, C.c_int, C.POINTER(CgStats)]
+    L.ycn_host_syncs.argtypes = [vp, C.POINTER(C.c_int64), C.POINTER(C.c_int64)]
     L.ycn_measure.argtypes = [vp, C.c_int, C.c_uint64, C.POINTER(Observables), dp, dp]
     L.ycn_sync.argtypes = [vp]
     L.ycn_profile_begin.argtypes = [vp, C.c_char_p]
@@ -284,6 +285,13 @@ class Context:
         st = CgStats()
         self._check(lib.ycn_langevin_run(self._h, step0, nsteps, C.byref(st)), allow_nc=True)
         return st.as_dict()
+
+    def host_syncs(self):
+        """(host waits on the context's work since creation, captured steps the host
+        finished)"""
+        n, h = C.c_int64(), C.c_int64()
+        self._check(lib.ycn_host_syncs(self._h, C.byref(n), C.byref(h)))
+        return n.value, h.value
 
     def measure(self, replica=0, step=0, slices=True, corr=True):
         ob = Observables()

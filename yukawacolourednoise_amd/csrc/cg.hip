@@ -527,9 +527,15 @@ k_spinor_slices(const double2 *__restrict__ psi, int Nt, int Nx, double *__restr
 // ---------------------------------------------------------------------------
 // mixed-precision kernels
 // ---------------------------------------------------------------------------
-__global__ void __launch_bounds__(kNT) k_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n) {
+__global__ void __launch_bounds__(kNT)
+k_to_f32(const double *__restrict__ a, float *__restrict__ b, int64_t n, const int *__restrict__ halt,
+         double *__restrict__ copy) {
+    if (halt && *halt) return;
     const int64_t i = (int64_t)blockIdx.x * kNT + threadIdx.x;
-    if (i < n) b[i] = (float)a[i];
+    if (i >= n) return;
+    const double v = a[i];
+    b[i] = (float)v;
+    if (copy) copy[i] = v;
 }
 
 template <bool GEN>
@@ -673,6 +679,13 @@ int cg_alloc(Ctx &c) {
 void cg_free(Ctx &c) {
     for (const Ctx::GraphEntry &e : c.graphs) (void)hipGraphExecDestroy(e.exec);
     c.graphs.clear();
+    for (const Ctx::StepGraph &e : c.stepGraphs) (void)hipGraphExecDestroy(e.exec);
+    c.stepGraphs.clear();
+    if (c.ctl_d) (void)hipFree(c.ctl_d);
+    if (c.ctl_h) (void)hipHostFree(c.ctl_h);
+    if (c.phiSolve) (void)hipFree(c.phiSolve);
+    c.ctl_d = c.ctl_h = nullptr;
+    c.phiSolve = nullptr;
     double2 **bufs[] = {&c.eta, &c.x, &c.r, &c.pv[0], &c.pv[1], &c.t, &c.Ap};
     for (double2 **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
@@ -793,8 +806,8 @@ int launch_mp_accum(Ctx &c, int nsys, int n) {
     return YCN_OK;
 }
 
-int to_f32_n(Ctx &c, const double *a, float *b, int64_t n) {
-    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, a, b, n);
+int to_f32_n(Ctx &c, const double *a, float *b, int64_t n, const int *halt, double *copy) {
+    hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((n + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, a, b, n, halt, copy);
     YCN_HIP(hipGetLastError());
     return YCN_OK;
 }
@@ -803,6 +816,7 @@ int to_f32_n(Ctx &c, const double *a, float *b, int64_t n) {
 // step, and each blocking wait leaves the GPU idle until the host thread is scheduled
 // again.
 int spin_sync(Ctx &c) {
+    ++c.hostSyncs;
     YCN_HIP(hipEventRecord(c.pollEv, c.stream));
     hipError_t e;
     while ((e = hipEventQuery(c.pollEv)) == hipErrorNotReady) {
@@ -1088,7 +1102,7 @@ int mp_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
     const int nrepUsed = (nsys + c.nNoise - 1) / c.nNoise;
     const int64_t nconv = std::min<int64_t>((int64_t)c.nRep * c.V, (int64_t)nrepUsed * c.V);
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((nconv + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, phi, c.phis,
-                       nconv);
+                       nconv, (const int *)nullptr, (double *)nullptr);
     int rc = set_int(c, c.active, nsys);
     if (rc) return rc;
     const int maxIter = c.p.cgMaxIter;
@@ -1185,7 +1199,7 @@ int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t
     const int nrepUsed = (nsys + c.nNoise - 1) / c.nNoise;
     const int64_t nconv = std::min<int64_t>((int64_t)c.nRep * c.V, (int64_t)nrepUsed * c.V);
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((nconv + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, phi, c.phis,
-                       nconv);
+                       nconv, (const int *)nullptr, (double *)nullptr);
     int rc = set_int(c, c.active, nsys);
     if (rc) return rc;
     const int maxIter = c.p.cgMaxIter;
@@ -1263,7 +1277,7 @@ int train_preconditioner(Ctx &c, uint64_t step0, int nsteps, int pairsPerSolve, 
         if (rc) return rc;
         const double *phi = c.phi[c.cur];
         hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((c.V + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, phi, c.phis,
-                           (int64_t)c.V);
+                           (int64_t)c.V, (const int *)nullptr, (double *)nullptr);
         // FP64 CG (the reference recurrence) with snapshots
         rc = set_int(c, c.active, 1);
         if (rc) return rc;

@@ -43,7 +43,13 @@ struct EoArgs {
     double r, M, g;
     int tilesX, tilesPerSys, xcdRemap;
     float invNt, invNx;              // 1 / Nt, 1 / Nx in single precision (wrapf)
+    // graph-captured steps (else null): the setup, drift kernels return at once while
+    // *halt is set, and k_eo_eta takes its noise counter from *stepPtr
+    const int *halt;
+    const uint64_t *stepPtr;
 };
+
+__device__ __forceinline__ bool eo_halted(const EoArgs &A) { return A.halt && *A.halt; }
 
 // site (t, x) of compact index j on parity par
 __device__ __forceinline__ void eo_site(const EoArgs &A, int par, int j, int &t, int &x) {
@@ -92,6 +98,8 @@ __global__ void __launch_bounds__(kNT)
 k_eo_eta(EoArgs A, const uint64_t *__restrict__ seeds, uint32_t streamBase, uint64_t step, const double *__restrict__ phi,
          double2 *__restrict__ eta, double2 *__restrict__ u, double2 *__restrict__ part) {
     __shared__ double2 lds[kNT / kWave];
+    if (eo_halted(A)) return;
+    if (A.stepPtr) step = *A.stepPtr;
     const int sys = blockIdx.y, s = blockIdx.x * kNT + threadIdx.x;
     const size_t V = A.V, off = (size_t)sys * 4 * V;
     double2 acc = make_double2(0.0, 0.0);
@@ -122,6 +130,7 @@ k_eo_eta(EoArgs A, const uint64_t *__restrict__ seeds, uint32_t streamBase, uint
 // b_e = eta_e - K_eo u_o with u_o = A_o^-1 eta_o (from k_eo_eta)
 __global__ void __launch_bounds__(kNT)
 k_eo_rhs(EoArgs A, const double2 *__restrict__ eta, const double2 *__restrict__ u, double2 *__restrict__ b) {
+    if (eo_halted(A)) return;
     const int sys = blockIdx.y, j = blockIdx.x * kNT + threadIdx.x;
     if (j >= A.Vh) return;
     int t, x;
@@ -141,6 +150,7 @@ k_eo_rhs(EoArgs A, const double2 *__restrict__ eta, const double2 *__restrict__ 
 template <bool DAG>
 __global__ void __launch_bounds__(kNT)
 k_eo_hop_oe(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__ in, double2 *__restrict__ U) {
+    if (eo_halted(A)) return;
     const int sys = blockIdx.y, j = blockIdx.x * kNT + threadIdx.x;
     if (j >= A.Vh) return;
     int t, x;
@@ -171,6 +181,7 @@ k_eo_hop_eo(EoArgs A, const double *__restrict__ phi, const double2 *__restrict_
             const double2 *__restrict__ b, double2 *__restrict__ out, float2 *__restrict__ out32,
             double2 *__restrict__ part, EoZero z) {
     __shared__ double2 lds[kNT / kWave];
+    if (eo_halted(A)) return;
     const int sys = blockIdx.y, j = blockIdx.x * kNT + threadIdx.x;
     const size_t Vh = A.Vh, off = (size_t)sys * 4 * Vh;
     double2 acc = make_double2(0.0, 0.0);
@@ -211,6 +222,7 @@ k_eo_hop_eo(EoArgs A, const double *__restrict__ phi, const double2 *__restrict_
 __global__ void __launch_bounds__(kNT)
 k_eo_drift(EoArgs A, const double *__restrict__ phi, const double2 *__restrict__ chi_e,
            const double2 *__restrict__ eta, double *__restrict__ drift, double fac) {
+    if (eo_halted(A)) return;
     const int s = blockIdx.x * kNT + threadIdx.x;
     if (s >= A.V) return;
     const int rep = blockIdx.y;
@@ -323,8 +335,12 @@ __device__ __forceinline__ double2 eo_fin_sum(const double2 *part, int nb, doubl
 __global__ void __launch_bounds__(kNT)
 k_fin_eo_setup(const double2 *__restrict__ partE, int nbE, const double2 *__restrict__ partC, int nbC,
                CgState *__restrict__ st, int *__restrict__ active, double tol, int mode, int maxIter, int maxRestarts,
-               double stopScale) {
+               double stopScale, const StepCtl *__restrict__ ctl) {
     __shared__ double2 lds0[kNT / kWave], lds1[kNT / kWave];
+    if (ctl) {   // graph-captured step: nothing while halted; the threshold learned on the device
+        if (ctl->halt) return;
+        stopScale = ctl->stopScale;
+    }
     const int sys = blockIdx.x;
     const double2 e = eo_fin_sum(partE + (size_t)sys * nbE, nbE, lds0);
     const double2 cc = eo_fin_sum(partC + (size_t)sys * nbC, nbC, lds1);
@@ -342,6 +358,7 @@ k_fin_eo_setup(const double2 *__restrict__ partE, int nbE, const double2 *__rest
     S.stop0 = (mode == YCN_TOL_ABS_RECURSIVE) ? tol : tol * sqrt(cc.x);
     S.stop = (mode == YCN_TOL_REL_TRUE) ? S.stop0 * stopScale : S.stop0;
     S.need = 0.0;
+    S.nRu = 0;
     S.beta = 0.0;
     S.alpha = make_double2(0.0, 0.0);
     S.iter = 0;
@@ -937,6 +954,8 @@ k_fin_eo_pf_true(const double2 *__restrict__ partRes, const double2 *__restrict_
     CgState &S = st[sys];
     S.true_rel = S.enorm2 > 0.0 ? sqrt(res.x / S.enorm2) : sqrt(res.x);
     S.restarts += 1;
+    if (S.nRu < kMaxRounds) S.ruIter[S.nRu] = S.iter;
+    S.nRu += 1;
     // paused at the stop (not for the drop since the last anchor): the normal-equation
     // threshold the true residual asks for
     if (S.done && sqrt(S.rr) <= S.stop && S.true_rel > 0.0) S.need = sqrt(rn.x) * S.tol / S.true_rel;
@@ -960,6 +979,60 @@ k_fin_eo_pf_true(const double2 *__restrict__ partRes, const double2 *__restrict_
         S.done = 0;
         atomicAdd(active, 1);
     }
+}
+
+// graph-captured Langevin steps: start of a step.  The count of active systems of its
+// solve, and the halt flag and counter the step's coloured noise reads, copied into a
+// second block (noise): the noise runs on a branch of the graph beside the solve, whose
+// check kernel may change the first block while the noise kernels run.
+__global__ void k_step_begin(const StepCtl *__restrict__ ctl, StepCtl *__restrict__ noise, int *__restrict__ active,
+                             int nsys) {
+    if (threadIdx.x) return;
+    if (!ctl->halt) *active = nsys;
+    noise->halt = ctl->halt;
+    noise->step = ctl->step;
+}
+
+// graph-captured Langevin steps: after the step's chunk schedule.  Every system final:
+// the step's statistics into the control block, the next solve's starting threshold
+// (relative mode: 0.9 times the largest ratio need / stop0 the systems reported, as
+// eo_learn_stop_scale does on the host) and the next step's counter.  A system still
+// iterating halts the run at this step: the drift, the update and every later step's
+// kernels outside the chunk schedule return at once.
+__global__ void k_step_check(StepCtl *__restrict__ ctl, const CgState *__restrict__ st, int nsys, int relMode) {
+    if (threadIdx.x || ctl->halt) return;
+    int itMax = -1, kMax = 0, rsMax = 0, conv = 1;
+    double itSum = 0.0, rrMax = 0.0, trMax = 0.0, ratio = 0.0;
+    for (int k = 0; k < nsys; ++k) {
+        const CgState &S = st[k];
+        if (!S.final_) {
+            ctl->halt = 1;
+            ctl->haltStep = ctl->step;
+            return;
+        }
+        if (S.iter > itMax) {
+            itMax = S.iter;
+            kMax = k;
+        }
+        rsMax = max(rsMax, S.restarts);
+        conv &= S.converged;
+        itSum += S.iter;
+        rrMax = fmax(rrMax, S.rr);
+        trMax = fmax(trMax, S.true_rel);
+        if (S.need > 0.0 && S.stop0 > 0.0) ratio = fmax(ratio, S.need / S.stop0);
+    }
+    ctl->itersMaxSum += itMax;
+    ctl->itersMeanSum += itSum / nsys;
+    ctl->restarts += rsMax;
+    ctl->converged &= conv;
+    ctl->rrMax = fmax(ctl->rrMax, rrMax);
+    ctl->trueRelMax = fmax(ctl->trueRelMax, trMax);
+    ctl->lastIters = itMax;
+    ctl->nRounds = min(st[kMax].nRu, kMaxRounds);
+    for (int i = 0; i < ctl->nRounds; ++i) ctl->rounds[i] = st[kMax].ruIter[i];
+    if (relMode && ratio > 0.0) ctl->stopScale = fmin(1.0, fmax(1e-3, 0.9 * ratio));
+    ctl->steps += 1;
+    ctl->step += 1;
 }
 
 // ---------------------------------------------------------------------------
@@ -995,6 +1068,8 @@ static EoArgs eo_args(const Ctx &c, int TT, int TX) {
     A.xcdRemap = (A.tilesPerSys % 8 == 0) ? 1 : 0;
     A.invNt = 1.0f / (float)A.Nt;
     A.invNx = 1.0f / (float)A.Nx;
+    A.halt = nullptr;
+    A.stepPtr = nullptr;
     return A;
 }
 
@@ -1328,24 +1403,32 @@ static void eo_learn_stop_scale(Ctx &c, int nsys) {
     if (r > 0.0) c.eoStopScale = std::min(1.0, std::max(1e-3, 0.9 * r));
 }
 
-// Solve S chi_e = b_e for every system (eta from the Philox streams when gen); chi_e is
-// left in c.x, eta in c.eta.  FP64 CG, or FP32 inner iterations with FP64 reliable
-// updates when cgPrecision is mixed.
-int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase, uint64_t step, SolveStats *st) {
-    if (!eo_available(c)) {
-        c.err = "even-odd preconditioning needs even Nt and Nx and a fused tile that divides the lattice";
-        return YCN_ERR_UNSUPPORTED;
+// the pipelined FP32 solve with FP64 reliable updates runs where its buffers exist
+static bool eo_mixed(const Ctx &c) {
+    return c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfR1 && c.pfA1 &&
+           c.pfSlot[0] && c.pfSlot[1];
+}
+
+// Setup of a solve: eta (from the Philox streams when gen) and u_o = A_o^-1 eta_o,
+// b_e = eta_e - K_eo u_o, r = S^T b (FP32 copy when mixed), x and the directions cleared,
+// the FP32 copy of the field (mixed) and the per-system state.  ctl (a graph-captured
+// step, else null): every launch returns at once while ctl->halt is set, the noise
+// counter is ctl->step and the starting threshold ctl->stopScale.
+static int eo_setup(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase, uint64_t step, bool mixed,
+                    const StepCtl *ctl) {
+    EoArgs A = eo_args(c, 0, 0);
+    if (ctl) {
+        A.halt = &ctl->halt;
+        A.stepPtr = &ctl->step;
     }
-    const bool mixed = c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfR1 && c.pfA1 &&
-                       c.pfSlot[0] && c.pfSlot[1];
-    const EoArgs A = eo_args(c, 0, 0);
-    const int Vh = c.V / 2;
     const dim3 gF((c.V + kNT - 1) / kNT, nsys), gH = half_grid(c, nsys);
     const int nbF = gF.x, nbH = gH.x;
     double2 *partE = c.partials, *partH = c.partials + (size_t)nsys * nbF;
     const EoBufs B = eo_bufs(c, nsys);
-    int rc = set_int(c, c.active, nsys);
-    if (rc) return rc;
+    if (!ctl) {   // (a captured step sets the count in k_step_begin)
+        const int rc = set_int(c, c.active, nsys);
+        if (rc) return rc;
+    }
     const int maxIter = c.p.cgMaxIter, maxRestarts = mixed ? 1 << 20 : 4;
     if (gen)
         hipLaunchKernelGGL(k_eo_eta<true>, gF, dim3(kNT), 0, c.stream, A, (const uint64_t *)c.seed_d, streamBase, step,
@@ -1372,84 +1455,140 @@ int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase,
                        zero);
     if (mixed) {
         const int nrepUsed = (nsys + c.nNoise - 1) / c.nNoise;
-        rc = to_f32_n(c, phi, c.phis, (int64_t)std::min(nrepUsed, c.nRep) * c.V);
+        // a captured step also keeps the FP64 field its solve reads in c.phiSolve: the
+        // reliable updates of a halted solve that later steps' chunks run read it there
+        const int rc = to_f32_n(c, phi, c.phis, (int64_t)std::min(nrepUsed, c.nRep) * c.V,
+                                ctl ? &ctl->halt : nullptr, ctl ? c.phiSolve : nullptr);
         if (rc) return rc;
     }
     hipLaunchKernelGGL(k_fin_eo_setup, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partE, nbF,
                        (const double2 *)partH, nbH, c.cg_d, c.active, c.p.cgTol, c.p.cgTolMode, maxIter, maxRestarts,
-                       c.eoStopScale);
+                       c.eoStopScale, ctl);
     YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
 
-    int launched = 0, active = nsys;
-    if (mixed) {
-        // reliable-update partials: at most V / 64 blocks per system each (smallest FP64 tile)
-        double2 *partRu = c.partials, *partRu2 = c.partials + (size_t)nsys * ((c.V + 63) / 64);
-        const int every = std::max(1, c.p.reliableEvery);
-        const int nIt = every + (every & 1);
-        const int hardCap = maxIter + nIt + 8;
-        int itDone = 0;   // systems already converged at setup exit the chunk kernels at once
-        // profiling the iteration with the prologue finalize: a chunk is nothing but
-        // back-to-back iteration launches, so one event pair brackets it and its working
-        // launches are the iterations the systems did in it (the most of any system)
-        const bool chunkProf = c.profTarget == PK_NORMAL_OP && c.eoFinFused;
-        std::vector<int> itPrev(chunkProf ? nsys : 0, 0);
-        bool polled = true;
-        while (active > 0 && launched < hardCap) {
-            // chunks of the reliable-update interval until the previous solve's iteration
-            // count is near, then just past it (a multiple of 8), so that few launches run
-            // after the systems have paused
-            int n = nIt;
-            const int rem = c.eoPredIters - itDone + 4;
-            if (c.eoPredIters > 0 && rem < nIt) n = std::max(8, (rem + 7) / 8 * 8);
-            const int slot = chunkProf ? prof_chunk_begin(c, PK_NORMAL_OP) : -1;
-            rc = run_chunk(c, nullptr, nsys, YCN_DAGGER, n, 4, c.tileEo32,
-                           [&](int i) { launch_eo_pf_iteration(c, nsys, i); });
-            prof_chunk_end(c, slot);
-            if (rc) return rc;
-            if (c.eoFinFused) launch_fin_pf_chunk(c, nsys, n);
-            launched += n;
-            // reliable update: x += x_s, p[0] = p[1]; res = b - S x; r = S^T res (FP64, FP32
-            // copy into r[0])
-            hipLaunchKernelGGL(k_pf_reliable, gH, dim3(kNT), 0, c.stream, Vh, (const CgState *)c.cg_d, c.x, c.xs,
-                               (const float2 *)c.ps[1], c.ps[0]);
-            const int nbR = launch_eo_ru(c, nsys, phi, c.x, B.b, c.r, c.rs, partRu, partRu2);
-            hipLaunchKernelGGL(k_fin_eo_pf_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partRu,
-                               (const double2 *)partRu2, nbR, c.cg_d, c.active);
-            YCN_HIP(hipGetLastError());
-            // poll only where the systems may have converged: well short of the previous
-            // solve's iteration count they cannot have, so the next chunk is enqueued at
-            // once (its kernels exit at once for systems that are final after all)
-            const int margin = std::max(8, c.eoPredIters / 8);
-            if (c.eoPredIters > 0 && itDone + n + margin < c.eoPredIters && !chunkProf) {
-                itDone += n;
-                polled = false;
-                continue;
-            }
-            rc = poll_iters(c, nsys, active, itDone);
-            if (rc) return rc;
-            polled = true;
-            if (chunkProf) {
-                int work = 0;
-                for (int k = 0; k < nsys; ++k) {
-                    work = std::max(work, c.cg_h[k].iter - itPrev[k]);
-                    itPrev[k] = c.cg_h[k].iter;
-                }
-                if (slot >= 0) c.evWork[slot] = work;
-            }
-        }
-        if (!polled) {   // the launch cap ended the loop after a skipped poll
-            rc = poll_iters(c, nsys, active, itDone);
-            if (rc) return rc;
-        }
-        SolveStats out;   // the last poll left the final states in the mirror
-        rc = stats_from_mirror(c, nsys, &out);
+// One chunk of n pipelined FP32 iterations and the reliable update that closes it:
+// x += x_s, p[0] = p[1]; res = b - S x; r = S^T res (FP64, FP32 copy into r[0]); then
+// converged on the FP64 true residual, or resume.  Systems already final exit every
+// kernel at once.
+static int eo_mixed_chunk(Ctx &c, const double *phi, int nsys, int n, bool graphed) {
+    const int Vh = c.V / 2;
+    const dim3 gH = half_grid(c, nsys);
+    const EoBufs B = eo_bufs(c, nsys);
+    // reliable-update partials: at most V / 64 blocks per system each (smallest FP64 tile)
+    double2 *partRu = c.partials, *partRu2 = c.partials + (size_t)nsys * ((c.V + 63) / 64);
+    if (graphed) {   // already inside a capture: the launches themselves
+        for (int i = 0; i < n; ++i) launch_eo_pf_iteration(c, nsys, i);
+    } else {
+        const int rc = run_chunk(c, nullptr, nsys, YCN_DAGGER, n, 4, c.tileEo32,
+                                 [&](int i) { launch_eo_pf_iteration(c, nsys, i); });
         if (rc) return rc;
-        c.eoPredIters = out.iters_max;
-        eo_learn_stop_scale(c, nsys);
-        if (st) *st = out;
-        return YCN_OK;
     }
+    if (c.eoFinFused) launch_fin_pf_chunk(c, nsys, n);
+    hipLaunchKernelGGL(k_pf_reliable, gH, dim3(kNT), 0, c.stream, Vh, (const CgState *)c.cg_d, c.x, c.xs,
+                       (const float2 *)c.ps[1], c.ps[0]);
+    const int nbR = launch_eo_ru(c, nsys, phi, c.x, B.b, c.r, c.rs, partRu, partRu2);
+    hipLaunchKernelGGL(k_fin_eo_pf_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partRu,
+                       (const double2 *)partRu2, nbR, c.cg_d, c.active);
+    YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
 
+// Length of the next chunk: the reliable-update interval until the predicted iteration
+// count is near, then just past it (a multiple of 8), so that few launches run after the
+// systems have paused.
+static int eo_chunk_len(const Ctx &c, int pred, int itDone) {
+    const int every = std::max(1, c.p.reliableEvery);
+    const int nIt = every + (every & 1);
+    const int rem = pred - itDone + 4;
+    if (pred > 0 && rem < nIt) return std::max(8, (rem + 7) / 8 * 8);
+    return nIt;
+}
+
+// the eager loop polls once it is within this many iterations of the prediction
+static int eo_poll_margin(int pred) { return std::max(8, pred / 8); }
+
+// The pipelined FP32 solve from the state the setup (or an unfinished captured solve)
+// left, until every system is final, polling the systems only where they may have
+// converged; itDone: iterations already done (the chunk lengths count from there).
+// The summary goes to *st, the iteration count and the starting threshold of the next
+// solve are learned from it.
+static int eo_mixed_loop(Ctx &c, const double *phi, int nsys, int itDone, SolveStats *st) {
+    const int every = std::max(1, c.p.reliableEvery);
+    const int nIt = every + (every & 1);
+    const int hardCap = c.p.cgMaxIter + nIt + 8;
+    int launched = 0, active = nsys, rc;
+    // profiling the iteration with the prologue finalize: a chunk is nothing but
+    // back-to-back iteration launches, so one event pair brackets it and its working
+    // launches are the iterations the systems did in it (the most of any system)
+    const bool chunkProf = c.profTarget == PK_NORMAL_OP && c.eoFinFused;
+    std::vector<int> itPrev(chunkProf ? nsys : 0, 0);
+    bool polled = true;
+    while (active > 0 && launched < hardCap) {
+        const int n = eo_chunk_len(c, c.eoPredIters, itDone);
+        const int slot = chunkProf ? prof_chunk_begin(c, PK_NORMAL_OP) : -1;
+        rc = eo_mixed_chunk(c, phi, nsys, n, false);
+        prof_chunk_end(c, slot);
+        if (rc) return rc;
+        launched += n;
+        // poll only where the systems may have converged: well short of the previous
+        // solve's iteration count they cannot have, so the next chunk is enqueued at
+        // once (its kernels exit at once for systems that are final after all)
+        if (c.eoPredIters > 0 && itDone + n + eo_poll_margin(c.eoPredIters) < c.eoPredIters && !chunkProf) {
+            itDone += n;
+            polled = false;
+            continue;
+        }
+        rc = poll_iters(c, nsys, active, itDone);
+        if (rc) return rc;
+        polled = true;
+        if (chunkProf) {
+            int work = 0;
+            for (int k = 0; k < nsys; ++k) {
+                work = std::max(work, c.cg_h[k].iter - itPrev[k]);
+                itPrev[k] = c.cg_h[k].iter;
+            }
+            if (slot >= 0) c.evWork[slot] = work;
+        }
+    }
+    if (!polled) {   // the launch cap ended the loop after a skipped poll
+        rc = poll_iters(c, nsys, active, itDone);
+        if (rc) return rc;
+    }
+    SolveStats out;   // the last poll left the final states in the mirror
+    rc = stats_from_mirror(c, nsys, &out);
+    if (rc) return rc;
+    c.eoPredIters = out.iters_max;
+    int kMax = 0;
+    for (int k = 1; k < nsys; ++k)
+        if (c.cg_h[k].iter > c.cg_h[kMax].iter) kMax = k;
+    c.eoRounds.assign(c.cg_h[kMax].ruIter, c.cg_h[kMax].ruIter + std::min(c.cg_h[kMax].nRu, kMaxRounds));
+    eo_learn_stop_scale(c, nsys);
+    if (st) *st = out;
+    return YCN_OK;
+}
+
+// Solve S chi_e = b_e for every system (eta from the Philox streams when gen); chi_e is
+// left in c.x, eta in c.eta.  FP64 CG, or FP32 inner iterations with FP64 reliable
+// updates when cgPrecision is mixed.
+int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase, uint64_t step, SolveStats *st) {
+    if (!eo_available(c)) {
+        c.err = "even-odd preconditioning needs even Nt and Nx and a fused tile that divides the lattice";
+        return YCN_ERR_UNSUPPORTED;
+    }
+    const bool mixed = eo_mixed(c);
+    int rc = eo_setup(c, phi, nsys, gen, streamBase, step, mixed, nullptr);
+    if (rc) return rc;
+    if (mixed) return eo_mixed_loop(c, phi, nsys, 0, st);
+
+    const EoArgs A = eo_args(c, 0, 0);
+    const dim3 gH = half_grid(c, nsys);
+    const int nbH = gH.x;
+    double2 *partH = c.partials + (size_t)nsys * ((c.V + kNT - 1) / kNT);
+    const EoBufs B = eo_bufs(c, nsys);
+    const int maxIter = c.p.cgMaxIter, maxRestarts = 4;
+    int launched = 0, active = nsys;
     int chunk = std::max(kGraphIters, std::min(c.lastIters, maxIter));
     const int hardCap = maxIter * (maxRestarts + 1) + 2 * kGraphIters;
     for (int guard = 0; guard < maxRestarts + 2; ++guard) {
@@ -1505,12 +1644,259 @@ int eo_propagator(Ctx &c, const double *phi, double2 *out, SolveStats *st) {
     return YCN_OK;
 }
 
-int eo_drift(Ctx &c, const double *phi, double *drift) {
-    const EoArgs A = eo_args(c, 0, 0);
+int eo_drift(Ctx &c, const double *phi, double *drift, const StepCtl *ctl) {
+    EoArgs A = eo_args(c, 0, 0);
+    if (ctl) A.halt = &ctl->halt;
     const double fac = -c.p.g * (c.p.driftMode == 1 ? c.p.cutFraction * c.p.cutFraction : 1.0);
     hipLaunchKernelGGL(k_eo_drift, dim3((c.V + kNT - 1) / kNT, c.nRep), dim3(kNT), 0, c.stream, A, phi,
                        (const double2 *)c.x, (const double2 *)c.eta, drift, fac);
     YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+
+// ---------------------------------------------------------------------------
+// graph-captured Langevin steps
+// ---------------------------------------------------------------------------
+// A step replays one captured graph -- the solve's setup, a fixed chunk schedule of the
+// pipelined FP32 iteration with its reliable updates, k_step_check and the drift --
+// between k_step_begin and the coloured noise before it and the update after it, with no
+// host wait.  The schedule repeats the previous solve's rounds (eo_schedule), then `tail`
+// chunks of 8.  A solve that needs more than the schedule holds halts the run
+// (k_step_check): the later steps' kernels return at once except their chunk schedules,
+// which keep iterating the halted solve (its FP32 and FP64 fields stay in c.phis and
+// c.phiSolve), and the host finishes the halted step when it reads the control block at
+// the end of the segment, then resumes.  Every chunk past the solve's end costs its
+// launches (their kernels exit at once) and a halted step the rest of its segment, so the
+// tail starts at one chunk, grows by one after each halted step and shrinks by one after
+// four segments without one.  The field ping-pongs between the context's two buffers with
+// a fixed step size and is updated in place with an adaptive one, so each schedule has at
+// most two executables (one per field buffer).
+constexpr int kStepSegment = 16;   // captured steps between two reads of the control block
+constexpr int kMaxTail = 8;
+constexpr size_t kMaxStepGraphs = 16;
+
+bool eo_graph_eligible(const Ctx &c) {
+    return c.stepGraphsOn && c.useGraphs && c.profTarget == PK_NONE && c.p.evenOdd && eo_available(c) &&
+           eo_mixed(c) && c.p.precond == YCN_PRECOND_NONE && c.eoPredIters > 0;
+}
+
+// chunks of the schedule's main part for a predicted count when no earlier solve's
+// reliable updates are known: the chunks the eager loop launches up to its first poll
+static std::vector<int> eo_main_chunks(const Ctx &c, int pred) {
+    const int every = std::max(1, c.p.reliableEvery);
+    const int cap = c.p.cgMaxIter + every + (every & 1) + 8;
+    std::vector<int> v;
+    int itDone = 0;
+    do {
+        v.push_back(eo_chunk_len(c, pred, itDone));
+        itDone += v.back();
+    } while (itDone + eo_poll_margin(pred) < pred && itDone < cap);
+    return v;
+}
+
+// The chunk schedule of a captured step: one chunk per reliable update of the previous
+// solve (its longest system), each as long as that solve's round (the iterations between
+// two reliable updates: up to a pause for the residual drop, to the stop, or to the end of
+// an eager chunk) plus 2, rounded up to a multiple of 8 (so that consecutive solves mostly
+// share one schedule and its captured graphs), then `tail` chunks of 8.  A
+// round of this solve that ends a little later than the previous one's ends at the chunk
+// boundary instead and the rest moves to the next round; a solve that needs more than
+// the schedule holds halts the run.
+constexpr int kTailRound = 8;
+static std::vector<int> eo_schedule(const Ctx &c, int tail) {
+    std::vector<int> v;
+    if (c.eoRounds.empty()) {
+        v = eo_main_chunks(c, c.eoPredIters);
+    } else {
+        int prev = 0;
+        for (int r : c.eoRounds) {
+            v.push_back(std::max(8, (r - prev + 2 + 7) / 8 * 8));
+            prev = r;
+        }
+    }
+    for (int k = 0; k < tail; ++k) v.push_back(kTailRound);
+    return v;
+}
+
+// the captured part of a step: the solve's setup, its chunk schedule, k_step_check and
+// the drift (the step's noise and update are launched around it by eo_enqueue_step)
+static int eo_capture_step(Ctx &c, int parity, const std::vector<int> &chunks, hipGraphExec_t *out) {
+    const double *phiIn = c.phi[parity];
+    const int nsys = c.nSys;
+    hipGraph_t g = nullptr;
+    YCN_HIP(hipStreamBeginCapture(c.stream, hipStreamCaptureModeThreadLocal));
+    int rc = eo_setup(c, phiIn, nsys, true, 1u, 0, true, c.ctl_d);
+    for (int n : chunks)
+        if (!rc) rc = eo_mixed_chunk(c, c.phiSolve, nsys, n, true);
+    if (!rc) {
+        hipLaunchKernelGGL(k_step_check, dim3(1), dim3(kWave), 0, c.stream, c.ctl_d, (const CgState *)c.cg_d, nsys,
+                           c.p.cgTolMode == YCN_TOL_REL_TRUE ? 1 : 0);
+        rc = eo_drift(c, phiIn, c.drift, c.ctl_d);   // fDrift.getForce(drift)
+    }
+    const hipError_t le = hipGetLastError();
+    const hipError_t ee = hipStreamEndCapture(c.stream, &g);
+    if (rc || le != hipSuccess || ee != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        if (!rc) {
+            c.err = std::string("step capture: ") + hipGetErrorString(le != hipSuccess ? le : ee);
+            rc = YCN_ERR_HIP;
+        }
+        return rc;
+    }
+    const hipError_t ie = hipGraphInstantiate(out, g, nullptr, nullptr, 0);
+    (void)hipGraphDestroy(g);
+    if (ie != hipSuccess) {
+        c.err = std::string("step graph instantiation: ") + hipGetErrorString(ie);
+        return YCN_ERR_HIP;
+    }
+    return YCN_OK;
+}
+
+// the executable of a schedule and field parity, captured at first use (the cache keeps
+// the most recently used last); called only when no step graph is in flight: a segment
+// fetches both of its executables before it launches a step
+static int eo_step_graph(Ctx &c, int parity, const std::vector<int> &chunks, hipGraphExec_t *out) {
+    for (size_t i = 0; i < c.stepGraphs.size(); ++i) {
+        const Ctx::StepGraph e = c.stepGraphs[i];
+        if (e.chunks == chunks && e.parity == parity) {
+            c.stepGraphs.erase(c.stepGraphs.begin() + (long)i);
+            c.stepGraphs.push_back(e);
+            *out = e.exec;
+            return YCN_OK;
+        }
+    }
+    if (c.stepGraphs.size() >= kMaxStepGraphs) {
+        (void)hipGraphExecDestroy(c.stepGraphs.front().exec);
+        c.stepGraphs.erase(c.stepGraphs.begin());
+    }
+    const int rc = eo_capture_step(c, parity, chunks, out);
+    if (rc) return rc;
+    c.stepGraphs.push_back({chunks, parity, *out});
+    return YCN_OK;
+}
+
+// One captured step on the context stream, no host wait: k_step_begin, the coloured
+// noise (on the noise stream beside the solve when the context has it, as a single step
+// does: a graph with a second branch measured slower), the captured solve and drift, then
+// the update once the noise is there.  The noise reads the copy of the halt flag and
+// counter k_step_begin made, so the solve's check kernel cannot change them under it.
+static int eo_enqueue_step(Ctx &c, int parity, hipGraphExec_t ex) {
+    const bool fixed = c.p.adaptive == 0;
+    double *phiIn = c.phi[parity], *phiOut = fixed ? c.phi[parity ^ 1] : c.phi[parity];
+    StepCtl *ctlNoise = c.ctl_d + 1;
+    const bool side = c.noiseSide && c.noiseStream;
+    hipLaunchKernelGGL(k_step_begin, dim3(1), dim3(kWave), 0, c.stream, (const StepCtl *)c.ctl_d, ctlNoise, c.active,
+                       c.nSys);
+    int rc;
+    if (side) {
+        YCN_HIP(hipEventRecord(c.evMain, c.stream));
+        YCN_HIP(hipStreamWaitEvent(c.noiseStream, c.evMain, 0));
+        rc = noise_generate_ctl(c, c.noiseStream, ctlNoise, c.noise);   // cn()
+        if (rc) return rc;
+        YCN_HIP(hipEventRecord(c.evNoise, c.noiseStream));
+    } else {
+        rc = noise_generate_ctl(c, c.stream, ctlNoise, c.noise);   // cn()
+        if (rc) return rc;
+    }
+    YCN_HIP(hipGraphLaunch(ex, c.stream));   // fDrift.getForce(drift)
+    if (side) YCN_HIP(hipStreamWaitEvent(c.stream, c.evNoise, 0));
+    return bosonic_update_gated(c, fixed ? 1 : 0, phiIn, phiOut, &c.ctl_d->halt);   // kli.Run(kAll)
+}
+
+static void add_step_stats(SolveStats *tot, const SolveStats &st) {
+    tot->iters_max += st.iters_max;
+    tot->restarts += st.restarts;
+    tot->converged &= st.converged;
+    tot->rr_max = std::max(tot->rr_max, st.rr_max);
+    tot->true_rel_max = std::max(tot->true_rel_max, st.true_rel_max);
+    tot->iters_mean += st.iters_mean;
+}
+
+int eo_graph_run(Ctx &c, uint64_t step0, int nsteps, SolveStats *tot) {
+    if (!c.ctl_d) {
+        YCN_HIP(hipMalloc(&c.ctl_d, 2 * sizeof(StepCtl)));   // the control block and the noise's copy
+        YCN_HIP(hipHostMalloc((void **)&c.ctl_h, sizeof(StepCtl), hipHostMallocDefault));
+        YCN_HIP(hipMalloc(&c.phiSolve, sizeof(double) * (size_t)c.nRep * c.V));
+    }
+    const bool fixed = c.p.adaptive == 0;
+    const int nsys = c.nSys;
+    StepCtl &h = *c.ctl_h;
+    h = StepCtl{};
+    h.converged = 1;
+    h.step = step0;
+    h.stopScale = c.eoStopScale;
+    h.lastIters = c.eoPredIters;
+    h.nRounds = std::min((int)c.eoRounds.size(), kMaxRounds);
+    for (int i = 0; i < h.nRounds; ++i) h.rounds[i] = c.eoRounds[i];
+    YCN_HIP(hipMemcpyAsync(c.ctl_d, c.ctl_h, sizeof(StepCtl), hipMemcpyHostToDevice, c.stream));
+    SolveStats host;   // figures of the steps the host finished
+    int done = 0, rc;
+    while (done < nsteps) {
+        const int seg = std::min(kStepSegment, nsteps - done);
+        const std::vector<int> chunks = eo_schedule(c, c.stepTail);
+        const int p0 = c.cur;
+        hipGraphExec_t ex[2] = {nullptr, nullptr};
+        if ((rc = eo_step_graph(c, p0, chunks, &ex[p0]))) return rc;
+        if (fixed && seg > 1 && (rc = eo_step_graph(c, p0 ^ 1, chunks, &ex[p0 ^ 1]))) return rc;
+        for (int i = 0; i < seg; ++i) {
+            const int parity = fixed ? p0 ^ (i & 1) : p0;
+            if ((rc = eo_enqueue_step(c, parity, ex[parity]))) return rc;
+        }
+        YCN_HIP(hipMemcpyAsync(c.ctl_h, c.ctl_d, sizeof(StepCtl), hipMemcpyDeviceToHost, c.stream));
+        if ((rc = spin_sync(c))) return rc;
+        if (!h.halt) {
+            c.cur = fixed ? p0 ^ (seg & 1) : p0;
+            done += seg;
+            if (++c.stepClean >= 4 && c.stepTail > 1) {
+                --c.stepTail;
+                c.stepClean = 0;
+            }
+        } else {
+            // the halted step's solve is still in the context's state (the later steps'
+            // chunk kernels may have iterated it further): finish it, then its drift and
+            // update, eagerly; the field it started from is the one that step read
+            const uint64_t first = step0 + (uint64_t)done;
+            if (h.haltStep < first || h.haltStep >= first + (uint64_t)seg) {
+                c.err = "captured steps: the control block names a step outside the segment";
+                return YCN_ERR_HIP;
+            }
+            const int k = (int)(h.haltStep - first);
+            c.cur = fixed ? p0 ^ (k & 1) : p0;
+            SolveStats st;
+            if ((rc = eo_mixed_loop(c, c.phi[c.cur], nsys, c.eoPredIters, &st))) return rc;
+            if ((rc = eo_drift(c, c.phi[c.cur], c.drift))) return rc;
+            if ((rc = bosonic_update(c, fixed ? 1 : 0))) return rc;
+            add_step_stats(&host, st);
+            ++c.haltedSteps;
+            c.stepTail = std::min(c.stepTail + 1, kMaxTail);
+            c.stepClean = 0;
+            h.halt = 0;
+            h.step = h.haltStep + 1;
+            h.stopScale = c.eoStopScale;
+            h.lastIters = st.iters_max;
+            h.nRounds = (int)c.eoRounds.size();
+            for (int i = 0; i < h.nRounds; ++i) h.rounds[i] = c.eoRounds[i];
+            YCN_HIP(hipMemcpyAsync(c.ctl_d, c.ctl_h, sizeof(StepCtl), hipMemcpyHostToDevice, c.stream));
+            done += k + 1;
+        }
+        // the next segment's schedule from the last finished step
+        if (h.lastIters > 0) {
+            c.eoPredIters = h.lastIters;
+            c.eoRounds.assign(h.rounds, h.rounds + std::min(h.nRounds, kMaxRounds));
+        }
+        c.eoStopScale = h.stopScale;
+    }
+    c.lastIters = std::max(kGraphIters, c.eoPredIters);
+    // the captured steps' figures (the last read of the block holds them all)
+    SolveStats dev;
+    dev.iters_max = (int)h.itersMaxSum;
+    dev.iters_mean = h.itersMeanSum;
+    dev.restarts = h.restarts;
+    dev.converged = h.converged;
+    dev.rr_max = h.rrMax;
+    dev.true_rel_max = h.trueRelMax;
+    add_step_stats(tot, dev);
+    add_step_stats(tot, host);
     return YCN_OK;
 }
 

@@ -102,8 +102,12 @@ __device__ double2 *lds_fft(double2 *a, double2 *b, int lines, const FftPlan &P,
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kFftThreads)
 k_noise_rows_fwd(const double *__restrict__ xi, const uint64_t *__restrict__ seeds, uint64_t step,
-                 double2 *__restrict__ spec, int Nt, int Nx, int H, int lpb, FftPlan P) {
+                 double2 *__restrict__ spec, int Nt, int Nx, int H, int lpb, FftPlan P, const StepCtl *ctl) {
     extern __shared__ double2 lds[];
+    if (ctl) {   // a graph-captured step: counter from the control block, nothing while halted
+        if (ctl->halt) return;
+        step = ctl->step;
+    }
     double2 *a = lds, *b = lds + lpb * Nx;
     const int rep = blockIdx.y;
     const int V = Nt * Nx;
@@ -154,8 +158,9 @@ k_noise_rows_fwd(const double *__restrict__ xi, const uint64_t *__restrict__ see
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kFftThreads)
 k_noise_cols(double2 *__restrict__ spec, const int *__restrict__ kmax, int kmaxAll, double invV, int Nt, int H,
-             int lpb, FftPlan P) {
+             int lpb, FftPlan P, const StepCtl *ctl) {
     extern __shared__ double2 lds[];
+    if (ctl && ctl->halt) return;
     double2 *a = lds, *b = lds + lpb * Nt;
     const int rep = blockIdx.y;
     const int k0 = blockIdx.x * lpb;
@@ -194,8 +199,9 @@ k_noise_cols(double2 *__restrict__ spec, const int *__restrict__ kmax, int kmaxA
 // ---------------------------------------------------------------------------
 __global__ void __launch_bounds__(kFftThreads)
 k_noise_rows_inv(const double2 *__restrict__ spec, double *__restrict__ noise, int Nt, int Nx, int H, int lpb,
-                 FftPlan P) {
+                 FftPlan P, const StepCtl *ctl) {
     extern __shared__ double2 lds[];
+    if (ctl && ctl->halt) return;
     double2 *a = lds, *b = lds + lpb * Nx;
     const int rep = blockIdx.y;
     const int pair0 = blockIdx.x * lpb;
@@ -349,26 +355,33 @@ int noise_generate(Ctx &c, uint64_t step, const double *xi_dev, double *out_dev)
 // column pass for a context's kernel timing
 static int launch_noise(hipStream_t s, int Nt, int Nx, int nRep, const double *xi_dev, const uint64_t *seeds,
                         uint64_t step, double2 *spec, const int *kmax, int kmaxAll, const FftPlan &planX,
-                        const FftPlan &planT, double *out_dev, Ctx *prof) {
+                        const FftPlan &planT, double *out_dev, Ctx *prof, const StepCtl *ctl = nullptr) {
     const int H = Nx / 2 + 1;
     const int lx = lines_per_block(Nx), lt = lines_per_block(Nt);
     const int npairs = (Nt + 1) / 2;
     dim3 g1((npairs + lx - 1) / lx, nRep), g2((H + lt - 1) / lt, nRep);
     const size_t sh1 = (size_t)2 * lx * Nx * sizeof(double2), sh2 = (size_t)2 * lt * Nt * sizeof(double2);
     hipLaunchKernelGGL(k_noise_rows_fwd, g1, dim3(kFftThreads), sh1, s, xi_dev, seeds, step, spec, Nt, Nx, H, lx,
-                       planX);
+                       planX, ctl);
     if (prof) prof_before(*prof, PK_NOISE_COLS, 32.0 * (double)nRep * Nt * H);
     hipLaunchKernelGGL(k_noise_cols, g2, dim3(kFftThreads), sh2, s, spec, kmax, kmaxAll,
-                       1.0 / ((double)Nt * (double)Nx), Nt, H, lt, planT);
+                       1.0 / ((double)Nt * (double)Nx), Nt, H, lt, planT, ctl);
     if (prof) prof_after(*prof, PK_NOISE_COLS);
     hipLaunchKernelGGL(k_noise_rows_inv, g1, dim3(kFftThreads), sh1, s, (const double2 *)spec, out_dev, Nt, Nx, H,
-                       lx, planX);
+                       lx, planX, ctl);
     return hipGetLastError() == hipSuccess ? YCN_OK : YCN_ERR_HIP;
 }
 
 int noise_generate_on(Ctx &c, hipStream_t s, uint64_t step, const double *xi_dev, double *out_dev) {
     const int rc = launch_noise(s, c.p.Nt, c.p.Nx, c.nRep, xi_dev, (const uint64_t *)c.seed_d, step, c.spec,
                                 (const int *)c.kmax, c.kmaxAll, c.planX, c.planT, out_dev, &c);
+    if (rc) c.err = "coloured-noise kernel launch failed";
+    return rc;
+}
+
+int noise_generate_ctl(Ctx &c, hipStream_t s, const StepCtl *ctl, double *out_dev) {
+    const int rc = launch_noise(s, c.p.Nt, c.p.Nx, c.nRep, nullptr, (const uint64_t *)c.seed_d, 0, c.spec,
+                                (const int *)c.kmax, c.kmaxAll, c.planX, c.planT, out_dev, nullptr, ctl);
     if (rc) c.err = "coloured-noise kernel launch failed";
     return rc;
 }

@@ -26,8 +26,9 @@ __device__ __forceinline__ double bosonic_drift(const double *__restrict__ ph, i
 // adaptive path, pass 1: total drift, one partial max |drift| per block
 __global__ void __launch_bounds__(kUT)
 k_bosonic_max(const double *__restrict__ phi, double *__restrict__ drift, const double *__restrict__ m2, double lam6,
-              int Nt, int Nx, double *__restrict__ part) {
+              int Nt, int Nx, double *__restrict__ part, const int *__restrict__ halt) {
     __shared__ double lds[kUT / kWave];
+    if (halt && *halt) return;
     const int rep = blockIdx.y;
     const int V = Nt * Nx;
     const int s = blockIdx.x * kUT + threadIdx.x;
@@ -45,8 +46,10 @@ k_bosonic_max(const double *__restrict__ phi, double *__restrict__ drift, const 
 // adaptive path, finalize (one block per replica): clamp max |drift| to [1e-5, 1e5]
 // (langevin_gpu_v2.cu:83-91), eps = epsBar * 30 / max (:112), Langevin time
 __global__ void __launch_bounds__(kUT)
-k_fin_eps(const double *__restrict__ part, int nb, double epsBar, int adaptive, StepState *__restrict__ st) {
+k_fin_eps(const double *__restrict__ part, int nb, double epsBar, int adaptive, StepState *__restrict__ st,
+          const int *__restrict__ halt) {
     __shared__ double lds[kUT / kWave];
+    if (halt && *halt) return;
     const int rep = blockIdx.x;
     double m = 0.0;
     for (int i = threadIdx.x; i < nb; i += kUT) m = fmax(m, part[(size_t)rep * nb + i]);
@@ -64,7 +67,8 @@ k_fin_eps(const double *__restrict__ part, int nb, double epsBar, int adaptive, 
 // adaptive path, pass 2: phi -= eps drift + sqrt(2 eps) noise (Evolve, :29-37)
 __global__ void __launch_bounds__(kUT)
 k_evolve(double *__restrict__ phi, const double *__restrict__ drift, const double *__restrict__ noise, int V,
-         const StepState *__restrict__ st) {
+         const StepState *__restrict__ st, const int *__restrict__ halt) {
+    if (halt && *halt) return;
     const int rep = blockIdx.y;
     const int s = blockIdx.x * kUT + threadIdx.x;
     if (s >= V) return;
@@ -78,7 +82,8 @@ k_evolve(double *__restrict__ phi, const double *__restrict__ drift, const doubl
 __global__ void __launch_bounds__(kUT)
 k_update_fixed(const double *__restrict__ phi, double *__restrict__ phin, const double *__restrict__ drift,
                const double *__restrict__ noise, const double *__restrict__ m2, double lam6, int Nt, int Nx,
-               double eps, StepState *__restrict__ st) {
+               double eps, StepState *__restrict__ st, const int *__restrict__ halt) {
+    if (halt && *halt) return;
     const int rep = blockIdx.y;
     const int V = Nt * Nx;
     const int s = blockIdx.x * kUT + threadIdx.x;
@@ -171,32 +176,38 @@ int field_to_f32(Ctx &c, const double *a, float *b) {
     return YCN_OK;
 }
 
-int bosonic_update(Ctx &c, int fixedEps) {
+int bosonic_update_gated(Ctx &c, int fixedEps, double *phi_in, double *phi_out, const int *halt) {
     const int Nt = c.p.Nt, Nx = c.p.Nx, V = c.V;
     dim3 g((V + kUT - 1) / kUT, c.nRep);
     const double lam6 = c.p.lambda / 6.0;
     if (fixedEps) {
         prof_before(c, PK_BOSONIC, (double)c.nRep * V * 32);
-        hipLaunchKernelGGL(k_update_fixed, g, dim3(kUT), 0, c.stream, (const double *)c.phi[c.cur], c.phi[c.cur ^ 1],
+        hipLaunchKernelGGL(k_update_fixed, g, dim3(kUT), 0, c.stream, (const double *)phi_in, phi_out,
                            (const double *)c.drift, (const double *)c.noise, (const double *)c.m2_d, lam6, Nt, Nx,
-                           c.epsBar, c.step_d);
+                           c.epsBar, c.step_d, halt);
         prof_after(c, PK_BOSONIC);
         YCN_HIP(hipGetLastError());
-        c.cur ^= 1;
         return YCN_OK;
     }
     // partials: the CG partial buffer holds >= nRep * blocks doubles
     double *part = reinterpret_cast<double *>(c.partials);
     const int nb = (int)g.x;
     prof_before(c, PK_BOSONIC, (double)c.nRep * V * 24);
-    hipLaunchKernelGGL(k_bosonic_max, g, dim3(kUT), 0, c.stream, (const double *)c.phi[c.cur], c.drift,
-                       (const double *)c.m2_d, lam6, Nt, Nx, part);
+    hipLaunchKernelGGL(k_bosonic_max, g, dim3(kUT), 0, c.stream, (const double *)phi_in, c.drift,
+                       (const double *)c.m2_d, lam6, Nt, Nx, part, halt);
     prof_after(c, PK_BOSONIC);
     hipLaunchKernelGGL(k_fin_eps, dim3(c.nRep), dim3(kUT), 0, c.stream, (const double *)part, nb, c.epsBar,
-                       c.p.adaptive, c.step_d);
-    hipLaunchKernelGGL(k_evolve, g, dim3(kUT), 0, c.stream, c.phi[c.cur], (const double *)c.drift,
-                       (const double *)c.noise, V, (const StepState *)c.step_d);
+                       c.p.adaptive, c.step_d, halt);
+    hipLaunchKernelGGL(k_evolve, g, dim3(kUT), 0, c.stream, phi_in, (const double *)c.drift,
+                       (const double *)c.noise, V, (const StepState *)c.step_d, halt);
     YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+
+int bosonic_update(Ctx &c, int fixedEps) {
+    const int rc = bosonic_update_gated(c, fixedEps, c.phi[c.cur], c.phi[c.cur ^ 1], nullptr);
+    if (rc) return rc;
+    if (fixedEps) c.cur ^= 1;   // the fixed-eps update writes the other buffer of the pair
     return YCN_OK;
 }
 

@@ -253,6 +253,9 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         c.useGraphs = std::atoi(gr) != 0;
     else if (std::getenv("ROCP_TOOL_LIBRARIES"))
         c.useGraphs = false;
+    // YCN_STEP_GRAPHS=0 keeps ycn_langevin_run on single eagerly launched steps (with
+    // captured CG chunks) where it would replay whole captured steps; results are identical
+    if (const char *sg = std::getenv("YCN_STEP_GRAPHS")) c.stepGraphsOn = std::atoi(sg) != 0;
     if ((rc = cg_autotune(c))) return bail(rc);
     if (c.p.evenOdd) {
         if (c.p.precond == YCN_PRECOND_CNN) {
@@ -501,14 +504,18 @@ int ycn_langevin_step(ycn_ctx *ctx, uint64_t step, double *eps, ycn_cg_stats *st
     return st.converged ? YCN_OK : YCN_ERR_NOT_CONVERGED;
 }
 
+// With the even-odd mixed solver (and graphs on, no kernel profiled) the steps after the
+// context's first solve replay captured whole steps: no host wait inside a step, one per
+// segment of steps (eo_graph_run); every other configuration runs single steps.
 int ycn_langevin_run(ycn_ctx *ctx, uint64_t step0, int nsteps, ycn_cg_stats *stats) {
     CHECK_CTX(ctx);
     if (nsteps < 0) return YCN_ERR_INVALID;
     Ctx &c = *ctx;
     SolveStats tot;
     tot.systems = c.nSys;
-    double meanSum = 0;
-    for (int i = 0; i < nsteps; ++i) {
+    tot.iters_mean = 0;   // summed over the steps, divided at the end
+    int i = 0;
+    for (; i < nsteps && !eo_graph_eligible(c); ++i) {
         SolveStats st;
         int rc = step_impl(c, step0 + (uint64_t)i, &st);
         if (rc) return rc;
@@ -517,12 +524,24 @@ int ycn_langevin_run(ycn_ctx *ctx, uint64_t step0, int nsteps, ycn_cg_stats *sta
         tot.converged &= st.converged;
         tot.rr_max = std::max(tot.rr_max, st.rr_max);
         tot.true_rel_max = std::max(tot.true_rel_max, st.true_rel_max);
-        meanSum += st.iters_mean;
+        tot.iters_mean += st.iters_mean;
     }
-    tot.iters_mean = nsteps ? meanSum / nsteps : 0;
-    YCN_HIP(hipStreamSynchronize(c.stream));
+    if (i < nsteps) {
+        const int rc = eo_graph_run(c, step0 + (uint64_t)i, nsteps - i, &tot);
+        if (rc) return rc;
+    } else {
+        YCN_HIP(hipStreamSynchronize(c.stream));
+    }
+    tot.iters_mean = nsteps ? tot.iters_mean / nsteps : 0;
     fill_stats(tot, stats);
     return tot.converged ? YCN_OK : YCN_ERR_NOT_CONVERGED;
+}
+
+int ycn_host_syncs(ycn_ctx *ctx, int64_t *syncs, int64_t *halted_steps) {
+    CHECK_CTX(ctx);
+    if (syncs) *syncs = ctx->hostSyncs;
+    if (halted_steps) *halted_steps = ctx->haltedSteps;
+    return YCN_OK;
 }
 
 int ycn_measure(ycn_ctx *ctx, int replica, uint64_t step, ycn_observables *obs, double *slices, double *corr) {

@@ -50,7 +50,10 @@ struct CgState {
     double stop0;      // even-odd solve: the unscaled starting threshold tol |r_0|
     double need;       // even-odd solve: threshold the true residual asked for at the last check at the stop
     double rrRel;      // FP32 loops: recursive |r|^2 when the recursion was last anchored (start, reliable update)
+    int nRu;           // pipelined even-odd solve: reliable updates so far
+    int ruIter[7];     // pipelined even-odd solve: iteration count at each of the first seven
 };
+constexpr int kMaxRounds = 7;
 
 // FP32 CG loops pause for an FP64 reliable update once the recursive |r| has fallen by
 // this factor since the recursion was last anchored to the true residual (Clark et al.,
@@ -66,6 +69,29 @@ struct StepState {
     double maxDrift;   // last (clamped) max |drift|
     int clamps;        // clamp events
     int pad;
+};
+
+// Control block of graph-captured Langevin steps (device resident).  Every kernel of a
+// captured step outside the CG iteration chunks returns at once while `halt` is set; the
+// step's check kernel sets it when the step's solve did not reach its stopping rule within
+// the captured chunk schedule, so that the later steps of the run leave the state as it
+// is (their chunk kernels keep iterating the unfinished solve) and the host finishes that
+// step when it reads the block at the end of the segment of steps.
+struct StepCtl {
+    int halt;             // a step's solve did not finish within its schedule
+    int converged;        // every finished step's solves met the stopping rule
+    uint64_t haltStep;    // step counter of that step
+    uint64_t step;        // step counter of the next step (noise and eta counters)
+    double itersMaxSum;   // sum over finished steps of the largest iteration count of a solve
+    double itersMeanSum;  // sum over finished steps of the mean iteration count
+    double trueRelMax;    // largest true relative residual of the finished steps
+    double rrMax;         // largest final recursive |r|^2 of the finished steps
+    double stopScale;     // starting threshold of the next solve / tol |r_0| (relative mode)
+    int steps;            // finished steps
+    int lastIters;        // largest iteration count of the last finished step
+    int restarts;         // reliable updates of the finished steps (largest over systems, summed)
+    int nRounds;          // reliable updates (at most kMaxRounds) of that step's longest solve
+    int rounds[7];        // and the iteration count at each
 };
 
 enum ProfKernel { PK_NONE = -1, PK_NORMAL_OP = 0, PK_CG_UPDATE, PK_DIRAC, PK_NOISE_COLS, PK_BOSONIC, PK_CNN, PK_COUNT };
@@ -108,6 +134,7 @@ struct Ctx {
     CgState *cg_h = nullptr;              // pinned host mirror of the CG states
     hipEvent_t pollEv = nullptr;          // spin-waited by the solvers' polls
     int eoPredIters = 0;                  // iterations of the last even-odd mixed solve (chunk predictor)
+    std::vector<int> eoRounds;            // iteration counts at that solve's reliable updates (its longest system)
     double eoStopScale = 1.0;             // starting threshold of even-odd solves / tol |r_0| (relative mode)
     int maxBlocks = 0;
     int lastIters = 32;                   // predictor for the CG launch chunk
@@ -140,6 +167,21 @@ struct Ctx {
     // statistics of the last ycn_precond_train call
     double trainLossFirst = 0, trainLossLast = 0;
     int trainPairs = 0;
+
+    // graph-captured Langevin steps (even-odd mixed precision): control block, its pinned
+    // mirror, and one executable per (chunk schedule, field parity)
+    StepCtl *ctl_d = nullptr, *ctl_h = nullptr;
+    double *phiSolve = nullptr;           // the field of the captured step's solve (FP64)
+    struct StepGraph {
+        std::vector<int> chunks;   // launches of each chunk of the schedule
+        int parity;                // field buffer the step reads
+        hipGraphExec_t exec;
+    };
+    std::vector<StepGraph> stepGraphs;
+    bool stepGraphsOn = true;             // YCN_STEP_GRAPHS=0: every step launches eagerly
+    int stepTail = 1, stepClean = 0;      // tail rounds of the schedule; clean segments since a halt
+    int64_t hostSyncs = 0;                // host waits on the context's work since creation
+    int64_t haltedSteps = 0;              // captured steps finished by the host
 
     // captured CG iteration chunks: key -> executable graph
     struct GraphEntry {
@@ -185,6 +227,8 @@ void noise_free(Ctx &c);
 // xi == nullptr: Philox field of (seeds, stream 0, step); else filter xi (device, nRep*V)
 int noise_generate(Ctx &c, uint64_t step, const double *xi_dev, double *out_dev);
 int noise_generate_on(Ctx &c, hipStream_t s, uint64_t step, const double *xi_dev, double *out_dev);
+// the noise of a graph-captured step: counter and halt flag from the control block
+int noise_generate_ctl(Ctx &c, hipStream_t s, const StepCtl *ctl, double *out_dev);
 // the allocation-free entry point's scratch: address lookup on device dev (current),
 // and the message of the last failed call on this thread (nullptr if none)
 int noise_noalloc_setup(int dev);
@@ -230,12 +274,15 @@ int train_preconditioner(Ctx &c, uint64_t step0, int nsteps, int pairsPerSolve, 
 
 // update.hip
 int bosonic_update(Ctx &c, int fixedEps);
+// the update of a graph-captured step: phi_in -> phi_out (fixed eps; adaptive: in place
+// on phi_in), no-op while *halt is set; leaves c.cur alone
+int bosonic_update_gated(Ctx &c, int fixedEps, double *phi_in, double *phi_out, const int *halt);
 int field_slices(Ctx &c, int rep, double *slices_dev, double *sums_dev);
 int hot_start(Ctx &c, uint64_t initSeed);
 int set_int(Ctx &c, int *ptr, int v);
 int poll_active(Ctx &c, int &active);
 int poll_iters(Ctx &c, int nsys, int &active, int &itMax);
-int spin_sync(Ctx &c);
+int spin_sync(Ctx &c);   // waits for the context stream (counted in c.hostSyncs)
 int stats_from_mirror(Ctx &c, int nsys, SolveStats *st);
 int read_stats(Ctx &c, int nsys, SolveStats *st);
 // CG building blocks shared with the even-odd solver (cg.hip): finalize alpha / beta from
@@ -247,14 +294,24 @@ void launch_fin_beta(Ctx &c, int nsys, int nb, bool mixed, bool even);
 int launch_cg_update(Ctx &c, int nsys, int n, const double2 *p);
 int launch_mp_update(Ctx &c, int nsys, int n, const float2 *p);
 int launch_mp_accum(Ctx &c, int nsys, int n);
-int to_f32_n(Ctx &c, const double *a, float *b, int64_t n);
+// b = (float)a over n entries; halt: return at once while *halt is set; copy: also
+// copy a there (FP64)
+int to_f32_n(Ctx &c, const double *a, float *b, int64_t n, const int *halt = nullptr, double *copy = nullptr);
 
 // even-odd preconditioned fermion force (evenodd.hip)
 bool eo_available(const Ctx &c);
 int eo_pf_alloc(Ctx &c);
 int eo_autotune(Ctx &c);
 int eo_solve(Ctx &c, const double *phi, int nsys, bool gen, uint32_t streamBase, uint64_t step, SolveStats *st);
-int eo_drift(Ctx &c, const double *phi, double *drift);
+// ctl (graph-captured steps, else null): the drift kernel returns at once while halted
+int eo_drift(Ctx &c, const double *phi, double *drift, const StepCtl *ctl = nullptr);
+// Langevin steps as graph-captured steps (even-odd mixed precision): eligible when the
+// context runs that solver with graphs on and no kernel profiled.  eo_graph_run does
+// nsteps steps from counter step0 with one host wait per segment of captured steps and
+// adds each step's figures to *tot the way a run of single steps sums them (iters_max,
+// restarts and iters_mean summed, converged and-ed, rr_max and true_rel_max maxed).
+bool eo_graph_eligible(const Ctx &c);
+int eo_graph_run(Ctx &c, uint64_t step0, int nsteps, SolveStats *tot);
 int eo_propagator(Ctx &c, const double *phi, double2 *out, SolveStats *st);
 
 // precision conversion
