@@ -69,7 +69,8 @@ typedef struct {
     double cgTol;          /* default 1e-8 (params.h:14)                                */
     int cgTolMode;         /* YCN_TOL_*; default absolute recursive (CGsolver.cu:107)   */
     int cgMaxIter;         /* default 1000 (params.h:15)                                */
-    int cgPrecision;       /* YCN_CG_*                                                  */
+    int cgPrecision;       /* YCN_CG_* (mixed: any lattice; the fused FP32 tiles where  */
+                           /* one divides it, an FP32 two-stencil pair elsewhere)       */
     int reliableEvery;     /* mixed precision: FP64 reliable update period (50)         */
     int precond;           /* YCN_PRECOND_*                                             */
     int device;            /* HIP device ordinal                                        */

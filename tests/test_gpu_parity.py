@@ -184,6 +184,31 @@ def test_sweep_configuration(gpu, Nt):
             assert abs(r.eps[0] - eps) <= 1e-12 * eps, step
 
 
+@pytest.mark.parametrize("Nt,Nx", [(3, 32), (7, 32), (6, 32), (22, 32), (10, 12), (13, 18)])
+def test_mixed_precision_on_untiled_lattices(gpu, Nt, Nx):
+    """Mixed-precision CG on lattices no fused FP32 tile divides (odd Nt of the runs.py
+    sweep included) runs the FP32 two-stencil pair: the converged drift matches the
+    oracle's FP64 drift to the solver's accuracy, every solve reaches the FP64 true
+    residual 1e-10, and 20 Langevin steps stay with an FP64 context's to 1e-9."""
+    Y = gpu
+    p, orc = mk(Y, Nt=Nt, Nx=Nx, epsBar=0.01)
+    phi0 = O.hot_start(11, Nt, Nx)
+    ref, _ = O.fermion_drift(orc, phi0, 4)
+    fields = {}
+    for prec in (Y.YCN_CG_MIXED, Y.YCN_CG_FP64):
+        p.cgPrecision = prec
+        with Y.Context(p) as c:
+            c.set_field(phi0)
+            if prec == Y.YCN_CG_MIXED:
+                drift, st = c.fermion_force(4)
+                assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+                assert rel(drift, ref) < 1e-8
+            st = c.langevin_run(0, 20)
+            assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+            fields[prec] = c.get_field()
+    assert rel(fields[Y.YCN_CG_MIXED], fields[Y.YCN_CG_FP64]) < 1e-9
+
+
 @pytest.mark.parametrize("adaptive", [0, 1])
 def test_langevin_step_converged(gpu, adaptive):
     """Solves to true residual 1e-10: the drift is fixed to the solver accuracy, so the

@@ -372,7 +372,71 @@ k_cg_a(LatticeArgs L, const double *__restrict__ phi, const CgState *__restrict_
     for (int c = 0; c < 4; ++c) t[off + c * V + s] = o[c];
 }
 
-// cg_b: Ap = D2 t; partial <p', Ap>
+// FP32 two-stencil pair for lattices that no fused FP32 tile divides (mixed precision on
+// any lattice).  cg_a32: p' = r + beta p_old in FP32 (written at the own site; the hop
+// reads the same FP32 values at the neighbours), t = D1 p'; cg_b32: Ap = D2 t, partial
+// <p', Ap>.  The hop arithmetic runs in FP64 on the FP32 vectors and FP32 field.
+template <bool DAG1>
+__global__ void __launch_bounds__(kNT)
+k_cg_a32(LatticeArgs L, const float *__restrict__ phi, const CgState *__restrict__ st, const float2 *__restrict__ r,
+         const float2 *__restrict__ pold, float2 *__restrict__ pnew, float2 *__restrict__ t) {
+    const int sys = blockIdx.y;
+    if (st[sys].done) return;
+    const int s = blockIdx.x * kNT + threadIdx.x;
+    if (s >= L.V) return;
+    const float beta = (float)st[sys].beta;
+    const size_t V = L.V, off = (size_t)sys * 4 * V;
+    const float2 *rr = r + off, *po = pold + off;
+    auto pf = [&](int i, int c) {
+        const float2 a = rr[c * V + i], b = po[c * V + i];
+        return make_float2(a.x + beta * b.x, a.y + beta * b.y);
+    };
+    auto ld = [&](int i, int c) {
+        const float2 v = pf(i, c);
+        return make_double2((double)v.x, (double)v.y);
+    };
+#pragma unroll
+    for (int c = 0; c < 4; ++c) pnew[off + c * V + s] = pf(s, c);
+    const WilsonCoef w = make_coef(L.r, L.M, L.g);
+    double2 o[4];
+    wilson_site<DAG1>(L, w, (double)phi[(size_t)(sys / L.nNoise) * V + s], s, ld, o);
+#pragma unroll
+    for (int c = 0; c < 4; ++c) t[off + c * V + s] = make_float2((float)o[c].x, (float)o[c].y);
+}
+
+template <bool DAG2>
+__global__ void __launch_bounds__(kNT)
+k_cg_b32(LatticeArgs L, const float *__restrict__ phi, const CgState *__restrict__ st, const float2 *__restrict__ t,
+         const float2 *__restrict__ p, float2 *__restrict__ Ap, double2 *__restrict__ part) {
+    __shared__ double2 lds[kNT / kWave];
+    const int sys = blockIdx.y;
+    if (st[sys].done) return;
+    const int s = blockIdx.x * kNT + threadIdx.x;
+    const size_t V = L.V, off = (size_t)sys * 4 * V;
+    double2 acc = make_double2(0.0, 0.0);
+    if (s < L.V) {
+        const float2 *tv = t + off;
+        const WilsonCoef w = make_coef(L.r, L.M, L.g);
+        double2 o[4];
+        wilson_site<DAG2>(L, w, (double)phi[(size_t)(sys / L.nNoise) * V + s], s,
+                          [&](int i, int c) {
+                              const float2 v = tv[c * V + i];
+                              return make_double2((double)v.x, (double)v.y);
+                          },
+                          o);
+#pragma unroll
+        for (int c = 0; c < 4; ++c) {
+            const float2 a = make_float2((float)o[c].x, (float)o[c].y);
+            Ap[off + c * V + s] = a;
+            const float2 pv = p[off + c * V + s];
+            acc = cadd(acc, cdotc(make_double2((double)pv.x, (double)pv.y), make_double2((double)a.x, (double)a.y)));
+        }
+    }
+    acc = block_sum2<kNT>(acc, lds);
+    if (threadIdx.x == 0) part[(size_t)sys * gridDim.x + blockIdx.x] = acc;
+}
+
+// cg_b: Ap = D2 t; partial <p', Ap>// cg_b: Ap = D2 t; partial <p', Ap>
 template <bool DAG2>
 __global__ void __launch_bounds__(kNT)
 k_cg_b(LatticeArgs L, const double *__restrict__ phi, const CgState *__restrict__ st, const double2 *__restrict__ t,
@@ -969,7 +1033,7 @@ int cg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t 
         return fcg_solve(c, phi, nsys, mtype, gen, streamBase, step, st);
     // mixed precision where the FP32 tiles fit the lattice (with even-odd preconditioning
     // the fermion force may run mixed on shapes the plain FP32 tiles do not cover)
-    if ((c.p.cgPrecision == YCN_CG_MIXED && c.tile32 >= 0) || c.p.precond == YCN_PRECOND_CNN)
+    if ((c.p.cgPrecision == YCN_CG_MIXED && c.rs) || c.p.precond == YCN_PRECOND_CNN)
         return mp_solve(c, phi, nsys, mtype, gen, streamBase, step, st);
     const LatticeArgs L = largs(c);
     const dim3 g = site_grid(c, nsys);
@@ -1075,11 +1139,35 @@ void mp_free(Ctx &c) {
 static void launch_mp_iteration(Ctx &c, int nsys, int mtype, int parity) {
     const float2 *pold = c.ps[parity];
     float2 *pnew = c.ps[parity ^ 1];
-    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (4 * 32 + 4));
-    launch_fused<float>(c, c.tile32, nsys, mtype != YCN_DAGGER, c.phis, c.cg_d, (const float2 *)c.rs, pold, pnew,
-                        c.Aps, c.partials);
-    prof_after(c, PK_NORMAL_OP);
-    const int nbF = fused_args(c, kTiles32[c.tile32].TT, kTiles32[c.tile32].TX).tilesPerSys;
+    int nbF;
+    if (c.tile32 >= 0) {
+        prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (4 * 32 + 4));
+        launch_fused<float>(c, c.tile32, nsys, mtype != YCN_DAGGER, c.phis, c.cg_d, (const float2 *)c.rs, pold, pnew,
+                            c.Aps, c.partials);
+        prof_after(c, PK_NORMAL_OP);
+        nbF = fused_args(c, kTiles32[c.tile32].TT, kTiles32[c.tile32].TX).tilesPerSys;
+    } else {
+        // no fused FP32 tile divides the lattice: the two-stencil pair, its intermediate
+        // in the FP64 scratch c.t (free during the iteration chunks)
+        const LatticeArgs L = largs(c);
+        const dim3 g = site_grid(c, nsys);
+        float2 *t = reinterpret_cast<float2 *>(c.t);
+        // read r, p_old, phi, write p', t; read t, p', phi, write Ap  (7S + 2P, FP32)
+        prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (7 * 32 + 8));
+        if (mtype == YCN_DAGGER) {
+            hipLaunchKernelGGL(k_cg_a32<false>, g, dim3(kNT), 0, c.stream, L, (const float *)c.phis,
+                               (const CgState *)c.cg_d, (const float2 *)c.rs, pold, pnew, t);
+            hipLaunchKernelGGL(k_cg_b32<true>, g, dim3(kNT), 0, c.stream, L, (const float *)c.phis,
+                               (const CgState *)c.cg_d, (const float2 *)t, (const float2 *)pnew, c.Aps, c.partials);
+        } else {
+            hipLaunchKernelGGL(k_cg_a32<true>, g, dim3(kNT), 0, c.stream, L, (const float *)c.phis,
+                               (const CgState *)c.cg_d, (const float2 *)c.rs, pold, pnew, t);
+            hipLaunchKernelGGL(k_cg_b32<false>, g, dim3(kNT), 0, c.stream, L, (const float *)c.phis,
+                               (const CgState *)c.cg_d, (const float2 *)t, (const float2 *)pnew, c.Aps, c.partials);
+        }
+        prof_after(c, PK_NORMAL_OP);
+        nbF = site_blocks(c);
+    }
     hipLaunchKernelGGL(k_fin_alpha, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)c.partials, nbF, c.cg_d);
     prof_before(c, PK_CG_UPDATE, (double)nsys * c.V * 6 * 32);
     hipLaunchKernelGGL(k_mp_update, site_grid(c, nsys), dim3(kNT), 0, c.stream, c.V, (const CgState *)c.cg_d, c.xs,
@@ -1091,8 +1179,8 @@ static void launch_mp_iteration(Ctx &c, int nsys, int mtype, int parity) {
 
 int mp_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t streamBase, uint64_t step,
              SolveStats *st) {
-    if (c.tile32 < 0) {
-        c.err = "mixed-precision CG needs Nt % 4 == 0 and Nx % 16 == 0 (fused FP32 tiles)";
+    if (!c.rs) {
+        c.err = "mixed-precision CG: the context has no FP32 buffers";
         return YCN_ERR_UNSUPPORTED;
     }
     const LatticeArgs L = largs(c);

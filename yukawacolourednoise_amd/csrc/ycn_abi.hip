@@ -234,11 +234,13 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
         return bail(YCN_ERR_HIP);
     if ((rc = noise_init(c))) return bail(rc);
     if ((rc = cg_alloc(c))) return bail(rc);
-    // FP32 and CNN buffers whenever the lattice admits the FP32 tiles / CNN tiles
+    // FP32 buffers whenever the lattice admits the FP32 tiles or mixed precision is asked
+    // for (lattices no tile divides run the FP32 two-stencil pair); CNN buffers whenever
+    // the lattice admits the CNN tiles
     const bool f32ok = (p->Nt % 4 == 0) && (p->Nx % 16 == 0);
     const bool cnnok = (p->Nt % 16 == 0) && (p->Nx % 16 == 0);
     const bool eoDims = (p->Nt % 2 == 0) && (p->Nx % 2 == 0);
-    if ((f32ok || (p->evenOdd && eoDims)) && (rc = mp_alloc(c))) return bail(rc);
+    if ((f32ok || p->cgPrecision == YCN_CG_MIXED || (p->evenOdd && eoDims)) && (rc = mp_alloc(c))) return bail(rc);
     if (cnnok && (rc = cnn_alloc(c))) return bail(rc);
     if (c.p.precond == YCN_PRECOND_CNN && !cnnok) {
         c.err = "the learned preconditioner needs Nt % 16 == 0 and Nx % 16 == 0";
@@ -267,10 +269,6 @@ int ycn_create(const ycn_params *p, ycn_ctx **out) {
             c.err = "even-odd preconditioning needs Nt even and Nx % 32 == 0, or Nt % 16 == 0 and Nx % 16 == 0";
             return bail(YCN_ERR_UNSUPPORTED);
         }
-    }
-    if (c.p.cgPrecision == YCN_CG_MIXED && c.tile32 < 0 && !(c.p.evenOdd && c.tileEo32 >= 0)) {
-        c.err = "mixed-precision CG needs Nt % 4 == 0 and Nx % 16 == 0";
-        return bail(YCN_ERR_UNSUPPORTED);
     }
     if (hipDeviceSynchronize() != hipSuccess) return bail(YCN_ERR_HIP);
     *out = ctx;
