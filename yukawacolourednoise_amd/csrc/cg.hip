@@ -1288,7 +1288,9 @@ int fcg_solve(Ctx &c, const double *phi, int nsys, int mtype, bool gen, uint32_t
     const int64_t nconv = std::min<int64_t>((int64_t)c.nRep * c.V, (int64_t)nrepUsed * c.V);
     hipLaunchKernelGGL(k_to_f32, dim3((unsigned)((nconv + kNT - 1) / kNT)), dim3(kNT), 0, c.stream, phi, c.phis,
                        nconv, (const int *)nullptr, (double *)nullptr);
-    int rc = set_int(c, c.active, nsys);
+    int rc = cnn_gates(c, (int)std::min<int64_t>(nrepUsed, c.nRep));   // the network's field gates, once per solve
+    if (rc) return rc;
+    rc = set_int(c, c.active, nsys);
     if (rc) return rc;
     const int maxIter = c.p.cgMaxIter;
     if (gen)

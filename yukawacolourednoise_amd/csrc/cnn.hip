@@ -16,10 +16,12 @@
 // G1[9][16], G2[9][16], [tap][in][out], tap = (dt+1)*3 + (dx+1),
 // out(site) = sum W * in(site + (dt, dx)).  The gates run in FP32 on the VALU.
 //
-// Inference (inside flexible CG): k_cnn_fused keeps all three layers of a 16x16 tile
-// (plus its 3-site halo) in LDS as BF16 and fuses the two dot products the flexible
-// CG recurrence needs.  Compulsory HBM traffic per site: read r_s, Ap (32 + 32 B) and
-// the FP32 field (4 B), write z (32 B).
+// Inference (inside flexible CG): the gates depend on the field only, so k_cnn_gates
+// computes both layers' gates once per solve into HBM (FP16, 64 B per site);
+// k_cnn_fused keeps all three layers of a 16x16 tile (plus its 3-site halo) in LDS as
+// BF16, multiplies the hidden layers by the gates it loads, and fuses the two dot
+// products the flexible CG recurrence needs.  Compulsory HBM traffic per site: read r_s,
+// Ap (32 + 32 B) and the gates (64 B), write z (32 B).
 // Training: layer-wise MFMA kernels over channels-last FP32 activations in HBM,
 // forward, data gradient (a forward convolution with mirrored, transposed weights),
 // weight gradient (an MFMA GEMM whose reduction runs over lattice sites), then Adam.
@@ -83,81 +85,98 @@ __global__ void k_make_frags(const float *__restrict__ W, bf16x8 *__restrict__ f
 // ---------------------------------------------------------------------------
 // fused inference: z = s * net(r_s / s) on a 16x16 tile, plus partial <z, r_s>, <z, Ap>
 // ---------------------------------------------------------------------------
-template <int CIN, int COUT, int WOUT, int WIN, int KS, typename Store>
-__device__ __forceinline__ void conv_lds(const __bf16 *__restrict__ in, const bf16x8 *__restrict__ frag, Store store) {
+// One layer on the LDS-resident tile: out[m][n] = sum_k in[site(m) + off(tap)][ci] W[k][n]
+// for the WOUT x WOUT output sites m (input array width WIN, one more site of halo on
+// each side).  The MFMA runs transposed, D[n][m] = sum_k W[k][n] A[m][k]: the weight
+// fragments are its first operand, the activations its second, so a lane ends with four
+// consecutive channels 4 kg .. 4 kg + 3 of one site m = 16 g + col and stores them with
+// one access.  Each lane's tap offsets are fixed for the layer (kg picks them); padded K
+// steps load the centre tap (finite activations against zero weights) instead of
+// branching.  pre(m, kg) runs before a group's MFMAs (its global loads overlap them) and
+// store(m, kg, acc, pre's result) gets the four channels.
+template <int CIN, int WOUT, int WIN, int KS, typename Pre, typename Store>
+__device__ __forceinline__ void conv_lds(const __bf16 *__restrict__ in, const bf16x8 *__restrict__ frag, Pre pre,
+                                         Store store) {
     constexpr int N = WOUT * WOUT;
     constexpr int G = (N + 15) / 16;
     const int lane = threadIdx.x & 63, wave = threadIdx.x >> 6;
     const int kg = lane >> 4, col = lane & 15;
     bf16x8 B[KS];   // this lane's weight fragments, held for all of the wave's site groups
+    int toff[KS];   // LDS offset of this lane's tap and channel block from its site's corner
 #pragma unroll
-    for (int ks = 0; ks < KS; ++ks) B[ks] = frag[ks * 64 + lane];
+    for (int ks = 0; ks < KS; ++ks) {
+        B[ks] = frag[ks * 64 + lane];
+        const int k0 = ks * 32 + kg * 8;
+        const int tap = k0 / CIN, ci0 = k0 % CIN;
+        toff[ks] = tap < 9 ? ((tap / 3) * WIN + tap % 3) * CIN + ci0 : (WIN + 1) * CIN;
+    }
     for (int g = wave; g < G; g += kCT / 64) {
         const int m = g * 16 + col;
-        const bool valid = m < N;
-        const int a = valid ? m / WOUT : 0, b = valid ? m % WOUT : 0;
+        const int mc = m < N ? m : N - 1;
+        const int a = mc / WOUT, b = mc - a * WOUT;
+        const __bf16 *base = in + (a * WIN + b) * CIN;
+        const auto pl = pre(mc, kg);
         f32x4 acc = {0.f, 0.f, 0.f, 0.f};
 #pragma unroll
-        for (int ks = 0; ks < KS; ++ks) {
-            const int k0 = ks * 32 + kg * 8;
-            const int tap = k0 / CIN, ci0 = k0 % CIN;
-            bf16x8 A = bf16x8_zero();
-            if (tap < 9 && valid) {
-                const int dt = tap / 3 - 1, dx = tap % 3 - 1;
-                const int site = (a + 1 + dt) * WIN + (b + 1 + dx);
-                A = *reinterpret_cast<const bf16x8 *>(in + site * CIN + ci0);
-            }
-            acc = mfma_bf16(A, B[ks], acc);
-        }
-#pragma unroll
-        for (int i = 0; i < 4; ++i) {
-            const int mm = g * 16 + kg * 4 + i;
-            if (mm < N && col < COUT) store(mm, col, acc[i]);
-        }
+        for (int ks = 0; ks < KS; ++ks)
+            acc = mfma_bf16(B[ks], *reinterpret_cast<const bf16x8 *>(base + toff[ks]), acc);
+        if (m < N) store(m, kg, acc, pl);
     }
 }
 
-// gates of one hidden layer on a WxW array of the tile: GT[m][n] = 1 + tanh(sum_tap
-// G[tap][n] q(site + off(tap))) for array site m = (a, b), which sits at (a + o, b + o)
-// of the q array Q (width WQ).  One thread per site: its nine q taps in registers, the
-// weights G[tap][16] in LDS, read as broadcast float4s.
-template <int W, int WQ, int o>
-__device__ __forceinline__ void gate_pass(const float *Q, const float4 *G, _Float16 *GT) {
-    for (int m = threadIdx.x; m < W * W; m += kCT) {
-        const int a = m / W, b = m - a * W;
-        float qn[9];
+// The field gates of both hidden layers at every site, once per solve (they depend on
+// the field and the gate weights only): gates[rep][site][32], halves 0-15 the layer-1
+// gates 1 + tanh(sum_tap G1[tap][n] q(site + off(tap))), halves 16-31 the layer-2 ones,
+// q = g phi.  One thread per site, nine q taps in registers, weights read as broadcast
+// float4s; FP32 arithmetic, stored as FP16 (the precision the fused kernel multiplies by).
+__global__ void __launch_bounds__(kCT)
+k_cnn_gates(int Nt, int Nx, const float *__restrict__ phis, float g, const float4 *__restrict__ G,
+            _Float16 *__restrict__ gates) {
+    const int V = Nt * Nx, rep = blockIdx.y;
+    const int s = blockIdx.x * kCT + threadIdx.x;
+    if (s >= V) return;
+    const float *ph = phis + (size_t)rep * V;
+    const int t = s / Nx, x = s - t * Nx;
+    float qn[9];
 #pragma unroll
-        for (int tap = 0; tap < 9; ++tap) qn[tap] = Q[(a + o + tap / 3 - 1) * WQ + (b + o + tap % 3 - 1)];
-        float t[16];
+    for (int tap = 0; tap < 9; ++tap) qn[tap] = g * ph[wrapn(t + tap / 3 - 1, Nt) * Nx + wrapn(x + tap % 3 - 1, Nx)];
+    _Float16 *out = gates + ((size_t)rep * V + s) * 32;
 #pragma unroll
-        for (int n = 0; n < 16; ++n) t[n] = 0.f;
+    for (int layer = 0; layer < 2; ++layer) {
+        const float4 *Gl = G + layer * (kCnnNWG / 4);
+        float tt[16];
+#pragma unroll
+        for (int n = 0; n < 16; ++n) tt[n] = 0.f;
 #pragma unroll
         for (int tap = 0; tap < 9; ++tap)
 #pragma unroll
             for (int n4 = 0; n4 < 4; ++n4) {
-                const float4 w = G[tap * 4 + n4];
-                t[4 * n4] += w.x * qn[tap];
-                t[4 * n4 + 1] += w.y * qn[tap];
-                t[4 * n4 + 2] += w.z * qn[tap];
-                t[4 * n4 + 3] += w.w * qn[tap];
+                const float4 w = Gl[tap * 4 + n4];
+                tt[4 * n4] += w.x * qn[tap];
+                tt[4 * n4 + 1] += w.y * qn[tap];
+                tt[4 * n4 + 2] += w.z * qn[tap];
+                tt[4 * n4 + 3] += w.w * qn[tap];
             }
+        typedef _Float16 h8 __attribute__((ext_vector_type(8)));
 #pragma unroll
-        for (int n = 0; n < 16; ++n) GT[m * 16 + n] = (_Float16)gate_fn(t[n]);
+        for (int h = 0; h < 2; ++h) {
+            h8 v;
+#pragma unroll
+            for (int i = 0; i < 8; ++i) v[i] = (_Float16)gate_fn(tt[8 * h + i]);
+            *reinterpret_cast<h8 *>(out + layer * 16 + 8 * h) = v;
+        }
     }
 }
 
 __global__ void __launch_bounds__(kCT)
-k_cnn_fused(int Nt, int Nx, int tilesX, int nNoise, float c0, float g, const CgState *__restrict__ st,
-            const float2 *__restrict__ rs, const float2 *__restrict__ Ap, const float *__restrict__ phis,
-            const float *__restrict__ gw, float2 *__restrict__ z, const bf16x8 *__restrict__ gf1,
-            const bf16x8 *__restrict__ gf2, const bf16x8 *__restrict__ gf3, double2 *__restrict__ part) {
+k_cnn_fused(int Nt, int Nx, int tilesX, int nNoise, float c0, const CgState *__restrict__ st,
+            const float2 *__restrict__ rs, const float2 *__restrict__ Ap, const _Float16 *__restrict__ gates,
+            float2 *__restrict__ z, const bf16x8 *__restrict__ gf1, const bf16x8 *__restrict__ gf2,
+            const bf16x8 *__restrict__ gf3, double2 *__restrict__ part) {
     constexpr int W0 = 22, W1 = 20, W2 = 18, W3 = 16;
     __shared__ __attribute__((aligned(16))) __bf16 U0[W0 * W0 * 8];
     __shared__ __attribute__((aligned(16))) __bf16 U1[W1 * W1 * 16];
     __shared__ __attribute__((aligned(16))) __bf16 U2[W2 * W2 * 16];
-    __shared__ float Q[W0 * W0];            // q = g phi on the tile and its 3-site halo
-    __shared__ _Float16 GT[W1 * W1 * 16];   // gates of layer 1, then of layer 2
-    __shared__ float4 GW[2 * kCnnNWG / 4];  // gate weights G1, G2
     __shared__ double2 red[kCT / kWave];
     const int sys = blockIdx.y;
     if (st[sys].done) return;
@@ -181,45 +200,94 @@ k_cnn_fused(int Nt, int Nx, int tilesX, int nNoise, float c0, float g, const CgS
             acc.y += (double)(rv.x * av.x + rv.y * av.y);
         }
     } else {
-        const float *ph = phis + (size_t)(sys / nNoise) * V;
-        for (int w = threadIdx.x; w < W0 * W0; w += kCT) {
-            const int a = w / W0, b = w - a * W0;
-            const size_t gs = (size_t)wrapn(t0 - 3 + a, Nt) * Nx + wrapn(x0 - 3 + b, Nx);
+        const _Float16 *gr = gates + (size_t)(sys / nNoise) * V * 32;
+        // all of a thread's loads are issued before the first conversion
+        constexpr int L0 = (W0 * W0 + kCT - 1) / kCT;
+        float2 rv0[L0][4];
 #pragma unroll
-            for (int c = 0; c < 4; ++c) {
-                const float2 v = rs[off + c * V + gs];
-                U0[w * 8 + 2 * c] = (__bf16)(v.x * inv);
-                U0[w * 8 + 2 * c + 1] = (__bf16)(v.y * inv);
+        for (int u = 0; u < L0; ++u) {
+            const int w = threadIdx.x + u * kCT;
+            if (w < W0 * W0) {
+                const int a = w / W0, b = w - a * W0;
+                const size_t gs = (size_t)wrapn(t0 - 3 + a, Nt) * Nx + wrapn(x0 - 3 + b, Nx);
+#pragma unroll
+                for (int c = 0; c < 4; ++c) rv0[u][c] = rs[off + c * V + gs];
             }
-            Q[w] = g * ph[gs];
         }
-        for (int i = threadIdx.x; i < 2 * kCnnNWG; i += kCT) reinterpret_cast<float *>(GW)[i] = gw[i];
+#pragma unroll
+        for (int u = 0; u < L0; ++u) {
+            const int w = threadIdx.x + u * kCT;
+            if (w < W0 * W0) {
+                bf16x8 q;
+#pragma unroll
+                for (int c = 0; c < 4; ++c) {
+                    q[2 * c] = (__bf16)(rv0[u][c].x * inv);
+                    q[2 * c + 1] = (__bf16)(rv0[u][c].y * inv);
+                }
+                *reinterpret_cast<bf16x8 *>(U0 + w * 8) = q;
+            }
+        }
         __syncthreads();
-        // layer-1 site (a, b) of the 20x20 array is (a + 1, b + 1) of the q array
-        gate_pass<W1, W0, 1>(Q, GW, GT);
+        typedef _Float16 h4 __attribute__((ext_vector_type(4)));
+        typedef __bf16 b4 __attribute__((ext_vector_type(4)));
+        // hidden layers: the lane's four gates of site m of a W x W array whose corner is
+        // lattice site (t0 - o, x0 - o), read from the per-solve gate array (halves
+        // `half` .. half + 15 of the site's 32) before the group's MFMAs; the four gated
+        // channels go to LDS with one 8-byte store
+        auto gate_of = [&](int W, int o, int half, int m, int kg) {
+            const int a = m / W, b = m - a * W;
+            const size_t gs = (size_t)wrapn(t0 - o + a, Nt) * Nx + wrapn(x0 - o + b, Nx);
+            return *reinterpret_cast<const h4 *>(gr + gs * 32 + half + 4 * kg);
+        };
+        auto gated = [&](__bf16 *U, int m, int kg, f32x4 v, h4 gt) {
+            b4 o;
+#pragma unroll
+            for (int i = 0; i < 4; ++i) o[i] = (__bf16)(v[i] * (float)gt[i]);
+            *reinterpret_cast<b4 *>(U + m * 16 + 4 * kg) = o;
+        };
+        conv_lds<8, W1, W0, ksteps(8)>(
+            U0, gf1, [&](int m, int kg) { return gate_of(W1, 2, 0, m, kg); },
+            [&](int m, int kg, f32x4 v, h4 gt) { gated(U1, m, kg, v, gt); });
         __syncthreads();
-        conv_lds<8, 16, W1, W0, ksteps(8)>(U0, gf1, [&](int m, int n, float v) {
-            U1[m * 16 + n] = (__bf16)(v * (float)GT[m * 16 + n]);
-        });
+        conv_lds<16, W2, W1, ksteps(16)>(
+            U1, gf2, [&](int m, int kg) { return gate_of(W2, 1, 16, m, kg); },
+            [&](int m, int kg, f32x4 v, h4 gt) { gated(U2, m, kg, v, gt); });
         __syncthreads();
-        // layer-2 site (a, b) of the 18x18 array is (a + 2, b + 2) of the q array
-        gate_pass<W2, W0, 2>(Q, GW + kCnnNWG / 4, GT);
-        __syncthreads();
-        conv_lds<16, 16, W2, W1, ksteps(16)>(U1, gf2, [&](int m, int n, float v) {
-            U2[m * 16 + n] = (__bf16)(v * (float)GT[m * 16 + n]);
-        });
-        __syncthreads();
-        conv_lds<16, 8, W3, W2, ksteps(16)>(U2, gf3, [&](int m, int n, float v) {
-            const int a = m / W3, b = m - a * W3;
-            const size_t gs = (size_t)(t0 + a) * Nx + x0 + b;
-            const size_t idx = off + (size_t)(n >> 1) * V + gs;
-            const float rv = reinterpret_cast<const float *>(rs + idx)[n & 1];
-            const float zv = c0 * rv + sf * v;
-            reinterpret_cast<float *>(z + idx)[n & 1] = zv;
-            const float av = reinterpret_cast<const float *>(Ap + idx)[n & 1];
-            acc.x += (double)(zv * rv);
-            acc.y += (double)(zv * av);
-        });
+        // output layer (8 channels: lanes kg = 0, 1 hold spinor components 2 kg, 2 kg + 1);
+        // r_s and Ap of the lane's two components are loaded before the group's MFMAs
+        struct RA {
+            float2 r[2], a[2];
+        };
+        conv_lds<16, W3, W2, ksteps(16)>(
+            U2, gf3,
+            [&](int m, int kg) {
+                RA q{};
+                if (kg < 2) {
+                    const int a = m / W3, b = m - a * W3;
+                    const size_t gs = (size_t)(t0 + a) * Nx + x0 + b;
+#pragma unroll
+                    for (int h = 0; h < 2; ++h) {
+                        const size_t idx = off + (size_t)(2 * kg + h) * V + gs;
+                        q.r[h] = rs[idx];
+                        q.a[h] = Ap[idx];
+                    }
+                }
+                return q;
+            },
+            [&](int m, int kg, f32x4 v, const RA &q) {
+                if (kg >= 2) return;
+                const int a = m / W3, b = m - a * W3;
+                const size_t gs = (size_t)(t0 + a) * Nx + x0 + b;
+#pragma unroll
+                for (int h = 0; h < 2; ++h) {
+                    const size_t idx = off + (size_t)(2 * kg + h) * V + gs;
+                    const float2 rv = q.r[h], av = q.a[h];
+                    const float2 zv = make_float2(c0 * rv.x + sf * v[2 * h], c0 * rv.y + sf * v[2 * h + 1]);
+                    z[idx] = zv;
+                    acc.x += (double)(zv.x * rv.x) + (double)(zv.y * rv.y);
+                    acc.y += (double)(zv.x * av.x) + (double)(zv.y * av.y);
+                }
+            });
     }
     acc = block_sum2<kCT>(acc, red);
     if (threadIdx.x == 0) part[(size_t)sys * gridDim.x + blockIdx.x] = acc;
@@ -471,6 +539,8 @@ int cnn_alloc(Ctx &c) {
     c.cnnD2 = c.cnnD3 + ksteps(8) * 64;
     YCN_HIP(hipMalloc(&c.zs, sizeof(float2) * (size_t)c.nSys * 4 * c.V));
     YCN_HIP(hipMemset(c.zs, 0, sizeof(float2) * (size_t)c.nSys * 4 * c.V));
+    YCN_HIP(hipMalloc(&c.cnnGates, sizeof(_Float16) * 32 * (size_t)c.nRep * c.V));
+    YCN_HIP(hipMemset(c.cnnGates, 0, sizeof(_Float16) * 32 * (size_t)c.nRep * c.V));
     const std::vector<float> W = default_weights(c);
     YCN_HIP(hipMemcpy(c.cnnW, W.data(), sizeof(float) * kCnnNW, hipMemcpyHostToDevice));
     c.adamStep = 0;
@@ -478,7 +548,7 @@ int cnn_alloc(Ctx &c) {
 }
 
 void cnn_free(Ctx &c) {
-    void **bufs[] = {(void **)&c.cnnW, (void **)&c.cnnF1, (void **)&c.zs, (void **)&c.cnnAct};
+    void **bufs[] = {(void **)&c.cnnW, (void **)&c.cnnF1, (void **)&c.zs, (void **)&c.cnnAct, (void **)&c.cnnGates};
     for (void **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     c.cnnActV = 0;
@@ -500,13 +570,20 @@ int cnn_get_weights(Ctx &c, float *w) {
     return YCN_OK;
 }
 
+int cnn_gates(Ctx &c, int nrep) {
+    hipLaunchKernelGGL(k_cnn_gates, dim3((unsigned)((c.V + kCT - 1) / kCT), nrep), dim3(kCT), 0, c.stream, c.p.Nt,
+                       c.p.Nx, (const float *)c.phis, (float)c.p.g, (const float4 *)(c.cnnW + kCnnOffG1), c.cnnGates);
+    YCN_HIP(hipGetLastError());
+    return YCN_OK;
+}
+
 int cnn_apply(Ctx &c, int nsys, const CgState *st, const float2 *rs, const float2 *Ap, float2 *z, double2 *part) {
     const int tilesX = c.p.Nx / 16, tiles = (c.p.Nt / 16) * tilesX;
-    prof_before(c, PK_CNN, (double)nsys * c.V * 100.0);
+    // read r_s, Ap, the gates of both layers; write z
+    prof_before(c, PK_CNN, (double)nsys * c.V * 160.0);
     hipLaunchKernelGGL(k_cnn_fused, dim3(tiles, nsys), dim3(kCT), 0, c.stream, c.p.Nt, c.p.Nx, tilesX, c.nNoise,
-                       (float)cnn_c0(c), (float)c.p.g, st, rs, Ap, (const float *)c.phis,
-                       (const float *)(c.cnnW + kCnnOffG1), z, (const bf16x8 *)c.cnnF1, (const bf16x8 *)c.cnnF2,
-                       (const bf16x8 *)c.cnnF3, part);
+                       (float)cnn_c0(c), st, rs, Ap, (const _Float16 *)c.cnnGates, z, (const bf16x8 *)c.cnnF1,
+                       (const bf16x8 *)c.cnnF2, (const bf16x8 *)c.cnnF3, part);
     prof_after(c, PK_CNN);
     YCN_HIP(hipGetLastError());
     return tiles;

@@ -646,6 +646,7 @@ int ycn_precond_apply(ycn_ctx *ctx, const double *r, double *z) {
     S.rr = rr;
     YCN_HIP(hipMemcpyAsync(c.cg_d, &S, sizeof(CgState), hipMemcpyHostToDevice, c.stream));
     if ((rc = field_to_f32(c, c.phi[c.cur], c.phis))) return rc;   // the gates read the field
+    if ((rc = cnn_gates(c, 1))) return rc;
     rc = cnn_apply(c, 1, c.cg_d, c.rs, c.Aps, c.zs, c.partials);
     if (rc < 0) return rc;
     if ((rc = spinor_to_f64(c, c.zs, c.t, 1))) return rc;

@@ -160,6 +160,7 @@ struct Ctx {
     bf16x8 *cnnF1 = nullptr, *cnnF2 = nullptr, *cnnF3 = nullptr, *cnnD3 = nullptr, *cnnD2 = nullptr;
     float2 *zs = nullptr;                 // z = CNN(r) [nSys][4][V]
     float *cnnAct = nullptr;              // training activations
+    _Float16 *cnnGates = nullptr;         // field gates of both hidden layers, [rep][site][32] (per solve)
     size_t cnnActV = 0;
     int adamStep = 0;
     double2 *pairR = nullptr, *pairX = nullptr;   // training snapshots [pairs][4][V]
@@ -264,6 +265,8 @@ int cnn_set_weights(Ctx &c, const float *w);
 int cnn_get_weights(Ctx &c, float *w);
 int cnn_get_grad(Ctx &c, float *g);
 // launches the fused CNN on nsys systems; returns the number of partials per system
+// the field gates of the first nrep replicas from c.phis (before cnn_apply, once per field)
+int cnn_gates(Ctx &c, int nrep);
 int cnn_apply(Ctx &c, int nsys, const CgState *st, const float2 *rs, const float2 *Ap, float2 *z, double2 *part);
 int cnn_train_step(Ctx &c, const double2 *r, const double2 *e, const float *phis, float lr, double *loss,
                    bool update);
