@@ -1673,7 +1673,7 @@ int eo_drift(Ctx &c, const double *phi, double *drift, const StepCtl *ctl) {
 // most two executables (one per field buffer).
 constexpr int kStepSegment = 16;   // captured steps between two reads of the control block
 constexpr int kMaxTail = 8;
-constexpr size_t kMaxStepGraphs = 16;
+constexpr size_t kMaxStepGraphs = 48;
 
 bool eo_graph_eligible(const Ctx &c) {
     return c.stepGraphsOn && c.useGraphs && c.profTarget == PK_NONE && c.p.evenOdd && eo_available(c) &&
@@ -1775,6 +1775,37 @@ static int eo_step_graph(Ctx &c, int parity, const std::vector<int> &chunks, hip
     return YCN_OK;
 }
 
+// While a segment runs on the GPU the host would only wait: it captures the schedules
+// the next segment most likely needs, the current one with one chunk 8 launches shorter
+// or longer (the rounds of consecutive solves move by a few iterations), for both field
+// buffers, at most kPrecapture new ones per segment.  They go in at the cache's old end,
+// so that they are evicted before anything the running segment uses.
+constexpr int kPrecapture = 8;
+static int eo_precapture(Ctx &c, const std::vector<int> &chunks, int nTail, bool bothParities) {
+    int made = 0;
+    const size_t nMain = chunks.size() - (size_t)nTail;
+    for (size_t i = 0; i < nMain && made < kPrecapture; ++i)
+        for (int d : {-8, 8}) {
+            if (chunks[i] + d < 8) continue;
+            std::vector<int> v = chunks;
+            v[i] += d;
+            for (int par = 0; par < (bothParities ? 2 : 1) && made < kPrecapture; ++par) {
+                const int parity = bothParities ? par : c.cur;
+                bool have = false;
+                for (const Ctx::StepGraph &e : c.stepGraphs)
+                    if (e.chunks == v && e.parity == parity) have = true;
+                if (have) continue;
+                if (c.stepGraphs.size() + 1 >= kMaxStepGraphs) return YCN_OK;   // never evict here
+                hipGraphExec_t ex;
+                const int rc = eo_capture_step(c, parity, v, &ex);
+                if (rc) return rc;
+                c.stepGraphs.insert(c.stepGraphs.begin(), {v, parity, ex});
+                ++made;
+            }
+        }
+    return YCN_OK;
+}
+
 // One captured step on the context stream, no host wait: k_step_begin, the coloured
 // noise (on the noise stream beside the solve when the context has it, as a single step
 // does: a graph with a second branch measured slower), the captured solve and drift, then
@@ -1843,6 +1874,7 @@ int eo_graph_run(Ctx &c, uint64_t step0, int nsteps, SolveStats *tot) {
             if ((rc = eo_enqueue_step(c, parity, ex[parity]))) return rc;
         }
         YCN_HIP(hipMemcpyAsync(c.ctl_h, c.ctl_d, sizeof(StepCtl), hipMemcpyDeviceToHost, c.stream));
+        if ((rc = eo_precapture(c, chunks, c.stepTail, fixed))) return rc;
         if ((rc = spin_sync(c))) return rc;
         if (!h.halt) {
             c.cur = fixed ? p0 ^ (seg & 1) : p0;
