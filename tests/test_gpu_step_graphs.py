@@ -34,10 +34,12 @@ def l2rel(a, b):
     return float(np.linalg.norm(np.ravel(a - b)) / np.linalg.norm(np.ravel(b)))
 
 
-def _ctx(Y, monkeypatch, graphs, n, reps, adaptive):
+def _ctx(Y, monkeypatch, graphs, n, reps, adaptive, every=None):
     p, _ = mk(Y, Nt=n, Nx=n, nReplicas=reps, adaptive=adaptive, **C3)
     p.cgPrecision = Y.YCN_CG_MIXED
     p.evenOdd = 1
+    if every is not None:
+        p.reliableEvery = every
     with monkeypatch.context() as m:
         if graphs:
             m.delenv("YCN_STEP_GRAPHS", raising=False)
@@ -115,4 +117,29 @@ def test_halted_step_is_finished_by_the_host(gpu, monkeypatch, adaptive):
     for st in (sg, se):
         assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
     assert e < 1e-9, e
+    assert wg < we
+
+
+def test_many_reliable_updates_per_solve(gpu, monkeypatch):
+    """With an FP64 reliable update every 8 iterations a solve makes many more of them
+    than the seven whose iteration counts the solver state keeps; the captured schedule
+    continues past the last recorded one in chunks of the update period up to the
+    previous solve's iteration count, so the steps do not halt."""
+    Y = gpu
+    n, reps, nsteps = 128, 2, 40
+    hot = np.random.default_rng(9).uniform(-1.0, 1.0, (reps, n, n))
+    out = {}
+    for graphs in (True, False):
+        c = _ctx(Y, monkeypatch, graphs, n, reps, 0, every=8)
+        with c:
+            out[graphs] = _run(c, [hot], nsteps)
+    (fg, sg, wg, hg), (fe, se, we, he) = out[True], out[False]
+    e = l2rel(fg, fe)
+    print(f"# reliable update every 8: |phi_graph - phi_eager| / |phi_eager| = {e:.2e}; halted steps {hg}; "
+          f"host waits {wg} (captured) vs {we} (eager); iterations per solve {sg['iters_mean']:.1f}", flush=True)
+    assert sg["iters_mean"] > 8 * 8   # more updates than the state records
+    for st in (sg, se):
+        assert st["converged"] == 1 and st["true_rel_resid"] <= 1e-10, st
+    assert e < 1e-9, e
+    assert hg <= 2, hg
     assert wg < we

@@ -1708,10 +1708,21 @@ static std::vector<int> eo_schedule(const Ctx &c, int tail) {
     if (c.eoRounds.empty()) {
         v = eo_main_chunks(c, c.eoPredIters);
     } else {
-        int prev = 0;
+        int prev = 0, total = 0;
         for (int r : c.eoRounds) {
             v.push_back(std::max(8, (r - prev + 2 + 7) / 8 * 8));
+            total += v.back();
             prev = r;
+        }
+        // a solve with more reliable updates than CgState records ends after its last
+        // recorded round: chunks of the reliable-update interval up to its iteration count
+        if (prev < c.eoPredIters) {
+            const int every = std::max(1, c.p.reliableEvery), nIt = every + (every & 1);
+            const int want = std::min(c.eoPredIters, c.p.cgMaxIter) + 2;
+            while (total < want) {
+                v.push_back(nIt);
+                total += nIt;
+            }
         }
     }
     for (int k = 0; k < tail; ++k) v.push_back(kTailRound);
