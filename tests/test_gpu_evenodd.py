@@ -267,7 +267,10 @@ def test_profile_counts_iterations(gpu, fin, monkeypatch):
         assert n == st["iters"]
     else:
         assert st["iters"] <= n <= st["iters"] + 3 * 64
-    assert b == N * N * (8 * 16 + 4)
+    # read r, Ap, p and write r, p, Ap (FP32 half-lattice vectors) and read phi, plus
+    # x_s read and written: by every launch with the separate finalize, by every other
+    # launch with the prologue finalize (the even launch defers its x_s update)
+    assert b == N * N * ((7 if fin == "fused" else 8) * 16 + 4)
     assert 0.0 < ms < 1e3 * wall
 
 
@@ -289,3 +292,35 @@ def test_eo_mixed_iteration_cap_with_prediction(gpu):
             print(f"# solve {k}: {st}", flush=True)
             assert st["converged"] == 0 and st["iters"] == cap, st
             assert np.all(np.isfinite(d)) and np.isfinite(st["true_rel_resid"])
+
+
+@pytest.mark.parametrize("cap", [61, 121])
+def test_eo_mixed_odd_cap_keeps_the_deferred_update(gpu, monkeypatch, cap):
+    """With the finalize in the next launch's prologue, the even launch of each pair
+    leaves its x_s update to the odd launch after it.  A solve stopped by an odd iteration
+    cap pauses in an odd launch, which must still add the even launch's term before it
+    returns.  Its FP64 true residual is then the one the separate finalize (every launch
+    updates x_s) reaches at the same cap, and not the one of an iteration earlier: the two
+    forms agree to well under the change one iteration makes."""
+    Y = gpu
+    Nt, Nx = 32, 64
+    phi = O.hot_start(11, Nt, Nx)
+    res = {}
+    for fin in ("fused", "separate"):
+        monkeypatch.setenv("YCN_EO_FIN", fin)
+        for n in (cap - 1, cap):
+            p, _ = mk(Y, Nt=Nt, Nx=Nx, M=0.05, g=0.5, cgTol=1e-30, cgTolMode=1, cgMaxIter=n)
+            p.evenOdd = 1
+            p.cgPrecision = Y.YCN_CG_MIXED
+            with Y.Context(p) as c:
+                rep = c.tuning_report()
+                assert ("prologue finalize" if fin == "fused" else "separate finalize") in rep.split("|")[-1], rep
+                c.set_field(phi)
+                _, st = c.fermion_force(5)
+            assert st["converged"] == 0 and st["iters"] == n, st
+            res[fin, n] = st["true_rel_resid"]
+    step = abs(res["separate", cap] - res["separate", cap - 1])
+    gap = abs(res["fused", cap] - res["separate", cap])
+    print(f"# cap {cap}: true residual {res['fused', cap]:.4e} (prologue finalize) vs {res['separate', cap]:.4e} "
+          f"(separate); one iteration earlier {res['separate', cap - 1]:.4e}", flush=True)
+    assert gap < 0.25 * step, (gap, step)

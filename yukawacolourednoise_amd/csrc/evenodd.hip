@@ -552,6 +552,8 @@ k_fin_pf_chunk(const double2 *__restrict__ slotLast, const double2 *__restrict__
 // the tile (= <p', Ap>) is the block's partial.  HBM traffic per launch: read r, p_old
 // (Vh sites each) and phi, write p', Ap: 4 S/2 + P per site of the full lattice.
 constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave : tt * tx / 2; }
+// below this beta the deferred x_s term reads p_{idx-2} instead of rebuilding it
+constexpr double kXBetaMin = 0.05;
 
 // PF = false: the operator alone, p' = r + beta p_old, partial <p', Ap> = |v|^2.
 // PF = true, the pipelined FP32 iteration with one reduction.  CG with the residual-norm
@@ -582,7 +584,19 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // partials are |res|^2 (part) and |r|^2 (part2).  Systems already final are skipped.
 // WR1 = true: Wilson parameter r = 1 (hop_flavour_r1), chosen at launch when A.r == 1 (every
 // precision and mode).
-template <typename R, int TT, int TX, bool PF, bool PERSIST = false, bool RU = false, bool WR1 = false>
+// XM (PF only): the x_s update.  1: x_s += alpha p_old every launch (finMode 0).  With the
+// finalize in the prologue (finMode 1, 2) launches alternate: an even launch idx (0) leaves
+// x_s alone and block 0 records its alpha_e, beta_e in the state (xAB); the odd launch after
+// it (2) adds both launches' terms, alpha_e p_{idx-2} + alpha p_old.  p_old = r + beta_e
+// p_{idx-2} with r and p_old its own inputs, so p_{idx-2} = (p_old - r) / beta_e costs no
+// extra traffic; its rounding error relative to the exact term is below 2 eps / sqrt(beta_e),
+// so for beta_e < kXBetaMin p_{idx-2} is read instead from the ping-pong buffer p_new, which
+// still holds it until this launch overwrites it.  Pairs never straddle a chunk (chunks have
+// an even number of launches), and a system pauses only in an even launch (after whole
+// pairs), except at the iteration cap: an odd launch that pauses adds the deferred term on
+// its tiles before it returns.  x_s moves every other launch: 200 instead of 264 B per even
+// site on the other.
+template <typename R, int TT, int TX, bool PF, bool PERSIST = false, bool RU = false, bool WR1 = false, int XM = 1>
 __global__ void __launch_bounds__(eo_threads(TT, TX))
 k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                   const typename Vec2<R>::T *__restrict__ r, typename Vec2<R>::T *__restrict__ r_out,
@@ -624,6 +638,11 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     if (RU) out32 += off;
     R beta = RU ? (R)0 : (R)st[sys].beta;
     R alpha = PF ? (R)st[sys].alpha.x : (R)0;
+    double alphaD = PF ? st[sys].alpha.x : 0.0, betaD = RU ? 0.0 : st[sys].beta;   // XM = 0: recorded in xAB
+    // XM = 2: the deferred term's alpha_e and beta_e, and the x_s coefficients
+    // x_s += cP p_old + cR r + cN p_{idx-2} (set once the prologue has fixed alpha)
+    const double2 xPrev = (PF && XM == 2) ? st[sys].xAB : make_double2(0.0, 0.0);
+    R cP = alpha, cR = 0, cN = 0;
     const R *ph = phi + (size_t)(sys / A.nNoise) * A.V;
     const R hp = (R)(0.5 * (A.r + 1.0)), hm = (R)(0.5 * (A.r - 1.0)), hr = (R)(0.5 * A.r), h = (R)0.5;
     const R d0 = (R)(2.0 * A.r + A.M), g = (R)A.g;
@@ -661,7 +680,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                     G.ra[u][c] = at(r, i);
                     if (!RU) G.pa[u][c] = at(pold, i);
                     if (PF) G.aa[u][c] = at(Ap_in, i);
-                    if (PF && inner) G.xa[u][c] = at(xs, i);
+                    if (PF && XM != 0 && inner) G.xa[u][c] = at(xs, i);
                 }
                 G.fe[u] = at(ph, (unsigned)(gt * Nx + gxE));
                 G.fo[u] = at(ph, (unsigned)(gt * Nx + gxO));
@@ -671,7 +690,10 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     // stage 0 proper: p' (PF: r' = r - alpha Ap, x_s += alpha p) into LDS and, on the
     // tile, to global memory; d on both parities
     double rho = 0.0;
-    auto compute0 = [&](const Regs &G) {
+    // exact (XM = 2 with a small beta_e): p_{idx-2} read from p_new; a separate
+    // instantiation, so that the usual form waits on no load between its stores
+    auto compute0 = [&](const Regs &G, auto exact) {
+        constexpr bool EXACT = decltype(exact)::value;
 #pragma unroll
         for (int u = 0; u < M0; ++u) {
             const int w = threadIdx.x + u * NT;
@@ -689,10 +711,16 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                     const V2 pv = RU ? a : mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
                     E[c][w] = pv;
                     if (inner && !RU) {
+                        V2 pn = mk2<R>(0, 0);   // EXACT: p_{idx-2}, read before it is overwritten
+                        if constexpr (PF && XM == 2 && EXACT) pn = at(pnew, i);
                         at(pnew, i) = pv;
                         if (PF) {
                             at(r_out, i) = a;
-                            at(xs, i) = mk2<R>(G.xa[u][c].x + alpha * b.x, G.xa[u][c].y + alpha * b.y);
+                            const V2 xa = G.xa[u][c], r0 = G.ra[u][c];
+                            if (XM == 1) at(xs, i) = mk2<R>(xa.x + alpha * b.x, xa.y + alpha * b.y);
+                            if (XM == 2)
+                                at(xs, i) = mk2<R>(xa.x + cP * b.x + cR * r0.x + cN * pn.x,
+                                                   xa.y + cP * b.y + cR * r0.y + cN * pn.y);
                             rs += a.x * a.x + a.y * a.y;
                         }
                     }
@@ -836,6 +864,8 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         if (!pause) {
             alpha = (R)bc[0];
             beta = (R)bc[1];
+            alphaD = bc[0];
+            betaD = bc[1];
             return true;
         }
         if (blockIdx.x == 0) {   // record the pause; rho of the iteration before it
@@ -857,22 +887,68 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         return false;
     };
 
+    // once alpha is fixed: XM = 0 records this launch's alpha, beta for the next launch;
+    // XM = 2 sets the x_s coefficients
+    auto x_coefs = [&]() {
+        if constexpr (PF && XM == 0) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) st[sys].xAB = make_double2(alphaD, betaD);
+        }
+        if constexpr (PF && XM == 2) {
+            const bool defer = xPrev.x != 0.0, rebuild = defer && xPrev.y >= kXBetaMin;
+            const double q = rebuild ? xPrev.x / xPrev.y : 0.0;
+            cP = (R)(q + alphaD);
+            cR = (R)-q;
+            cN = (defer && !rebuild) ? (R)xPrev.x : (R)0;
+        }
+    };
+    // XM = 2, a launch that pauses (at the iteration cap): the deferred alpha_e p_{idx-2}
+    // on the block's tiles, p_{idx-2} from p_new
+    auto x_deferred = [&]() {
+        if constexpr (PF && XM == 2) {
+            if (xPrev.x == 0.0) return;
+            const R ae = (R)xPrev.x;
+            for (int t = blockIdx.x; t < (PERSIST ? nTiles : (int)blockIdx.x + 1); t += (int)gridDim.x) {
+                int a0, b0;
+                origin(t, a0, b0);
+                for (int w = threadIdx.x; w < TT * (TX / 2); w += NT) {
+                    const Site q = site_of(w, 0, 0);
+                    const unsigned j = (unsigned)((a0 - 4 + q.row) * Nh + ((b0 - 4 + q.col) >> 1));
+#pragma unroll
+                    for (int c = 0; c < 4; ++c) {
+                        const V2 pn = at(pnew, c * Vh + j), xv = at(xs, c * Vh + j);
+                        at(xs, c * Vh + j) = mk2<R>(xv.x + ae * pn.x, xv.y + ae * pn.y);
+                    }
+                }
+            }
+        }
+    };
+
     Regs G;
     int t0, x0;
     if (!PERSIST) {
         origin(blockIdx.x, t0, x0);
         load0(t0, x0, G);
-        if (!prologue()) return;
-        compute0(G);
+        if (!prologue()) {
+            x_deferred();
+            return;
+        }
+        x_coefs();
+        if (XM == 2 && cN != (R)0) compute0(G, std::true_type());
+        else compute0(G, std::false_type());
         __syncthreads();
         stages(t0, x0);
     } else {
         int t = blockIdx.x;
         origin(t, t0, x0);
         load0(t0, x0, G);
-        if (!prologue()) return;
+        if (!prologue()) {
+            x_deferred();
+            return;
+        }
+        x_coefs();
         while (t < nTiles) {
-            compute0(G);
+            if (XM == 2 && cN != (R)0) compute0(G, std::true_type());
+            else compute0(G, std::false_type());
             __syncthreads();
             const int tn = t + (int)gridDim.x;
             int n0 = 0, nx0 = 0;
@@ -1123,23 +1199,40 @@ static int resident_blocks(const void *kernel, int threads) {
 // returns the number of blocks per system (= partial slots per system)
 // the pipelined FP32 iteration's launch (persistent or one block per tile), WR1 as in
 // k_eo_normal_fused; returns the blocks per system
+// XM as in k_eo_normal_fused: 1 with the separate finalize, else 0 and 2 alternating.
+// Persistent grids are sized by the variant with the fewest resident blocks, so that
+// consecutive launches leave the same number of partials.
 template <int TT, int TX, bool WR1>
 static int launch_eo_pf_tt(const Ctx &c, int nsys, const EoArgs &A, const float *phi, bool persist,
                            const EoPtrs<float> &b, int finMode) {
     const dim3 block(eo_threads(TT, TX));
+    const int xm = finMode == 0 ? 1 : (b.idx & 1) ? 2 : 0;
     if (persist) {
-        auto *k = &k_eo_normal_fused<float, TT, TX, true, true, false, WR1>;
-        static int resident = resident_blocks(reinterpret_cast<const void *>(k), eo_threads(TT, TX));
+        const void *ks[3] = {reinterpret_cast<const void *>(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 0>),
+                             reinterpret_cast<const void *>(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 1>),
+                             reinterpret_cast<const void *>(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 2>)};
+        static const int resident = std::min({resident_blocks(ks[0], block.x), resident_blocks(ks[1], block.x),
+                                              resident_blocks(ks[2], block.x)});
         int g = std::min(A.tilesPerSys, std::max(resident, 8));
         if (A.xcdRemap && g >= 8) g -= g % 8;   // keeps every block's tiles on one XCD's band
-        hipLaunchKernelGGL(k, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew,
-                           b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
-                           (double2 *)nullptr);
+        auto go = [&](auto *k) {
+            hipLaunchKernelGGL(k, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold,
+                               b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
+                               (double2 *)nullptr);
+        };
+        if (xm == 0) go(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 0>);
+        else if (xm == 1) go(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 1>);
+        else go(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 2>);
         return g;
     }
-    hipLaunchKernelGGL((k_eo_normal_fused<float, TT, TX, true, false, false, WR1>), dim3(A.tilesPerSys, nsys), block, 0,
-                       c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, b.part, finMode,
-                       b.idx, b.prevPart, c.active, (float2 *)nullptr, (double2 *)nullptr);
+    auto go = [&](auto *k) {
+        hipLaunchKernelGGL(k, dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in,
+                           b.pold, b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
+                           (double2 *)nullptr);
+    };
+    if (xm == 0) go(&k_eo_normal_fused<float, TT, TX, true, false, false, WR1, 0>);
+    else if (xm == 1) go(&k_eo_normal_fused<float, TT, TX, true, false, false, WR1, 1>);
+    else go(&k_eo_normal_fused<float, TT, TX, true, false, false, WR1, 2>);
     return A.tilesPerSys;
 }
 
@@ -1230,8 +1323,10 @@ static int launch_eo_pf(Ctx &c, int ti, int nsys, int idx, int finMode) {
     b.pold = c.ps[par];
     b.pnew = c.ps[par ^ 1];
     b.xs = c.xs;
-    // read r, Ap, p, x_s and write r, p, Ap, x_s (half-lattice FP32 vectors) and read phi
-    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (8 * 16 + 4));
+    // read r, Ap, p, x_s and write r, p, Ap, x_s (half-lattice FP32 vectors) and read phi;
+    // an even launch with the finalize in the prologue leaves x_s alone
+    const int vecs = (finMode != 0 && !(idx & 1)) ? 6 : 8;
+    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (vecs * 16 + 4));
     const bool ps = t.persist;
     int nb = 0;
     switch (t.TT * 1000 + t.TX) {
@@ -1478,6 +1573,10 @@ static int eo_mixed_chunk(Ctx &c, const double *phi, int nsys, int n, bool graph
     const EoBufs B = eo_bufs(c, nsys);
     // reliable-update partials: at most V / 64 blocks per system each (smallest FP64 tile)
     double2 *partRu = c.partials, *partRu2 = c.partials + (size_t)nsys * ((c.V + 63) / 64);
+    if (n & 1) {   // the prologue-finalized launches defer x_s updates in pairs
+        c.err = "internal: a chunk of the pipelined iteration has an odd number of launches";
+        return YCN_ERR_INVALID;
+    }
     if (graphed) {   // already inside a capture: the launches themselves
         for (int i = 0; i < n; ++i) launch_eo_pf_iteration(c, nsys, i);
     } else {

@@ -21,7 +21,8 @@ namespace ycn {
 
 void prof_before(Ctx &c, int k, double bytes) {
     if (c.profTarget != k) return;
-    c.profBytes = bytes;
+    c.profBytes += bytes;
+    ++c.profLaunches;
     if (c.profChunk || c.evCount >= (int)c.evStart.size()) return;
     (void)hipEventRecord(c.evStart[c.evCount], c.stream);
 }
@@ -746,6 +747,7 @@ int ycn_profile_begin(ycn_ctx *ctx, const char *kernel) {
     c.profTarget = k;
     c.evCount = 0;
     c.profBytes = 0;
+    c.profLaunches = 0;
     return YCN_OK;
 }
 
@@ -777,7 +779,7 @@ int ycn_profile_end(ycn_ctx *ctx, double *total_ms, int64_t *launches, double *b
     }
     if (total_ms) *total_ms = tot;
     if (launches) *launches = n;
-    if (bytes_per_launch) *bytes_per_launch = c.profBytes;
+    if (bytes_per_launch) *bytes_per_launch = c.profLaunches ? c.profBytes / (double)c.profLaunches : 0.0;
     c.profTarget = PK_NONE;
     return YCN_OK;
 }

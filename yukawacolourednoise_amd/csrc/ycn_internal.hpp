@@ -52,6 +52,7 @@ struct CgState {
     double rrRel;      // FP32 loops: recursive |r|^2 when the recursion was last anchored (start, reliable update)
     int nRu;           // pipelined even-odd solve: reliable updates so far
     int ruIter[7];     // pipelined even-odd solve: iteration count at each of the first seven
+    double2 xAB;       // pipelined even-odd solve: alpha, beta of the last launch that deferred its x_s update
 };
 constexpr int kMaxRounds = 7;
 
@@ -199,7 +200,8 @@ struct Ctx {
     std::vector<int> evWork;   // per event pair: -1 = one launch; n >= 0 = a chunk of n working launches
     int evCount = 0;
     int profChunk = 0;         // 1 while a chunk is bracketed as a whole (per-launch records off)
-    double profBytes = 0;
+    double profBytes = 0;      // algorithmic bytes of the profiled kernel's launches, summed
+    int64_t profLaunches = 0;  // and their count
 };
 
 // error helpers
