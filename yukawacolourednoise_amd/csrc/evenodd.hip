@@ -552,8 +552,6 @@ k_fin_pf_chunk(const double2 *__restrict__ slotLast, const double2 *__restrict__
 // the tile (= <p', Ap>) is the block's partial.  HBM traffic per launch: read r, p_old
 // (Vh sites each) and phi, write p', Ap: 4 S/2 + P per site of the full lattice.
 constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave : tt * tx / 2; }
-// below this beta the deferred x_s term reads p_{idx-2} instead of rebuilding it
-constexpr double kXBetaMin = 0.05;
 
 // PF = false: the operator alone, p' = r + beta p_old, partial <p', Ap> = |v|^2.
 // PF = true, the pipelined FP32 iteration with one reduction.  CG with the residual-norm
@@ -584,19 +582,29 @@ constexpr double kXBetaMin = 0.05;
 // partials are |res|^2 (part) and |r|^2 (part2).  Systems already final are skipped.
 // WR1 = true: Wilson parameter r = 1 (hop_flavour_r1), chosen at launch when A.r == 1 (every
 // precision and mode).
-// XM (PF only): the x_s update.  1: x_s += alpha p_old every launch (finMode 0).  With the
-// finalize in the prologue (finMode 1, 2) launches alternate: an even launch idx (0) leaves
-// x_s alone and block 0 records its alpha_e, beta_e in the state (xAB); the odd launch after
-// it (2) adds both launches' terms, alpha_e p_{idx-2} + alpha p_old.  p_old = r + beta_e
-// p_{idx-2} with r and p_old its own inputs, so p_{idx-2} = (p_old - r) / beta_e costs no
-// extra traffic; its rounding error relative to the exact term is below 2 eps / sqrt(beta_e),
-// so for beta_e < kXBetaMin p_{idx-2} is read instead from the ping-pong buffer p_new, which
-// still holds it until this launch overwrites it.  Pairs never straddle a chunk (chunks have
-// an even number of launches), and a system pauses only in an even launch (after whole
-// pairs), except at the iteration cap: an odd launch that pauses adds the deferred term on
-// its tiles before it returns.  x_s moves every other launch: 200 instead of 264 B per even
-// site on the other.
-template <typename R, int TT, int TX, bool PF, bool PERSIST = false, bool RU = false, bool WR1 = false, int XM = 1>
+// With the finalize in the prologue (finMode 1, 2) the iteration keeps neither r nor x_s in
+// every launch.  Block 0 of each launch records the alpha, beta it used in the state
+// (ab[idx & 1]) for the next launch, and the directions rotate through three buffers.
+// RM (PF only), where r_{idx-1} comes from:
+//   0: read from r, and r' written to r_out (finMode 0);
+//   1: read from r, nothing written (launch 0 of a chunk, after the setup or a reliable
+//      update has put the residual there);
+//   2: rebuilt as r_{idx-1} = p_{idx-1} - beta_{idx-1} p_{idx-2}, the relation that formed
+//      p_{idx-1} (same FP32 beta); r then carries p_{idx-2}, read in place of the residual.
+//      The rebuild's rounding error is a few eps |p| against the eps |r| of the stored r
+//      (|p| is a few |r| in CG), and the FP64 reliable updates bound its drift as they do
+//      the recursion's.
+// XM (PF only), the x_s update:
+//   1: x_s += alpha p_old every launch (finMode 0);
+//   0: none (an even launch);
+//   2: the odd launch after it adds both terms, alpha_e p_{idx-2} + alpha p_{idx-1}, with
+//      p_{idx-2} the input RM = 2 reads anyway.
+// Pairs never straddle a chunk (chunks have an even number of launches), and a system
+// pauses only in an even launch (after whole pairs), except at the iteration cap: an odd
+// launch that pauses adds the deferred term on its tiles before it returns.  Per even site
+// and launch: 264 B with RM = 0, XM = 1; 168 B (even) and 232 B (odd) with RM = 1, 2.
+template <typename R, int TT, int TX, bool PF, bool PERSIST = false, bool RU = false, bool WR1 = false, int XM = 1,
+          int RM = 0>
 __global__ void __launch_bounds__(eo_threads(TT, TX))
 k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                   const typename Vec2<R>::T *__restrict__ r, typename Vec2<R>::T *__restrict__ r_out,
@@ -606,6 +614,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                   const double2 *__restrict__ prevPart, int *active, float2 *__restrict__ out32,
                   double2 *__restrict__ part2) {
     static_assert(!(PF && RU), "the reliable-update residual runs in FP64 only");
+    static_assert(XM != 2 || RM == 2, "the pair's x_s update reads p_{idx-2}");
     using V2 = typename Vec2<R>::T;
     constexpr int H = TT + 8, K = (TX + 8) / 2, NT = eo_threads(TT, TX);
     constexpr int M0 = (H * K + NT - 1) / NT;
@@ -638,11 +647,10 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     if (RU) out32 += off;
     R beta = RU ? (R)0 : (R)st[sys].beta;
     R alpha = PF ? (R)st[sys].alpha.x : (R)0;
-    double alphaD = PF ? st[sys].alpha.x : 0.0, betaD = RU ? 0.0 : st[sys].beta;   // XM = 0: recorded in xAB
-    // XM = 2: the deferred term's alpha_e and beta_e, and the x_s coefficients
-    // x_s += cP p_old + cR r + cN p_{idx-2} (set once the prologue has fixed alpha)
-    const double2 xPrev = (PF && XM == 2) ? st[sys].xAB : make_double2(0.0, 0.0);
-    R cP = alpha, cR = 0, cN = 0;
+    double alphaD = PF ? st[sys].alpha.x : 0.0, betaD = RU ? 0.0 : st[sys].beta;   // recorded in ab[idx & 1]
+    // RM = 2, XM = 2: the alpha, beta of launch idx - 1
+    const double2 prev = (PF && RM == 2) ? st[sys].ab[(idx - 1) & 1] : make_double2(0.0, 0.0);
+    const R betaPrev = (R)prev.y, alphaPrev = (R)prev.x;
     const R *ph = phi + (size_t)(sys / A.nNoise) * A.V;
     const R hp = (R)(0.5 * (A.r + 1.0)), hm = (R)(0.5 * (A.r - 1.0)), hr = (R)(0.5 * A.r), h = (R)0.5;
     const R d0 = (R)(2.0 * A.r + A.M), g = (R)A.g;
@@ -690,10 +698,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
     // stage 0 proper: p' (PF: r' = r - alpha Ap, x_s += alpha p) into LDS and, on the
     // tile, to global memory; d on both parities
     double rho = 0.0;
-    // exact (XM = 2 with a small beta_e): p_{idx-2} read from p_new; a separate
-    // instantiation, so that the usual form waits on no load between its stores
-    auto compute0 = [&](const Regs &G, auto exact) {
-        constexpr bool EXACT = decltype(exact)::value;
+    auto compute0 = [&](const Regs &G) {
 #pragma unroll
         for (int u = 0; u < M0; ++u) {
             const int w = threadIdx.x + u * NT;
@@ -705,22 +710,21 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
 #pragma unroll
                 for (int c = 0; c < 4; ++c) {
                     const unsigned i = c * Vh + G.jj[u];
-                    V2 a = G.ra[u][c];
+                    V2 a = G.ra[u][c];   // r_{idx-1}, or p_{idx-2} (RM = 2)
                     const V2 b = RU ? a : G.pa[u][c];
+                    if (PF && RM == 2) a = mk2<R>(b.x - betaPrev * a.x, b.y - betaPrev * a.y);
                     if (PF) a = mk2<R>(a.x - alpha * G.aa[u][c].x, a.y - alpha * G.aa[u][c].y);
                     const V2 pv = RU ? a : mk2<R>(a.x + beta * b.x, a.y + beta * b.y);
                     E[c][w] = pv;
                     if (inner && !RU) {
-                        V2 pn = mk2<R>(0, 0);   // EXACT: p_{idx-2}, read before it is overwritten
-                        if constexpr (PF && XM == 2 && EXACT) pn = at(pnew, i);
                         at(pnew, i) = pv;
                         if (PF) {
-                            at(r_out, i) = a;
-                            const V2 xa = G.xa[u][c], r0 = G.ra[u][c];
+                            if (RM == 0) at(r_out, i) = a;
+                            const V2 xa = G.xa[u][c], p2 = G.ra[u][c];
                             if (XM == 1) at(xs, i) = mk2<R>(xa.x + alpha * b.x, xa.y + alpha * b.y);
                             if (XM == 2)
-                                at(xs, i) = mk2<R>(xa.x + cP * b.x + cR * r0.x + cN * pn.x,
-                                                   xa.y + cP * b.y + cR * r0.y + cN * pn.y);
+                                at(xs, i) = mk2<R>(xa.x + alphaPrev * p2.x + alpha * b.x,
+                                                   xa.y + alphaPrev * p2.y + alpha * b.y);
                             rs += a.x * a.x + a.y * a.y;
                         }
                     }
@@ -887,26 +891,18 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         return false;
     };
 
-    // once alpha is fixed: XM = 0 records this launch's alpha, beta for the next launch;
-    // XM = 2 sets the x_s coefficients
-    auto x_coefs = [&]() {
-        if constexpr (PF && XM == 0) {
-            if (blockIdx.x == 0 && threadIdx.x == 0) st[sys].xAB = make_double2(alphaD, betaD);
-        }
-        if constexpr (PF && XM == 2) {
-            const bool defer = xPrev.x != 0.0, rebuild = defer && xPrev.y >= kXBetaMin;
-            const double q = rebuild ? xPrev.x / xPrev.y : 0.0;
-            cP = (R)(q + alphaD);
-            cR = (R)-q;
-            cN = (defer && !rebuild) ? (R)xPrev.x : (R)0;
+    // once alpha is fixed, the prologue forms record this launch's alpha, beta for the next
+    auto record_ab = [&]() {
+        if constexpr (PF && XM != 1) {
+            if (blockIdx.x == 0 && threadIdx.x == 0) st[sys].ab[idx & 1] = make_double2(alphaD, betaD);
         }
     };
     // XM = 2, a launch that pauses (at the iteration cap): the deferred alpha_e p_{idx-2}
-    // on the block's tiles, p_{idx-2} from p_new
+    // on the block's tiles (p_{idx-2} in r)
     auto x_deferred = [&]() {
         if constexpr (PF && XM == 2) {
-            if (xPrev.x == 0.0) return;
-            const R ae = (R)xPrev.x;
+            if (alphaPrev == (R)0) return;
+            const R ae = alphaPrev;
             for (int t = blockIdx.x; t < (PERSIST ? nTiles : (int)blockIdx.x + 1); t += (int)gridDim.x) {
                 int a0, b0;
                 origin(t, a0, b0);
@@ -915,7 +911,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                     const unsigned j = (unsigned)((a0 - 4 + q.row) * Nh + ((b0 - 4 + q.col) >> 1));
 #pragma unroll
                     for (int c = 0; c < 4; ++c) {
-                        const V2 pn = at(pnew, c * Vh + j), xv = at(xs, c * Vh + j);
+                        const V2 pn = at(r, c * Vh + j), xv = at(xs, c * Vh + j);
                         at(xs, c * Vh + j) = mk2<R>(xv.x + ae * pn.x, xv.y + ae * pn.y);
                     }
                 }
@@ -932,9 +928,8 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
             x_deferred();
             return;
         }
-        x_coefs();
-        if (XM == 2 && cN != (R)0) compute0(G, std::true_type());
-        else compute0(G, std::false_type());
+        record_ab();
+        compute0(G);
         __syncthreads();
         stages(t0, x0);
     } else {
@@ -945,10 +940,9 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
             x_deferred();
             return;
         }
-        x_coefs();
+        record_ab();
         while (t < nTiles) {
-            if (XM == 2 && cN != (R)0) compute0(G, std::true_type());
-            else compute0(G, std::false_type());
+            compute0(G);
             __syncthreads();
             const int tn = t + (int)gridDim.x;
             int n0 = 0, nx0 = 0;
@@ -995,14 +989,19 @@ static void launch_fin_pf(Ctx &c, int nsys, int nb, int commit) {
 }
 
 // reliable update of the pipelined iteration: x += x_s (FP64), x_s = 0, and the last
-// direction applied (in p[1]) copied to p[0], where the next iteration reads it
+// direction applied copied to p[0], where the next chunk's launch 0 reads it.  After m
+// launches that is p_{m-2}: in p[1] with the separate finalize (m even, two buffers), in
+// rotation slot (m - 1) mod 3 of p[0], p[1], p2 with the prologue finalize (rot3), m
+// counted from the state per system.
 __global__ void __launch_bounds__(kNT)
 k_pf_reliable(int n, const CgState *__restrict__ st, double2 *__restrict__ x, float2 *__restrict__ xs,
-              const float2 *__restrict__ p1, float2 *__restrict__ p0) {
+              const float2 *__restrict__ p1, const float2 *__restrict__ p2, float2 *__restrict__ p0, int rot3) {
     const int sys = blockIdx.y;
     if (st[sys].final_) return;
     const int s = blockIdx.x * kNT + threadIdx.x;
     if (s >= n) return;
+    const int m = st[sys].iter - st[sys].iter0, slot = rot3 ? (m >= 1 ? (m - 1) % 3 : 0) : 1;
+    const float2 *src = slot == 1 ? p1 : slot == 2 ? p2 : nullptr;
     const size_t off = (size_t)sys * 4 * n;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
@@ -1011,7 +1010,7 @@ k_pf_reliable(int n, const CgState *__restrict__ st, double2 *__restrict__ x, fl
         const double2 xv = x[i];
         x[i] = make_double2(xv.x + (double)d.x, xv.y + (double)d.y);
         xs[i] = make_float2(0.f, 0.f);
-        p0[i] = p1[i];
+        if (src) p0[i] = src[i];
     }
 }
 
@@ -1199,40 +1198,42 @@ static int resident_blocks(const void *kernel, int threads) {
 // returns the number of blocks per system (= partial slots per system)
 // the pipelined FP32 iteration's launch (persistent or one block per tile), WR1 as in
 // k_eo_normal_fused; returns the blocks per system
-// XM as in k_eo_normal_fused: 1 with the separate finalize, else 0 and 2 alternating.
-// Persistent grids are sized by the variant with the fewest resident blocks, so that
-// consecutive launches leave the same number of partials.
+// XM, RM as in k_eo_normal_fused: (1, 0) with the separate finalize; with the prologue
+// finalize (0, 1) for launch 0 of a chunk, then (2, 2) and (0, 2) alternating.  Persistent
+// grids are sized by the form with the fewest resident blocks, so that consecutive
+// launches leave the same number of partials.
 template <int TT, int TX, bool WR1>
 static int launch_eo_pf_tt(const Ctx &c, int nsys, const EoArgs &A, const float *phi, bool persist,
                            const EoPtrs<float> &b, int finMode) {
     const dim3 block(eo_threads(TT, TX));
-    const int xm = finMode == 0 ? 1 : (b.idx & 1) ? 2 : 0;
+    const int form = finMode == 0 ? 0 : b.idx == 0 ? 1 : (b.idx & 1) ? 2 : 3;
+    auto pick = [&](auto persistTag) {
+        constexpr bool P = decltype(persistTag)::value;
+        using K = decltype(&k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 1, 0>);
+        const K ks[4] = {&k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 1, 0>,
+                         &k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 0, 1>,
+                         &k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 2, 2>,
+                         &k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 0, 2>};
+        return std::make_pair(ks[form], ks);
+    };
     if (persist) {
-        const void *ks[3] = {reinterpret_cast<const void *>(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 0>),
-                             reinterpret_cast<const void *>(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 1>),
-                             reinterpret_cast<const void *>(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 2>)};
-        static const int resident = std::min({resident_blocks(ks[0], block.x), resident_blocks(ks[1], block.x),
-                                              resident_blocks(ks[2], block.x)});
+        const auto kk = pick(std::true_type());
+        static const int resident = [&] {
+            int m = 1 << 30;
+            for (int i = 0; i < 4; ++i)
+                m = std::min(m, resident_blocks(reinterpret_cast<const void *>(kk.second[i]), block.x));
+            return m;
+        }();
         int g = std::min(A.tilesPerSys, std::max(resident, 8));
         if (A.xcdRemap && g >= 8) g -= g % 8;   // keeps every block's tiles on one XCD's band
-        auto go = [&](auto *k) {
-            hipLaunchKernelGGL(k, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold,
-                               b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
-                               (double2 *)nullptr);
-        };
-        if (xm == 0) go(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 0>);
-        else if (xm == 1) go(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 1>);
-        else go(&k_eo_normal_fused<float, TT, TX, true, true, false, WR1, 2>);
+        hipLaunchKernelGGL(kk.first, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold,
+                           b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
+                           (double2 *)nullptr);
         return g;
     }
-    auto go = [&](auto *k) {
-        hipLaunchKernelGGL(k, dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in,
-                           b.pold, b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
-                           (double2 *)nullptr);
-    };
-    if (xm == 0) go(&k_eo_normal_fused<float, TT, TX, true, false, false, WR1, 0>);
-    else if (xm == 1) go(&k_eo_normal_fused<float, TT, TX, true, false, false, WR1, 1>);
-    else go(&k_eo_normal_fused<float, TT, TX, true, false, false, WR1, 2>);
+    hipLaunchKernelGGL(pick(std::false_type()).first, dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi, c.cg_d,
+                       b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active,
+                       (float2 *)nullptr, (double2 *)nullptr);
     return A.tilesPerSys;
 }
 
@@ -1305,6 +1306,13 @@ static int launch_eo_ru(Ctx &c, int nsys, const double *phi, const double2 *x, c
     }
 }
 
+// direction p_k (k >= -1) of a chunk of prologue-finalized launches: the three buffers in
+// rotation, p_{-1} (the direction kept across the reliable update) in ps[0]
+static float2 *eo_pdir(const Ctx &c, int k) {
+    float2 *const rot[3] = {c.ps[0], c.ps[1], c.pfP2};
+    return rot[(k + 1) % 3];
+}
+
 // the pipelined FP32 iteration's fused kernel (finMode as in k_eo_normal_fused) with
 // tile `ti` as launch idx of a chunk (idx & 1 is the ping-pong parity); returns the block
 // partials per system
@@ -1316,17 +1324,24 @@ static int launch_eo_pf(Ctx &c, int ti, int nsys, int idx, int finMode) {
     b.idx = idx;
     b.part = finMode ? c.pfSlot[par] : c.partials;
     b.prevPart = c.pfSlot[par ^ 1];
-    b.r = par ? c.pfR1 : c.rs;
-    b.r_out = par ? c.rs : c.pfR1;
     b.Ap_in = par ? c.pfA1 : c.Aps;
     b.Ap = par ? c.Aps : c.pfA1;
-    b.pold = c.ps[par];
-    b.pnew = c.ps[par ^ 1];
     b.xs = c.xs;
-    // read r, Ap, p, x_s and write r, p, Ap, x_s (half-lattice FP32 vectors) and read phi;
-    // an even launch with the finalize in the prologue leaves x_s alone
-    const int vecs = (finMode != 0 && !(idx & 1)) ? 6 : 8;
-    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (vecs * 16 + 4));
+    int vecs;   // half-lattice FP32 vectors read or written, 16 B per site of the full lattice
+    if (finMode == 0) {   // r and p ping-pong; read r, Ap, p, x_s and write r, p, Ap, x_s
+        b.r = par ? c.pfR1 : c.rs;
+        b.r_out = par ? c.rs : c.pfR1;
+        b.pold = c.ps[par];
+        b.pnew = c.ps[par ^ 1];
+        vecs = 8;
+    } else {   // direction p_k in eo_pdir(c, k); launch 0 reads r, the others p_{idx-2}
+        b.r = idx == 0 ? c.rs : eo_pdir(c, idx - 2);
+        b.r_out = c.pfR1;   // not written
+        b.pold = eo_pdir(c, idx - 1);
+        b.pnew = eo_pdir(c, idx);
+        vecs = (idx & 1) ? 7 : 5;   // read p_{idx-2} (or r), Ap, p and write p, Ap; x_s in odd launches
+    }
+    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (vecs * 16 + 4));  // and read phi
     const bool ps = t.persist;
     int nb = 0;
     switch (t.TT * 1000 + t.TX) {
@@ -1367,7 +1382,7 @@ int eo_autotune(Ctx &c) {
         return 1e3 * ms / reps;
     };
     char line[96];
-    const bool pfOk = c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfSlot[0] && c.pfSlot[1];
+    const bool pfOk = c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfP2 && c.pfSlot[0] && c.pfSlot[1];
     // measured times: kNumEo64 FP64 tiles, then kNumEo32 x {separate, prologue} FP32
     // pipelined iterations (1e30 where a tile does not fit); reused from an earlier
     // context of the same shape when there is one
@@ -1500,7 +1515,7 @@ static void eo_learn_stop_scale(Ctx &c, int nsys) {
 
 // the pipelined FP32 solve with FP64 reliable updates runs where its buffers exist
 static bool eo_mixed(const Ctx &c) {
-    return c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfR1 && c.pfA1 &&
+    return c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfP2 &&
            c.pfSlot[0] && c.pfSlot[1];
 }
 
@@ -1586,7 +1601,7 @@ static int eo_mixed_chunk(Ctx &c, const double *phi, int nsys, int n, bool graph
     }
     if (c.eoFinFused) launch_fin_pf_chunk(c, nsys, n);
     hipLaunchKernelGGL(k_pf_reliable, gH, dim3(kNT), 0, c.stream, Vh, (const CgState *)c.cg_d, c.x, c.xs,
-                       (const float2 *)c.ps[1], c.ps[0]);
+                       (const float2 *)c.ps[1], (const float2 *)c.pfP2, c.ps[0], c.eoFinFused ? 1 : 0);
     const int nbR = launch_eo_ru(c, nsys, phi, c.x, B.b, c.r, c.rs, partRu, partRu2);
     hipLaunchKernelGGL(k_fin_eo_pf_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partRu,
                        (const double2 *)partRu2, nbR, c.cg_d, c.active);

@@ -52,7 +52,7 @@ struct CgState {
     double rrRel;      // FP32 loops: recursive |r|^2 when the recursion was last anchored (start, reliable update)
     int nRu;           // pipelined even-odd solve: reliable updates so far
     int ruIter[7];     // pipelined even-odd solve: iteration count at each of the first seven
-    double2 xAB;       // pipelined even-odd solve: alpha, beta of the last launch that deferred its x_s update
+    double2 ab[2];     // pipelined even-odd solve (prologue finalize): alpha, beta of launch idx in ab[idx & 1]
 };
 constexpr int kMaxRounds = 7;
 
@@ -151,6 +151,7 @@ struct Ctx {
     float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *Aps = nullptr, *xs = nullptr;
     // pipelined even-odd FP32 iteration: the second buffers of r and Ap ([nSys][4][V/2])
     float2 *pfR1 = nullptr, *pfA1 = nullptr;
+    float2 *pfP2 = nullptr;   // third direction buffer of the prologue-finalized iteration (rotation of three)
     double2 *pfSlot[2] = {nullptr, nullptr};   // block partials of prologue-finalized launches
     bool eoFinFused = false;              // pipelined iteration finalizes in the next launch's prologue
     int eoPfBlocks = 0;                   // blocks per system of those launches (same tile throughout)
