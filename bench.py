@@ -43,9 +43,10 @@ HBM_PEAK_GBS = 8000.0
 # for a 1 GiB device copy (FETCH_SIZE 0.5 KiB, WRITE_SIZE 1.0 KiB per KiB moved); the
 # summaries with their commands are profiles/hbm_traffic_mixed_cg_1024x1024.md (fused
 # normal operator) and profiles/hbm_traffic_eo_mixed_cg_1024x1024.md (pipelined
-# even-odd iteration).  Keyed by (kernel, precision, lattice extent); others report null.
+# even-odd iteration: the mean of its two alternating launch forms, with and without the
+# x_s update; its r is rebuilt from the directions).  Keyed by (kernel, precision, lattice extent); others report null.
 MEASURED_TRAFFIC = {("cg_normal_op", "fp32", 1024): 140720905,
-                    ("cg_normal_op-eo", "fp32", 1024): 142437840}
+                    ("cg_normal_op-eo", "fp32", 1024): 108734822}
 
 WORKLOAD = dict(Nt=1024, Nx=1024, m2=-0.4, lam=1.0, g=0.5, M=0.05, r=1.0, cutFraction=0.5,
                 driftMode=0, epsBar=0.005, adaptive=0, seed=1234, cgTol=1e-10, cgTolMode=1,
