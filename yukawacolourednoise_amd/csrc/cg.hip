@@ -1114,9 +1114,9 @@ int mp_alloc(Ctx &c) {
     }
     YCN_HIP(hipMalloc(&c.phis, sizeof(float) * (size_t)c.nRep * c.V));
     YCN_HIP(hipMemset(c.phis, 0, sizeof(float) * (size_t)c.nRep * c.V));
-    if (c.p.evenOdd) {   // second r and Ap buffers and third p buffer of the pipelined even-odd iteration (half lattice)
+    if (c.p.evenOdd) {   // second Ap buffer and third direction buffer of the pipelined even-odd iteration (half lattice)
         const size_t h = (size_t)c.nSys * 4 * (c.V / 2);
-        float2 **pf[] = {&c.pfR1, &c.pfA1, &c.pfP2};
+        float2 **pf[] = {&c.pfA1, &c.pfP2};
         for (float2 **b : pf) {
             YCN_HIP(hipMalloc(b, sizeof(float2) * h));
             YCN_HIP(hipMemset(*b, 0, sizeof(float2) * h));
@@ -1128,7 +1128,7 @@ int mp_alloc(Ctx &c) {
 }
 
 void mp_free(Ctx &c) {
-    float2 **bufs[] = {&c.rs, &c.ps[0], &c.ps[1], &c.Aps, &c.xs, &c.pfR1, &c.pfA1, &c.pfP2};
+    float2 **bufs[] = {&c.rs, &c.ps[0], &c.ps[1], &c.Aps, &c.xs, &c.pfA1, &c.pfP2};
     for (float2 **b : bufs)
         if (*b) { (void)hipFree(*b); *b = nullptr; }
     if (c.phis) { (void)hipFree(c.phis); c.phis = nullptr; }

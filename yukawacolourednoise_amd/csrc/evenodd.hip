@@ -559,10 +559,11 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // rho = |r|^2, mu = <p, Ap>, nu = |Ap|^2, alpha_k = rho_k / mu_k and <r_k, Ap_k> = mu_k,
 // rho_{k+1} = alpha_k^2 nu_k - rho_k, so beta_{k+1} is known before r_{k+1} exists and
 // the x, r updates move into the next operator application.  Stage 0 then forms, on its
-// even sites, r' = r - alpha Ap, p' = r' + beta p and (tile only) x_s += alpha p; the block
+// even sites, r' = r - alpha Ap, p' = r' + beta p and (tile only) the x_s update; the block
 // partials are (mu, nu) and (rho', 0), rho' the exact |r'|^2 that re-anchors the
-// recurrence every iteration.  r, Ap and p ping-pong between two buffers: blocks read the
-// old ones on their halo while other blocks write the new ones on their tiles.
+// recurrence every iteration.  Ap ping-pongs between two buffers and the directions rotate
+// through three: blocks read the old ones on their halo while other blocks write the new
+// ones on their tiles.
 // PERSIST = true: the grid holds only as many blocks as are resident; each block walks
 // the tiles blockIdx.x, blockIdx.x + gridDim.x, ... and loads the next tile's stage-0
 // data into registers while it runs stages 1-4 of the current one, so global loads
@@ -582,38 +583,37 @@ constexpr int eo_threads(int tt, int tx) { return (tt * tx / 2 < kWave) ? kWave 
 // partials are |res|^2 (part) and |r|^2 (part2).  Systems already final are skipped.
 // WR1 = true: Wilson parameter r = 1 (hop_flavour_r1), chosen at launch when A.r == 1 (every
 // precision and mode).
-// With the finalize in the prologue (finMode 1, 2) the iteration keeps neither r nor x_s in
-// every launch.  Block 0 of each launch records the alpha, beta it used in the state
-// (ab[idx & 1]) for the next launch, and the directions rotate through three buffers.
+// The pipelined iteration keeps neither r nor x_s in every launch.  Block 0 of each launch
+// records the alpha, beta it used (from the prologue, or from the state that the separate
+// finalize wrote) in the state as ab[idx & 1] for the next launch, and the directions
+// rotate through three buffers.
 // RM (PF only), where r_{idx-1} comes from:
-//   0: read from r, and r' written to r_out (finMode 0);
-//   1: read from r, nothing written (launch 0 of a chunk, after the setup or a reliable
-//      update has put the residual there);
+//   1: read from r (launch 0 of a chunk, after the setup or a reliable update has put the
+//      residual there);
 //   2: rebuilt as r_{idx-1} = p_{idx-1} - beta_{idx-1} p_{idx-2}, the relation that formed
 //      p_{idx-1} (same FP32 beta); r then carries p_{idx-2}, read in place of the residual.
-//      The rebuild's rounding error is a few eps |p| against the eps |r| of the stored r
+//      The rebuild's rounding error is a few eps |p| against the eps |r| of a stored r
 //      (|p| is a few |r| in CG), and the FP64 reliable updates bound its drift as they do
 //      the recursion's.
 // XM (PF only), the x_s update:
-//   1: x_s += alpha p_old every launch (finMode 0);
 //   0: none (an even launch);
 //   2: the odd launch after it adds both terms, alpha_e p_{idx-2} + alpha p_{idx-1}, with
 //      p_{idx-2} the input RM = 2 reads anyway.
 // Pairs never straddle a chunk (chunks have an even number of launches), and a system
-// pauses only in an even launch (after whole pairs), except at the iteration cap: an odd
-// launch that pauses adds the deferred term on its tiles before it returns.  Per even site
-// and launch: 264 B with RM = 0, XM = 1; 168 B (even) and 232 B (odd) with RM = 1, 2.
+// pauses only after an odd launch (after whole pairs), except at the iteration cap; the
+// reliable update's k_pf_reliable then adds the deferred term.  Per even site and launch:
+// 168 B (even) and 232 B (odd).  XM and RM do not apply to the FP64 forms (PF = false).
 template <typename R, int TT, int TX, bool PF, bool PERSIST = false, bool RU = false, bool WR1 = false, int XM = 1,
           int RM = 0>
 __global__ void __launch_bounds__(eo_threads(TT, TX))
-k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
-                  const typename Vec2<R>::T *__restrict__ r, typename Vec2<R>::T *__restrict__ r_out,
+k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st, const typename Vec2<R>::T *__restrict__ r,
                   const typename Vec2<R>::T *__restrict__ Ap_in, const typename Vec2<R>::T *__restrict__ pold,
                   typename Vec2<R>::T *__restrict__ pnew, typename Vec2<R>::T *__restrict__ Ap,
                   typename Vec2<R>::T *__restrict__ xs, double2 *__restrict__ part, int finMode, int idx,
                   const double2 *__restrict__ prevPart, int *active, float2 *__restrict__ out32,
                   double2 *__restrict__ part2) {
     static_assert(!(PF && RU), "the reliable-update residual runs in FP64 only");
+    static_assert(!PF || ((XM == 0 || XM == 2) && (RM == 1 || RM == 2)), "pipelined forms: XM 0, 2 and RM 1, 2");
     static_assert(XM != 2 || RM == 2, "the pair's x_s update reads p_{idx-2}");
     using V2 = typename Vec2<R>::T;
     constexpr int H = TT + 8, K = (TX + 8) / 2, NT = eo_threads(TT, TX);
@@ -639,10 +639,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         pnew += off;
     }
     Ap += off;
-    if (PF) {
-        r_out += off;
-        xs += off;
-    }
+    if (PF) xs += off;
     if (PF || RU) Ap_in += off;
     if (RU) out32 += off;
     R beta = RU ? (R)0 : (R)st[sys].beta;
@@ -719,9 +716,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
                     if (inner && !RU) {
                         at(pnew, i) = pv;
                         if (PF) {
-                            if (RM == 0) at(r_out, i) = a;
                             const V2 xa = G.xa[u][c], p2 = G.ra[u][c];
-                            if (XM == 1) at(xs, i) = mk2<R>(xa.x + alpha * b.x, xa.y + alpha * b.y);
                             if (XM == 2)
                                 at(xs, i) = mk2<R>(xa.x + alphaPrev * p2.x + alpha * b.x,
                                                    xa.y + alphaPrev * p2.y + alpha * b.y);
@@ -891,43 +886,18 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         return false;
     };
 
-    // once alpha is fixed, the prologue forms record this launch's alpha, beta for the next
+    // once alpha is fixed, a pipelined launch records its alpha, beta for the next one
     auto record_ab = [&]() {
-        if constexpr (PF && XM != 1) {
+        if constexpr (PF) {
             if (blockIdx.x == 0 && threadIdx.x == 0) st[sys].ab[idx & 1] = make_double2(alphaD, betaD);
         }
     };
-    // XM = 2, a launch that pauses (at the iteration cap): the deferred alpha_e p_{idx-2}
-    // on the block's tiles (p_{idx-2} in r)
-    auto x_deferred = [&]() {
-        if constexpr (PF && XM == 2) {
-            if (alphaPrev == (R)0) return;
-            const R ae = alphaPrev;
-            for (int t = blockIdx.x; t < (PERSIST ? nTiles : (int)blockIdx.x + 1); t += (int)gridDim.x) {
-                int a0, b0;
-                origin(t, a0, b0);
-                for (int w = threadIdx.x; w < TT * (TX / 2); w += NT) {
-                    const Site q = site_of(w, 0, 0);
-                    const unsigned j = (unsigned)((a0 - 4 + q.row) * Nh + ((b0 - 4 + q.col) >> 1));
-#pragma unroll
-                    for (int c = 0; c < 4; ++c) {
-                        const V2 pn = at(r, c * Vh + j), xv = at(xs, c * Vh + j);
-                        at(xs, c * Vh + j) = mk2<R>(xv.x + ae * pn.x, xv.y + ae * pn.y);
-                    }
-                }
-            }
-        }
-    };
-
     Regs G;
     int t0, x0;
     if (!PERSIST) {
         origin(blockIdx.x, t0, x0);
         load0(t0, x0, G);
-        if (!prologue()) {
-            x_deferred();
-            return;
-        }
+        if (!prologue()) return;
         record_ab();
         compute0(G);
         __syncthreads();
@@ -936,10 +906,7 @@ k_eo_normal_fused(EoArgs A, const R *__restrict__ phi, CgState *st,
         int t = blockIdx.x;
         origin(t, t0, x0);
         load0(t0, x0, G);
-        if (!prologue()) {
-            x_deferred();
-            return;
-        }
+        if (!prologue()) return;
         record_ab();
         while (t < nTiles) {
             compute0(G);
@@ -990,27 +957,32 @@ static void launch_fin_pf(Ctx &c, int nsys, int nb, int commit) {
 
 // reliable update of the pipelined iteration: x += x_s (FP64), x_s = 0, and the last
 // direction applied copied to p[0], where the next chunk's launch 0 reads it.  After m
-// launches that is p_{m-2}: in p[1] with the separate finalize (m even, two buffers), in
-// rotation slot (m - 1) mod 3 of p[0], p[1], p2 with the prologue finalize (rot3), m
-// counted from the state per system.
+// launches (m counted from the state per system) that is p_{m-2}, in rotation slot
+// (m - 1) mod 3 of p[0], p[1], p2.  For odd m (a stop at the iteration cap after an even
+// launch) the even launch's deferred term alpha_{m-1} p_{m-2} is added too, with the alpha
+// it recorded in ab.
 __global__ void __launch_bounds__(kNT)
 k_pf_reliable(int n, const CgState *__restrict__ st, double2 *__restrict__ x, float2 *__restrict__ xs,
-              const float2 *__restrict__ p1, const float2 *__restrict__ p2, float2 *__restrict__ p0, int rot3) {
+              const float2 *__restrict__ p1, const float2 *__restrict__ p2, float2 *__restrict__ p0) {
     const int sys = blockIdx.y;
     if (st[sys].final_) return;
     const int s = blockIdx.x * kNT + threadIdx.x;
     if (s >= n) return;
-    const int m = st[sys].iter - st[sys].iter0, slot = rot3 ? (m >= 1 ? (m - 1) % 3 : 0) : 1;
-    const float2 *src = slot == 1 ? p1 : slot == 2 ? p2 : nullptr;
+    const int m = st[sys].iter - st[sys].iter0, slot = m >= 1 ? (m - 1) % 3 : 0;
+    const float2 *src = slot == 1 ? p1 : slot == 2 ? p2 : p0;
+    const double a = (m & 1) ? st[sys].ab[(m - 1) & 1].x : 0.0;
     const size_t off = (size_t)sys * 4 * n;
 #pragma unroll
     for (int c = 0; c < 4; ++c) {
         const size_t i = off + (size_t)c * n + s;
         const float2 d = xs[i];
         const double2 xv = x[i];
-        x[i] = make_double2(xv.x + (double)d.x, xv.y + (double)d.y);
+        double2 xn = make_double2(xv.x + (double)d.x, xv.y + (double)d.y);
+        const float2 pk = (slot != 0 || (m & 1)) ? src[i] : make_float2(0.f, 0.f);
+        if (m & 1) xn = make_double2(xn.x + a * (double)pk.x, xn.y + a * (double)pk.y);
+        x[i] = xn;
         xs[i] = make_float2(0.f, 0.f);
-        if (src) p0[i] = src[i];
+        if (slot != 0) p0[i] = pk;
     }
 }
 
@@ -1171,12 +1143,11 @@ static bool eo_shape_ok(const Ctx &c) {
 
 bool eo_available(const Ctx &c) { return eo_shape_ok(c) && c.tileEo64 >= 0; }
 
-// buffers of one fused launch (PF: the pipelined iteration's ping-pong pairs and x_s)
+// buffers of one fused launch (PF: the pipelined iteration's Ap pair, directions and x_s)
 template <typename R>
 struct EoPtrs {
     using V2 = typename Vec2<R>::T;
     const V2 *r = nullptr;
-    V2 *r_out = nullptr;
     const V2 *Ap_in = nullptr;
     const V2 *pold = nullptr;
     V2 *pnew = nullptr, *Ap = nullptr, *xs = nullptr;
@@ -1198,20 +1169,18 @@ static int resident_blocks(const void *kernel, int threads) {
 // returns the number of blocks per system (= partial slots per system)
 // the pipelined FP32 iteration's launch (persistent or one block per tile), WR1 as in
 // k_eo_normal_fused; returns the blocks per system
-// XM, RM as in k_eo_normal_fused: (1, 0) with the separate finalize; with the prologue
-// finalize (0, 1) for launch 0 of a chunk, then (2, 2) and (0, 2) alternating.  Persistent
-// grids are sized by the form with the fewest resident blocks, so that consecutive
-// launches leave the same number of partials.
+// XM, RM as in k_eo_normal_fused: (0, 1) for launch 0 of a chunk, then (2, 2) and (0, 2)
+// alternating.  Persistent grids are sized by the form with the fewest resident blocks,
+// so that consecutive launches leave the same number of partials.
 template <int TT, int TX, bool WR1>
 static int launch_eo_pf_tt(const Ctx &c, int nsys, const EoArgs &A, const float *phi, bool persist,
                            const EoPtrs<float> &b, int finMode) {
     const dim3 block(eo_threads(TT, TX));
-    const int form = finMode == 0 ? 0 : b.idx == 0 ? 1 : (b.idx & 1) ? 2 : 3;
+    const int form = b.idx == 0 ? 0 : (b.idx & 1) ? 1 : 2;
     auto pick = [&](auto persistTag) {
         constexpr bool P = decltype(persistTag)::value;
-        using K = decltype(&k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 1, 0>);
-        const K ks[4] = {&k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 1, 0>,
-                         &k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 0, 1>,
+        using K = decltype(&k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 0, 1>);
+        const K ks[3] = {&k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 0, 1>,
                          &k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 2, 2>,
                          &k_eo_normal_fused<float, TT, TX, true, P, false, WR1, 0, 2>};
         return std::make_pair(ks[form], ks);
@@ -1220,19 +1189,19 @@ static int launch_eo_pf_tt(const Ctx &c, int nsys, const EoArgs &A, const float 
         const auto kk = pick(std::true_type());
         static const int resident = [&] {
             int m = 1 << 30;
-            for (int i = 0; i < 4; ++i)
+            for (int i = 0; i < 3; ++i)
                 m = std::min(m, resident_blocks(reinterpret_cast<const void *>(kk.second[i]), block.x));
             return m;
         }();
         int g = std::min(A.tilesPerSys, std::max(resident, 8));
         if (A.xcdRemap && g >= 8) g -= g % 8;   // keeps every block's tiles on one XCD's band
-        hipLaunchKernelGGL(kk.first, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold,
+        hipLaunchKernelGGL(kk.first, dim3(g, nsys), block, 0, c.stream, A, phi, c.cg_d, b.r, b.Ap_in, b.pold,
                            b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active, (float2 *)nullptr,
                            (double2 *)nullptr);
         return g;
     }
     hipLaunchKernelGGL(pick(std::false_type()).first, dim3(A.tilesPerSys, nsys), block, 0, c.stream, A, phi, c.cg_d,
-                       b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active,
+                       b.r, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, b.part, finMode, b.idx, b.prevPart, c.active,
                        (float2 *)nullptr, (double2 *)nullptr);
     return A.tilesPerSys;
 }
@@ -1249,11 +1218,11 @@ static int launch_eo_tt(const Ctx &c, int nsys, const EoArgs &A, const R *phi, b
     }
     if (A.r == 1.0)
         hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false, false, false, true>), dim3(A.tilesPerSys, nsys), block,
-                           0, c.stream, A, phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0,
+                           0, c.stream, A, phi, c.cg_d, b.r, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0,
                            0, (const double2 *)nullptr, (int *)nullptr, (float2 *)nullptr, (double2 *)nullptr);
     else
         hipLaunchKernelGGL((k_eo_normal_fused<R, TT, TX, false>), dim3(A.tilesPerSys, nsys), block, 0, c.stream, A,
-                           phi, c.cg_d, b.r, b.r_out, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0, 0,
+                           phi, c.cg_d, b.r, b.Ap_in, b.pold, b.pnew, b.Ap, b.xs, c.partials, 0, 0,
                            (const double2 *)nullptr, (int *)nullptr, (float2 *)nullptr, (double2 *)nullptr);
     return A.tilesPerSys;
 }
@@ -1288,9 +1257,9 @@ static int launch_eo_ru_tt(Ctx &c, int nsys, const EoArgs &A, const double *phi,
                            double2 *r, float2 *r32, double2 *partRes, double2 *partN) {
     auto *k = A.r == 1.0 ? &k_eo_normal_fused<double, TT, TX, false, false, true, true>
                          : &k_eo_normal_fused<double, TT, TX, false, false, true, false>;
-    hipLaunchKernelGGL(k, dim3(A.tilesPerSys, nsys), dim3(eo_threads(TT, TX)), 0, c.stream, A, phi, c.cg_d, x,
-                       (double2 *)nullptr, b, (const double2 *)nullptr, (double2 *)nullptr, r, (double2 *)nullptr,
-                       partRes, 0, 0, (const double2 *)nullptr, (int *)nullptr, r32, partN);
+    hipLaunchKernelGGL(k, dim3(A.tilesPerSys, nsys), dim3(eo_threads(TT, TX)), 0, c.stream, A, phi, c.cg_d, x, b,
+                       (const double2 *)nullptr, (double2 *)nullptr, r, (double2 *)nullptr, partRes, 0, 0,
+                       (const double2 *)nullptr, (int *)nullptr, r32, partN);
     return A.tilesPerSys;
 }
 static int launch_eo_ru(Ctx &c, int nsys, const double *phi, const double2 *x, const double2 *b, double2 *r,
@@ -1314,8 +1283,8 @@ static float2 *eo_pdir(const Ctx &c, int k) {
 }
 
 // the pipelined FP32 iteration's fused kernel (finMode as in k_eo_normal_fused) with
-// tile `ti` as launch idx of a chunk (idx & 1 is the ping-pong parity); returns the block
-// partials per system
+// tile `ti` as launch idx of a chunk (idx & 1: the Ap and partial-slot parity, the
+// directions in rotation); returns the block partials per system
 static int launch_eo_pf(Ctx &c, int ti, int nsys, int idx, int finMode) {
     const EoTileCfg t = kEoTiles32[ti];
     const EoArgs A = eo_args(c, t.TT, t.TX);
@@ -1327,21 +1296,14 @@ static int launch_eo_pf(Ctx &c, int ti, int nsys, int idx, int finMode) {
     b.Ap_in = par ? c.pfA1 : c.Aps;
     b.Ap = par ? c.Aps : c.pfA1;
     b.xs = c.xs;
-    int vecs;   // half-lattice FP32 vectors read or written, 16 B per site of the full lattice
-    if (finMode == 0) {   // r and p ping-pong; read r, Ap, p, x_s and write r, p, Ap, x_s
-        b.r = par ? c.pfR1 : c.rs;
-        b.r_out = par ? c.rs : c.pfR1;
-        b.pold = c.ps[par];
-        b.pnew = c.ps[par ^ 1];
-        vecs = 8;
-    } else {   // direction p_k in eo_pdir(c, k); launch 0 reads r, the others p_{idx-2}
-        b.r = idx == 0 ? c.rs : eo_pdir(c, idx - 2);
-        b.r_out = c.pfR1;   // not written
-        b.pold = eo_pdir(c, idx - 1);
-        b.pnew = eo_pdir(c, idx);
-        vecs = (idx & 1) ? 7 : 5;   // read p_{idx-2} (or r), Ap, p and write p, Ap; x_s in odd launches
-    }
-    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (vecs * 16 + 4));  // and read phi
+    // direction p_k in eo_pdir(c, k); launch 0 reads r, the others p_{idx-2}
+    b.r = idx == 0 ? c.rs : eo_pdir(c, idx - 2);
+    b.pold = eo_pdir(c, idx - 1);
+    b.pnew = eo_pdir(c, idx);
+    // half-lattice FP32 vectors (16 B per site of the full lattice): read p_{idx-2} (or r),
+    // Ap, p and write p, Ap, and x_s read and written in odd launches; and read phi
+    const int vecs = (idx & 1) ? 7 : 5;
+    prof_before(c, PK_NORMAL_OP, (double)nsys * c.V * (vecs * 16 + 4));
     const bool ps = t.persist;
     int nb = 0;
     switch (t.TT * 1000 + t.TX) {
@@ -1382,7 +1344,7 @@ int eo_autotune(Ctx &c) {
         return 1e3 * ms / reps;
     };
     char line[96];
-    const bool pfOk = c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfP2 && c.pfSlot[0] && c.pfSlot[1];
+    const bool pfOk = c.rs && c.phis && c.pfA1 && c.pfP2 && c.pfSlot[0] && c.pfSlot[1];
     // measured times: kNumEo64 FP64 tiles, then kNumEo32 x {separate, prologue} FP32
     // pipelined iterations (1e30 where a tile does not fit); reused from an earlier
     // context of the same shape when there is one
@@ -1448,7 +1410,7 @@ int eo_autotune(Ctx &c) {
     }
     if (const char *f = std::getenv("YCN_TILE_EO32")) {
         const int i = std::atoi(f);
-        if (c.rs && c.phis && c.pfR1 && c.pfSlot[0] && i >= 0 && i < kNumEo32 && eo_fits(c, kEoTiles32[i])) {
+        if (c.rs && c.phis && c.pfA1 && c.pfP2 && c.pfSlot[0] && i >= 0 && i < kNumEo32 && eo_fits(c, kEoTiles32[i])) {
             c.tileEo32 = i;
             c.eoFinFused = allowFused && (!allowSep || us32[i][1] < us32[i][0]);
         }
@@ -1515,7 +1477,7 @@ static void eo_learn_stop_scale(Ctx &c, int nsys) {
 
 // the pipelined FP32 solve with FP64 reliable updates runs where its buffers exist
 static bool eo_mixed(const Ctx &c) {
-    return c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfR1 && c.pfA1 && c.pfP2 &&
+    return c.p.cgPrecision == YCN_CG_MIXED && c.tileEo32 >= 0 && c.rs && c.phis && c.pfA1 && c.pfP2 &&
            c.pfSlot[0] && c.pfSlot[1];
 }
 
@@ -1601,7 +1563,7 @@ static int eo_mixed_chunk(Ctx &c, const double *phi, int nsys, int n, bool graph
     }
     if (c.eoFinFused) launch_fin_pf_chunk(c, nsys, n);
     hipLaunchKernelGGL(k_pf_reliable, gH, dim3(kNT), 0, c.stream, Vh, (const CgState *)c.cg_d, c.x, c.xs,
-                       (const float2 *)c.ps[1], (const float2 *)c.pfP2, c.ps[0], c.eoFinFused ? 1 : 0);
+                       (const float2 *)c.ps[1], (const float2 *)c.pfP2, c.ps[0]);
     const int nbR = launch_eo_ru(c, nsys, phi, c.x, B.b, c.r, c.rs, partRu, partRu2);
     hipLaunchKernelGGL(k_fin_eo_pf_true, dim3(nsys), dim3(kNT), 0, c.stream, (const double2 *)partRu,
                        (const double2 *)partRu2, nbR, c.cg_d, c.active);

@@ -67,7 +67,7 @@ std::mutex g_tuneMutex;
 std::vector<std::pair<TuneKey, std::vector<double>>> g_tune;
 TuneKey tune_key(int kind, const Ctx &c) {
     return {kind, c.p.device, c.p.Nt, c.p.Nx, c.nSys, c.p.r == 1.0, c.rs != nullptr,
-            c.pfR1 != nullptr && c.pfA1 != nullptr && c.pfSlot[0] != nullptr && c.pfSlot[1] != nullptr};
+            c.pfA1 != nullptr && c.pfP2 != nullptr && c.pfSlot[0] != nullptr && c.pfSlot[1] != nullptr};
 }
 }  // namespace
 

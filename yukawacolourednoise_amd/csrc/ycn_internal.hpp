@@ -149,9 +149,9 @@ struct Ctx {
 
     // mixed precision: [nSys][4][V] float2
     float2 *rs = nullptr, *ps[2] = {nullptr, nullptr}, *Aps = nullptr, *xs = nullptr;
-    // pipelined even-odd FP32 iteration: the second buffers of r and Ap ([nSys][4][V/2])
-    float2 *pfR1 = nullptr, *pfA1 = nullptr;
-    float2 *pfP2 = nullptr;   // third direction buffer of the prologue-finalized iteration (rotation of three)
+    // pipelined even-odd FP32 iteration ([nSys][4][V/2]): the second Ap buffer, and the third
+    // direction buffer (the directions rotate through ps[0], ps[1], pfP2)
+    float2 *pfA1 = nullptr, *pfP2 = nullptr;
     double2 *pfSlot[2] = {nullptr, nullptr};   // block partials of prologue-finalized launches
     bool eoFinFused = false;              // pipelined iteration finalizes in the next launch's prologue
     int eoPfBlocks = 0;                   // blocks per system of those launches (same tile throughout)
