@@ -253,8 +253,8 @@ int ycn_precond_train(ycn_ctx *ctx, uint64_t step0, int nsteps, int pairsPerSolv
  * between begin and end every launch of the named kernel is bracketed by events
  * (iteration chunks are then launched one kernel at a time, not as graph replays);
  * end returns the summed duration and count of the launches that did work (launches
- * shorter than a quarter of the median exited at once for converged systems and are
- * left out) and the algorithmic bytes moved per launch (averaged over the launches: the
+ * shorter than a quarter of the 90th percentile exited at once for converged systems and
+ * are left out) and the algorithmic bytes moved per launch (averaged over the launches: the
  * pipelined iteration's launches alternate between two traffic figures).  The pipelined even-odd FP32
  * iteration with the finalize in the next launch's prologue is the exception: its
  * chunks hold nothing but back-to-back launches of that kernel, so one event pair

@@ -756,8 +756,9 @@ int ycn_profile_end(ycn_ctx *ctx, double *total_ms, int64_t *launches, double *b
     Ctx &c = *ctx;
     YCN_HIP(hipStreamSynchronize(c.stream));
     // Single launches: those replayed after every system converged exit at once (a few us
-    // against tens for a working launch); those shorter than a quarter of the median of
-    // the single launches are not counted.  Chunks: the whole bracket over the launches
+    // against tens for a working launch); those shorter than a quarter of the 90th
+    // percentile of the single launches are not counted (a percentile, not the median: in
+    // batched solves more than half of the launches can be ones that exit at once).  Chunks: the whole bracket over the launches
     // that did an iteration (launches after a pause count as time, not as launches).
     std::vector<float> d(c.evCount), single;
     for (int i = 0; i < c.evCount; ++i) {
@@ -765,7 +766,7 @@ int ycn_profile_end(ycn_ctx *ctx, double *total_ms, int64_t *launches, double *b
         if (c.evWork[i] < 0) single.push_back(d[i]);
     }
     std::sort(single.begin(), single.end());
-    const float cut = single.empty() ? 0.f : 0.25f * single[single.size() / 2];
+    const float cut = single.empty() ? 0.f : 0.25f * single[(single.size() * 9) / 10];
     double tot = 0;
     int64_t n = 0;
     for (int i = 0; i < c.evCount; ++i) {
