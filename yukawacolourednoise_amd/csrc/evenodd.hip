@@ -1275,8 +1275,8 @@ static int launch_eo_ru(Ctx &c, int nsys, const double *phi, const double2 *x, c
     }
 }
 
-// direction p_k (k >= -1) of a chunk of prologue-finalized launches: the three buffers in
-// rotation, p_{-1} (the direction kept across the reliable update) in ps[0]
+// direction p_k (k >= -1) of a chunk of pipelined launches: the three buffers in rotation,
+// p_{-1} (the direction kept across the reliable update) in ps[0]
 static float2 *eo_pdir(const Ctx &c, int k) {
     float2 *const rot[3] = {c.ps[0], c.ps[1], c.pfP2};
     return rot[(k + 1) % 3];
@@ -1550,7 +1550,7 @@ static int eo_mixed_chunk(Ctx &c, const double *phi, int nsys, int n, bool graph
     const EoBufs B = eo_bufs(c, nsys);
     // reliable-update partials: at most V / 64 blocks per system each (smallest FP64 tile)
     double2 *partRu = c.partials, *partRu2 = c.partials + (size_t)nsys * ((c.V + 63) / 64);
-    if (n & 1) {   // the prologue-finalized launches defer x_s updates in pairs
+    if (n & 1) {   // the pipelined launches defer x_s updates in pairs
         c.err = "internal: a chunk of the pipelined iteration has an odd number of launches";
         return YCN_ERR_INVALID;
     }
